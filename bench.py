@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark of the SSR hot path on one MI355X.
+
+Workload (the constitutive + assembly microbenchmark): 3D box 40x20x20 m,
+100x50x40 hexes x 6 P2 tetrahedra = 1.2M elements x 11 integration points =
+13.2M points.  Trial strains N(0, 1e-3^2) per Voigt component with a seeded
+30% subset scaled x5; c0 = 15 kPa, phi = 30 deg, psi = 0, E = 20 MPa,
+nu = 0.3, lambda ~ U[1,2] per point, Davis B.
+
+One step = one Mohr-Coulomb stress + consistent-tangent evaluation (S and DS)
+over all 13.2M points through the MEX-layout kernel (point-major E/S/DS, five
+per-point material arrays), inputs resident in HBM.  Throughput is reported in
+Gpt/s (1e9 points/s).  The Davis reduction producing c_bar/sin_phi runs once
+before the timed region (it is a separate, cheaper kernel; its own rate is
+reported as an extra field).
+
+Roofline: 424 algorithmic bytes per point (E 48 + 5 x 8 material + S 48 +
+DS 288), HBM peak 8 TB/s.  CPU baseline: the reference's own per-point kernel
+compiled from its sources (oracle/_ref) with OpenMP, timed on a bounded
+sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_HEX = (100, 50, 40)
+N_Q = 11
+BYTES_PER_POINT = 424
+HBM_PEAK_GBS = 8000.0
+METRIC = "SSR time-to-FoS (|ΔFoS|≤1e-4) chosen vs plain GPU; constitutive Gpt/s"
+
+
+def cpu_baseline(d, cb, sp, budget_s=12.0, sample=1_320_000):
+    """Time the reference kernel (or the oracle restatement) on a bounded
+    sample of the same inputs with all host OpenMP threads."""
+    from oracle import oracle as O
+
+    O.build()
+    kind = "reference" if O.ref_available() else "port"
+    n = min(sample, d["E"].shape[0])
+    E = np.ascontiguousarray(d["E"][:n].T)
+    mats = (cb[:n], sp[:n], d["shear"][:n], d["bulk"][:n], d["lame"][:n])
+    O.mc3d(E[:, :1000], *[m[:1000] for m in mats], use_ref=(kind == "reference"))
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.mc3d(E, *mats, use_ref=(kind == "reference"))
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 200:
+            break
+    rate = reps * n / el / 1e9
+    return {
+        "value": rate,
+        "unit": "Gpt/s",
+        "cores": O.num_threads(),
+        "kind": kind,
+        "sample": f"{reps} x {n} points (first {n} of the {d['E'].shape[0]}-point workload), "
+                  f"S+DS, OpenMP static schedule, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the hot path is measured on one "
+              "MI355X; the multi-GPU decomposition is a design note, not a launch mode)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from slope_stability_octave_amd import _lib
+    from slope_stability_octave_amd import constitutive_mex as cm
+    from slope_stability_octave_amd import workloads as W
+    from slope_stability_octave_amd._lib import DeviceArray, EventTimer
+
+    _lib.require_gpu()
+    n = args.points
+    t_gen = time.perf_counter()
+    d = W.config2_points(n)
+    print(f"[bench] generated {n} points in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    dE = DeviceArray.from_host(d["E"])
+    dc0, dphi, dpsi, dlam = (DeviceArray.from_host(d[k]) for k in ("c0", "phi", "psi", "lam"))
+    dG, dK, dL = (DeviceArray.from_host(d[k]) for k in ("shear", "bulk", "lame"))
+    dcb, dsp = DeviceArray(n), DeviceArray(n)
+    dS, dD = DeviceArray((n, 6)), DeviceArray((n, 36))
+
+    tim = EventTimer()
+    tim.start()
+    cm.reduction_dev(n, "B", dc0, dphi, dpsi, dlam, True, dcb, dsp)
+    tim.stop()
+    red_ms = tim.elapsed_ms()
+    for _ in range(args.warmup):
+        cm.constitutive_dev(3, n, dE, dcb, dsp, dG, dK, dL, dS, dD)
+    _lib.synchronize()
+
+    tw0 = time.perf_counter()
+    tim.start()
+    for _ in range(args.steps):
+        cm.constitutive_dev(3, n, dE, dcb, dsp, dG, dK, dL, dS, dD)
+    tim.stop()
+    kern_ms = tim.elapsed_ms() / args.steps
+    _lib.synchronize()
+    wall_ms = (time.perf_counter() - tw0) * 1e3 / args.steps
+    print(f"[bench] {args.steps} steps: {kern_ms:.3f} ms/step (events), {wall_ms:.3f} ms/step (wall)",
+          file=sys.stderr, flush=True)
+
+    # parity spot check on a slice against the oracle (not timed)
+    parity = None
+    try:
+        from oracle import oracle as O
+        m = min(n, 200_000)
+        S = dS.get()[:m].T
+        DS = dD.get()[:m].T
+        cbh, sph = dcb.get()[:m], dsp.get()[:m]
+        S0, D0, br0 = O.mc3d(np.ascontiguousarray(d["E"][:m].T), cbh, sph, d["shear"][:m],
+                             d["bulk"][:m], d["lame"][:m])
+        den = np.maximum(np.abs(D0).max(axis=0), 1e-300)
+        parity = {
+            "points_checked": m,
+            "max_rel_S": float((np.abs(S - S0).max(axis=0) / np.abs(S0).max(axis=0)).max()),
+            "max_rel_DS": float((np.abs(DS - D0).max(axis=0) / den).max()),
+            "branch_histogram": np.bincount(br0, minlength=5).tolist(),
+        }
+        cpu = None if args.no_cpu_baseline else cpu_baseline(d, cbh if m == n else dcb.get(),
+                                                             sph if m == n else dsp.get())
+    except Exception as exc:  # the checker must not hide the measurement
+        print(f"[bench] oracle check skipped: {exc}", file=sys.stderr)
+        cpu = None
+
+    gpts = n / (kern_ms * 1e-3) / 1e9
+    achieved = BYTES_PER_POINT * n / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "constitutive_sds_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": gpts,
+        "unit": "Gpt/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": kern_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "constitutive microbench: 3D box 40x20x20 m, 100x50x40 hexes x 6 P2 tets, "
+                        "11 points each (13.2M points), Mohr-Coulomb S+DS, Davis B, lambda~U[1,2]",
+            "points": n,
+            "layout": "point-major E/S/DS (MEX interface), per-point material arrays",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "extra": {
+            "wall_ms_per_step": wall_ms,
+            "reduction_ms": red_ms,
+            "reduction_gpts": n / (red_ms * 1e-3) / 1e9,
+            "parity": parity,
+        },
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,214 @@
+// C ABI of the SSR hot-path library: runtime helpers (device memory, copies,
+// stream, events) and the host-pointer entry points that keep the signatures
+// of the reference's MEX files.  Declarations: include/ssr_hip.h.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ssr_common.h"
+
+static thread_local char g_err[512] = "";
+static hipStream_t g_stream = nullptr;
+
+void ssr_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+hipStream_t ssr_stream(void) { return g_stream; }
+
+extern "C" {
+
+const char *ssr_last_error(void) { return g_err; }
+
+int ssr_version(void) { return 1; }
+
+int ssr_device_count(int *count)
+{
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+    return SSR_OK;
+}
+
+int ssr_set_stream(void *stream)
+{
+    g_stream = (hipStream_t)stream;
+    return SSR_OK;
+}
+
+void *ssr_get_stream(void) { return (void *)g_stream; }
+
+int ssr_dmalloc(void **ptr, size_t bytes)
+{
+    SSR_REQUIRE(ptr, "dmalloc: null out pointer");
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        ssr_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return SSR_ERR_ALLOC;
+    }
+    return SSR_OK;
+}
+
+int ssr_dfree(void *ptr)
+{
+    if (ptr) SSR_HIP_CHECK(hipFree(ptr));
+    return SSR_OK;
+}
+
+int ssr_htod(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    SSR_HIP_CHECK(hipStreamSynchronize(g_stream));
+    return SSR_OK;
+}
+
+int ssr_dtoh(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    SSR_HIP_CHECK(hipStreamSynchronize(g_stream));
+    return SSR_OK;
+}
+
+int ssr_dtod(void *dst, const void *src, size_t bytes)
+{
+    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    return SSR_OK;
+}
+
+int ssr_dzero(void *dst, size_t bytes)
+{
+    if (bytes) SSR_HIP_CHECK(hipMemsetAsync(dst, 0, bytes, g_stream));
+    return SSR_OK;
+}
+
+int ssr_synchronize(void)
+{
+    SSR_HIP_CHECK(hipStreamSynchronize(g_stream));
+    return SSR_OK;
+}
+
+int ssr_event_create(void **ev)
+{
+    hipEvent_t e;
+    SSR_HIP_CHECK(hipEventCreate(&e));
+    *ev = (void *)e;
+    return SSR_OK;
+}
+
+int ssr_event_destroy(void *ev)
+{
+    if (ev) SSR_HIP_CHECK(hipEventDestroy((hipEvent_t)ev));
+    return SSR_OK;
+}
+
+int ssr_event_record(void *ev)
+{
+    SSR_HIP_CHECK(hipEventRecord((hipEvent_t)ev, g_stream));
+    return SSR_OK;
+}
+
+int ssr_event_elapsed_ms(void *start, void *stop, float *ms)
+{
+    SSR_HIP_CHECK(hipEventSynchronize((hipEvent_t)stop));
+    SSR_HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return SSR_OK;
+}
+
+int ssr_dev_constitutive_3D(int64_t n, const double *E, const double *c_bar, const double *sin_phi,
+                            const double *shear, const double *bulk, const double *lame, double *S,
+                            double *DS, int8_t *branch);
+int ssr_dev_constitutive_2D(int64_t n, const double *E, const double *c_bar, const double *sin_phi,
+                            const double *shear, const double *bulk, const double *lame, double *S,
+                            double *DS, int8_t *branch);
+
+}  // extern "C"
+
+// Host-pointer path shared by the four constitutive MEX-signature entry points.
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) { return ssr_dmalloc(&p, bytes); }
+    double *d() { return (double *)p; }
+};
+
+int constitutive_host(int dim, int64_t n, const double *E, const double *c_bar, const double *sin_phi,
+                      const double *shear, const double *bulk, const double *lame, double *S,
+                      double *DS)
+{
+    SSR_REQUIRE(n >= 0, "n_int must be >= 0");
+    if (n == 0) return SSR_OK;
+    SSR_REQUIRE(E && c_bar && sin_phi && shear && bulk && lame && S, "null argument");
+    const int ne = dim == 3 ? 6 : 3, nsd = dim == 3 ? 6 : 4, nds = dim == 3 ? 36 : 9;
+    DevBuf dE, dm, dS, dD;
+    int rc;
+    if ((rc = dE.alloc(sizeof(double) * ne * n))) return rc;
+    if ((rc = dm.alloc(sizeof(double) * 5 * n))) return rc;
+    if ((rc = dS.alloc(sizeof(double) * nsd * n))) return rc;
+    if (DS && (rc = dD.alloc(sizeof(double) * nds * n))) return rc;
+    if ((rc = ssr_htod(dE.p, E, sizeof(double) * ne * n))) return rc;
+    const double *src[5] = {c_bar, sin_phi, shear, bulk, lame};
+    for (int k = 0; k < 5; ++k)
+        if ((rc = ssr_htod(dm.d() + k * n, src[k], sizeof(double) * n))) return rc;
+    if (dim == 3)
+        rc = ssr_dev_constitutive_3D(n, dE.d(), dm.d(), dm.d() + n, dm.d() + 2 * n, dm.d() + 3 * n,
+                                     dm.d() + 4 * n, dS.d(), DS ? dD.d() : nullptr, nullptr);
+    else
+        rc = ssr_dev_constitutive_2D(n, dE.d(), dm.d(), dm.d() + n, dm.d() + 2 * n, dm.d() + 3 * n,
+                                     dm.d() + 4 * n, dS.d(), DS ? dD.d() : nullptr, nullptr);
+    if (rc) return rc;
+    if ((rc = ssr_dtoh(S, dS.p, sizeof(double) * nsd * n))) return rc;
+    if (DS && (rc = ssr_dtoh(DS, dD.p, sizeof(double) * nds * n))) return rc;
+    return SSR_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ssr_constitutive_problem_3D_SDS(int64_t n_int, const double *E, const double *c_bar,
+                                    const double *sin_phi, const double *shear, const double *bulk,
+                                    const double *lame, double *S, double *DS)
+{
+    SSR_REQUIRE(DS, "constitutive_problem_3D_SDS: DS output required");
+    return constitutive_host(3, n_int, E, c_bar, sin_phi, shear, bulk, lame, S, DS);
+}
+
+int ssr_constitutive_problem_3D_S(int64_t n_int, const double *E, const double *c_bar,
+                                  const double *sin_phi, const double *shear, const double *bulk,
+                                  const double *lame, double *S)
+{
+    return constitutive_host(3, n_int, E, c_bar, sin_phi, shear, bulk, lame, S, nullptr);
+}
+
+int ssr_constitutive_problem_2D_SDS(int64_t n_int, int64_t rows_E, const double *E,
+                                    const double *c_bar, const double *sin_phi, const double *shear,
+                                    const double *bulk, const double *lame, double *S, double *DS)
+{
+    SSR_REQUIRE(rows_E == 3, "constitutive_problem_2D_SDS_mex: E must have 3 rows.");
+    SSR_REQUIRE(DS, "constitutive_problem_2D_SDS: DS output required");
+    return constitutive_host(2, n_int, E, c_bar, sin_phi, shear, bulk, lame, S, DS);
+}
+
+int ssr_constitutive_problem_2D_S(int64_t n_int, int64_t rows_E, const double *E,
+                                  const double *c_bar, const double *sin_phi, const double *shear,
+                                  const double *bulk, const double *lame, double *S)
+{
+    SSR_REQUIRE(rows_E == 3, "constitutive_problem_2D_S_mex: E must have 3 rows.");
+    return constitutive_host(2, n_int, E, c_bar, sin_phi, shear, bulk, lame, S, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,537 @@
+// Mohr-Coulomb return mapping (stress S and consistent tangent DS per
+// integration point) for gfx950.
+//
+// Reference behaviour: constitutive_3D_kernel.h:101-410 (3D) and
+// constitutive_2D_kernel.h:36-369 (plane strain), driven per point by
+// constitutive_problem_{2D,3D}_{S,SDS}_mex.c.
+//
+// Design (one lane per point):
+//  * Branch selection (elastic / smooth / left edge / right edge / apex) uses
+//    the same invariants, trigonometric eigenvalues and tests as the
+//    reference, evaluated with FP contraction OFF so that the decisions see
+//    the reference's rounding.
+//  * Every branch is then reduced to a handful of scalars: S and DS are
+//    linear combinations of the eigenprojections P1..P3, the identity vector
+//    iota, the trial strain t, the operator d(T^2)/dT and the Voigt identity.
+//    One shared epilogue evaluates DS = a*dT2 + b*Id + sum C_uv u (x) v, so
+//    lanes of a wavefront that took different branches do not serialise over
+//    five separate 6x6 loops.  DS is symmetric (associated flow), so only the
+//    21 upper entries are formed.
+//  * The MEX-layout entry points read/write point-major (AoS) arrays;
+//    packed/SoA variants serve the device-resident Newton path.
+#include "ssr_common.h"
+
+namespace {
+
+constexpr double kTwoPiOver3 = 2.0 * 3.14159265358979323846 / 3.0;
+
+// d(T^2)/dT entry (row i, col j) in the mixed Voigt convention of
+// constitutive_3D_kernel.h:48-75 (symmetric).
+__host__ __device__ __forceinline__ double dsq3(int i, int j, const double t[6])
+{
+    if (i > j) { int k = i; i = j; j = k; }
+    if (i == j) {
+        if (i < 3) return 2.0 * t[i];
+        if (i == 3) return 0.5 * (t[0] + t[1]);
+        if (i == 4) return 0.5 * (t[1] + t[2]);
+        return 0.5 * (t[0] + t[2]);
+    }
+    if ((i == 0 && j == 3) || (i == 1 && j == 3)) return t[3];
+    if ((i == 0 && j == 5) || (i == 2 && j == 5)) return t[5];
+    if ((i == 1 && j == 4) || (i == 2 && j == 4)) return t[4];
+    if (i == 3 && j == 4) return 0.5 * t[5];
+    if (i == 3 && j == 5) return 0.5 * t[4];
+    if (i == 4 && j == 5) return 0.5 * t[3];
+    return 0.0;
+}
+
+// Branch-reduced coefficients for one 3D point.
+struct Coef3 {
+    double sP1, sP2, sP3, sI, sT;             // S = sP1 P1 + sP2 P2 + sP3 P3 + sI iota + sT t
+    double a, b;                              // DS += a dT2 + b Id
+    double c11, c22, c33, c13, cII, c1I, c3I, cIT, c1T, c3T;  // symmetric C over (P1,P2,P3,iota,t)
+    bool use1, use2, use3;
+};
+
+// Returns branch code 0..4 (elastic, smooth, left edge, right edge, apex).
+__host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, double sphi, double G,
+                                         double lam, double &e1, double &e2, double &e3, Coef3 &k)
+{
+#pragma clang fp contract(off)
+    const double spsi = sphi;  // associated flow
+    const double I1 = t[0] + t[1] + t[2];
+    const double I2 = t[0] * t[1] + t[0] * t[2] + t[1] * t[2] - t[3] * t[3] - t[4] * t[4] - t[5] * t[5];
+    const double I3 = t[0] * t[1] * t[2] - t[2] * t[3] * t[3] - t[1] * t[5] * t[5] - t[0] * t[4] * t[4] +
+                      2.0 * t[3] * t[4] * t[5];
+    double Q = (I1 * I1 - 3.0 * I2) / 9.0;
+    Q = Q < 0.0 ? 0.0 : Q;
+    const double R = (-2.0 * I1 * I1 * I1 + 9.0 * I1 * I2 - 27.0 * I3) / 54.0;
+    const double rq = sqrt(Q);
+    double th = 0.0;
+    if (!(Q < 1e-30)) {
+        double c = R / (rq * Q);
+        c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+        th = acos(c) / 3.0;
+    }
+    e1 = -2.0 * rq * cos(th + kTwoPiOver3) + I1 / 3.0;
+    e2 = -2.0 * rq * cos(th - kTwoPiOver3) + I1 / 3.0;
+    e3 = -2.0 * rq * cos(th) + I1 / 3.0;
+
+    k = Coef3{};
+    const double f_tr = 2.0 * G * ((1.0 + sphi) * e1 - (1.0 - sphi) * e3) + 2.0 * lam * sphi * I1 - cb;
+    if (f_tr <= 0.0) {
+        k.sI = lam * I1;
+        k.sT = 2.0 * G;
+        k.b = 2.0 * G;
+        k.cII = lam;
+        return 0;
+    }
+    const double g_sl = (e1 - e2) / (1.0 + spsi);
+    const double g_sr = (e2 - e3) / (1.0 - spsi);
+    const double den_s = 4.0 * lam * sphi * spsi + 4.0 * G * (1.0 + sphi * spsi);
+    const double lam_s = f_tr / den_s;
+    if (lam_s <= g_sl && lam_s <= g_sr) {
+        const double g1 = 1.0 / ((e1 - e2) * (e1 - e3));
+        const double g2 = 1.0 / ((e2 - e1) * (e2 - e3));
+        const double g3 = 1.0 / ((e3 - e1) * (e3 - e2));
+        const double s1 = lam * I1 + 2.0 * G * e1 - lam_s * (2.0 * lam * spsi + 2.0 * G * (1.0 + spsi));
+        const double s2 = lam * I1 + 2.0 * G * e2 - lam_s * 2.0 * lam * spsi;
+        const double s3 = lam * I1 + 2.0 * G * e3 - lam_s * (2.0 * lam * spsi - 2.0 * G * (1.0 - spsi));
+        const double A1 = s1 * g1, A2 = s2 * g2, A3 = s3 * g3;
+        const double k1a = 2.0 * e1 - e2 - e3, k1b = e2 - e3;
+        const double k2a = 2.0 * e2 - e1 - e3, k2b = e1 - e3;
+        const double k3a = 2.0 * e3 - e1 - e2, k3b = e1 - e2;
+        const double r = 1.0 / den_s;
+        const double al1 = 2.0 * G * (1.0 + sphi), al3 = -2.0 * G * (1.0 - sphi), alI = 2.0 * lam * sphi;
+        k.sP1 = s1; k.sP2 = s2; k.sP3 = s3;
+        k.a = A1 + A2 + A3;
+        k.b = -(A1 * (e2 + e3) + A2 * (e1 + e3) + A3 * (e1 + e2));
+        k.c11 = -A1 * k1a - A2 * k2b - A3 * k3b + 2.0 * G - r * al1 * al1;
+        k.c22 = -A1 * k1b - A2 * k2a + A3 * k3b + 2.0 * G;
+        k.c33 = A1 * k1b + A2 * k2b - A3 * k3a + 2.0 * G - r * al3 * al3;
+        k.c13 = -r * al1 * al3;
+        k.cII = lam - r * alI * alI;
+        k.c1I = -r * al1 * alI;
+        k.c3I = -r * al3 * alI;
+        k.use1 = k.use2 = k.use3 = true;
+        return 1;
+    }
+    if (g_sl < g_sr) {
+        const double den_l = 4.0 * lam * sphi * spsi + G * (1.0 + sphi) * (1.0 + spsi) +
+                             2.0 * G * (1.0 - sphi) * (1.0 - spsi);
+        const double lam_l = (G * ((1.0 + sphi) * (e1 + e2) - 2.0 * (1.0 - sphi) * e3) + 2.0 * lam * sphi * I1 - cb) / den_l;
+        const double g_la = (e1 + e2 - 2.0 * e3) / (3.0 - spsi);
+        if (lam_l >= g_sl && lam_l <= g_la) {
+            const double g3 = 1.0 / ((e3 - e1) * (e3 - e2));
+            const double s1 = lam * I1 + G * (e1 + e2) - lam_l * (2.0 * lam * spsi + G * (1.0 + spsi));
+            const double s3 = lam * I1 + 2.0 * G * e3 - lam_l * (2.0 * lam * spsi - 2.0 * G * (1.0 - spsi));
+            const double A = (s3 - s1) * g3;
+            const double a12 = e1 + e2;
+            const double r = 1.0 / den_l;
+            const double bI = G * (1.0 + sphi) + 2.0 * lam * sphi;
+            const double b3 = -(G * (1.0 + sphi) + 2.0 * G * (1.0 - sphi));
+            k.sI = s1; k.sP3 = s3 - s1;
+            k.a = A;
+            k.b = -A * a12;
+            k.cII = A * a12 + lam + G - r * bI * bI;
+            k.c3I = -A * a12 + A * e3 - G - r * b3 * bI;
+            k.c33 = A * (2.0 * a12 - 4.0 * e3) + 3.0 * G - r * b3 * b3;
+            k.cIT = -A;
+            k.c3T = A;
+            k.use3 = true;
+            return 2;
+        }
+    }
+    if (g_sl > g_sr) {
+        const double den_r = 4.0 * lam * sphi * spsi + 2.0 * G * (1.0 + sphi) * (1.0 + spsi) +
+                             G * (1.0 - sphi) * (1.0 - spsi);
+        const double lam_r = (G * (2.0 * (1.0 + sphi) * e1 - (1.0 - sphi) * (e2 + e3)) + 2.0 * lam * sphi * I1 - cb) / den_r;
+        const double g_ra = (2.0 * e1 - e2 - e3) / (3.0 + spsi);
+        if (lam_r >= g_sr && lam_r <= g_ra) {
+            const double g1 = 1.0 / ((e1 - e2) * (e1 - e3));
+            const double s1 = lam * I1 + 2.0 * G * e1 - lam_r * (2.0 * lam * spsi + 2.0 * G * (1.0 + spsi));
+            const double s3 = lam * I1 + G * (e2 + e3) - lam_r * (2.0 * lam * spsi - G * (1.0 - spsi));
+            const double A = (s1 - s3) * g1;
+            const double a23 = e2 + e3;
+            const double r = 1.0 / den_r;
+            const double b1 = 2.0 * G * (1.0 + sphi) + G * (1.0 - sphi);
+            const double bI = 2.0 * lam * sphi - G * (1.0 - sphi);
+            k.sI = s3; k.sP1 = s1 - s3;
+            k.a = A;
+            k.b = -A * a23;
+            k.cII = A * a23 + lam + G - r * bI * bI;
+            k.c1I = -A * a23 + A * e1 - G - r * b1 * bI;
+            k.c11 = A * (2.0 * a23 - 4.0 * e1) + 3.0 * G - r * b1 * b1;
+            k.cIT = -A;
+            k.c1T = A;
+            k.use1 = true;
+            return 3;
+        }
+    }
+    k.sI = cb / (2.0 * sphi);
+    return 4;
+}
+
+// Eigenprojection (T^2 - (e_m+e_n) T + e_m e_n I) / ((e_k-e_m)(e_k-e_n)).
+__host__ __device__ __forceinline__ void proj3(const double t2[6], const double t[6], double ek, double em,
+                                      double en, bool use, double P[6])
+{
+    const double g = 1.0 / ((ek - em) * (ek - en));
+    const double sm = em + en, pm = em * en;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double v = g * (t2[i] - sm * t[i] + (i < 3 ? pm : 0.0));
+        P[i] = use ? v : 0.0;
+    }
+}
+
+// Full point evaluation.  S[6] always, D21 (upper triangle, row-major packed)
+// when WITH_DS.
+template <bool WITH_DS>
+__host__ __device__ __forceinline__ int mc3d_point(const double eps[6], double cb, double sphi, double G,
+                                          double lam, double S[6], double D21[21])
+{
+    double t[6] = {eps[0], eps[1], eps[2], 0.5 * eps[3], 0.5 * eps[4], 0.5 * eps[5]};
+    double e1, e2, e3;
+    Coef3 k;
+    const int br = mc3d_coef(t, cb, sphi, G, lam, e1, e2, e3, k);
+
+    double t2[6];
+    t2[0] = t[0] * t[0] + t[3] * t[3] + t[5] * t[5];
+    t2[1] = t[1] * t[1] + t[3] * t[3] + t[4] * t[4];
+    t2[2] = t[2] * t[2] + t[4] * t[4] + t[5] * t[5];
+    t2[3] = t[0] * t[3] + t[1] * t[3] + t[4] * t[5];
+    t2[4] = t[3] * t[5] + t[1] * t[4] + t[2] * t[4];
+    t2[5] = t[0] * t[5] + t[3] * t[4] + t[2] * t[5];
+    double P1[6], P2[6], P3[6];
+    proj3(t2, t, e1, e2, e3, k.use1, P1);
+    proj3(t2, t, e2, e1, e3, k.use2, P2);
+    proj3(t2, t, e3, e1, e2, k.use3, P3);
+
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double io = i < 3 ? 1.0 : 0.0;
+        S[i] = k.sP1 * P1[i] + k.sP2 * P2[i] + k.sP3 * P3[i] + k.sI * io + k.sT * t[i];
+    }
+    if (WITH_DS) {
+        // W_u(j) = sum_v C_uv v(j) for u in (P1,P2,P3,iota,t)
+        double W1[6], W2[6], W3[6], WI[6], WT[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const double io = j < 3 ? 1.0 : 0.0;
+            W1[j] = k.c11 * P1[j] + k.c13 * P3[j] + k.c1I * io + k.c1T * t[j];
+            W2[j] = k.c22 * P2[j];
+            W3[j] = k.c13 * P1[j] + k.c33 * P3[j] + k.c3I * io + k.c3T * t[j];
+            WI[j] = k.c1I * P1[j] + k.c3I * P3[j] + k.cII * io + k.cIT * t[j];
+            WT[j] = k.c1T * P1[j] + k.c3T * P3[j] + k.cIT * io;
+        }
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+#pragma unroll
+            for (int j = i; j < 6; ++j) {
+                const double id = (i == j) ? (i < 3 ? 1.0 : 0.5) : 0.0;
+                double v = P1[i] * W1[j] + P2[i] * W2[j] + P3[i] * W3[j] + t[i] * WT[j];
+                if (i < 3) v += WI[j];
+                v += k.a * dsq3(i, j, t) + k.b * id;
+                D21[q++] = v;
+            }
+        }
+    }
+    return br;
+}
+
+// Packed upper index of (i,j), i<=j, for 6x6.
+__host__ __device__ constexpr int sym6(int i, int j) { return i * 6 - (i * (i - 1)) / 2 + (j - i); }
+
+// ---------------------------------------------------------------------------
+// MEX layout kernels: E (6 x n), S (6 x n), DS (36 x n, each 6x6 column-major)
+// point-major, material arrays 1 x n.
+// ---------------------------------------------------------------------------
+template <bool WITH_DS>
+__global__ __launch_bounds__(256) void k_mc3d_aos(int64_t n, const double *__restrict__ E,
+                                                  const double *__restrict__ cbar,
+                                                  const double *__restrict__ sphi,
+                                                  const double *__restrict__ shear,
+                                                  const double *__restrict__ lame,
+                                                  double *__restrict__ S, double *__restrict__ DS,
+                                                  int8_t *__restrict__ branch)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double2 *e2 = reinterpret_cast<const double2 *>(E + 6 * p);
+    double2 a = e2[0], b = e2[1], c = e2[2];
+    double eps[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+    double s[6], d[21];
+    const int br = mc3d_point<WITH_DS>(eps, cbar[p], sphi[p], shear[p], lame[p], s, d);
+    double2 *s2 = reinterpret_cast<double2 *>(S + 6 * p);
+    s2[0] = make_double2(s[0], s[1]);
+    s2[1] = make_double2(s[2], s[3]);
+    s2[2] = make_double2(s[4], s[5]);
+    if (WITH_DS) {
+        double2 *o = reinterpret_cast<double2 *>(DS + 36 * p);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) {
+            const int i0 = (2 * k) % 6, j0 = (2 * k) / 6;  // column-major (row i, col j)
+            const int i1 = (2 * k + 1) % 6, j1 = (2 * k + 1) / 6;
+            const double v0 = d[i0 <= j0 ? sym6(i0, j0) : sym6(j0, i0)];
+            const double v1 = d[i1 <= j1 ? sym6(i1, j1) : sym6(j1, i1)];
+            o[k] = make_double2(v0, v1);
+        }
+    }
+    if (branch) branch[p] = (int8_t)br;
+}
+
+// ---------------------------------------------------------------------------
+// 2D (plane strain)
+// ---------------------------------------------------------------------------
+// Ordered slot data for the three principal strains.
+template <bool WITH_DS>
+__host__ __device__ __forceinline__ int mc2d_point(const double eps[3], double cb, double sphi, double G,
+                                          double lam, double S[4], double D6[6])
+{
+    const double x = eps[0], y = eps[1], gm = eps[2];
+    double e1, e2, e3;
+    int br;
+    // slot -> unordered eigenvalue: 0 = l+, 1 = l-, 2 = out-of-plane
+    int o1, o2, o3;
+    double s1 = 0, s2 = 0, s3 = 0, r = 0, b1 = 0, b2 = 0, b3 = 0, bI = 0;
+    double c1 = 0, c2 = 0, c3 = 0, c12 = 0, c23 = 0, cII = 0, bId = 0, sI = 0, sT = 0;
+    double h1 = 0, h2 = 0, h3 = 0;  // coefficient of the second-derivative operator of each slot
+    double tr, rad, lp, lm;
+    {
+#pragma clang fp contract(off)
+        tr = x + y;
+        rad = sqrt((x - y) * (x - y) + gm * gm);
+        lp = 0.5 * (tr + rad);
+        lm = 0.5 * (tr - rad);
+        const double lz = 0.0;
+        e1 = lp; e2 = lm; e3 = lz; o1 = 0; o2 = 1; o3 = 2;
+        if ((lp >= lz) && (lz > lm)) { e2 = lz; e3 = lm; o2 = 2; o3 = 1; }
+        if (lz > lp) { e1 = lz; e2 = lp; e3 = lm; o1 = 2; o2 = 0; o3 = 1; }
+        const double trE = e1 + e2 + e3;
+        const double f_tr = 2.0 * G * ((1.0 + sphi) * e1 - (1.0 - sphi) * e3) + 2.0 * lam * sphi * trE - cb;
+        const double g_sl = (e1 - e2) / (1.0 + sphi);
+        const double g_sr = (e2 - e3) / (1.0 - sphi);
+        const double g_la = (e1 + e2 - 2.0 * e3) / (3.0 - sphi);
+        const double g_ra = (2.0 * e1 - e2 - e3) / (3.0 + sphi);
+        const double den_s = 4.0 * lam * sphi * sphi + 2.0 * G * (1.0 + sphi) * (1.0 + sphi) +
+                             2.0 * G * (1.0 - sphi) * (1.0 - sphi);
+        const double den_l = 4.0 * lam * sphi * sphi + G * (1.0 + sphi) * (1.0 + sphi) +
+                             2.0 * G * (1.0 - sphi) * (1.0 - sphi);
+        const double den_r = 4.0 * lam * sphi * sphi + 2.0 * G * (1.0 + sphi) * (1.0 + sphi) +
+                             G * (1.0 - sphi) * (1.0 - sphi);
+        const double lam_s = f_tr / den_s;
+        const double lam_l = (G * ((1.0 + sphi) * (e1 + e2) - 2.0 * (1.0 - sphi) * e3) + 2.0 * lam * sphi * trE - cb) / den_l;
+        const double lam_r = (G * (2.0 * (1.0 + sphi) * e1 - (1.0 - sphi) * (e2 + e3)) + 2.0 * lam * sphi * trE - cb) / den_r;
+        if (f_tr <= 0.0) br = 0;
+        else if (lam_s <= ((g_sl < g_sr) ? g_sl : g_sr)) br = 1;
+        else if ((g_sl < g_sr) && (lam_l >= g_sl) && (lam_l <= g_la)) br = 2;
+        else if ((g_sl > g_sr) && (lam_r >= g_sr) && (lam_r <= g_ra)) br = 3;
+        else br = 4;
+
+        if (br == 0) {
+            sI = lam * tr; sT = 2.0 * G; bId = 2.0 * G; cII = lam;
+        } else if (br == 1) {
+            s1 = lam * trE + 2.0 * G * e1 - lam_s * (2.0 * lam * sphi + 2.0 * G * (1.0 + sphi));
+            s2 = lam * trE + 2.0 * G * e2 - lam_s * (2.0 * lam * sphi);
+            s3 = lam * trE + 2.0 * G * e3 - lam_s * (2.0 * lam * sphi - 2.0 * G * (1.0 - sphi));
+            h1 = s1; h2 = s2; h3 = s3;
+            c1 = c2 = c3 = 2.0 * G; cII = lam;
+            b1 = 2.0 * G * (1.0 + sphi); b3 = -2.0 * G * (1.0 - sphi); bI = 2.0 * lam * sphi;
+            r = 1.0 / den_s;
+        } else if (br == 2) {
+            const double sa = lam * trE + G * (e1 + e2) - lam_l * (2.0 * lam * sphi + G * (1.0 + sphi));
+            const double sc = lam * trE + 2.0 * G * e3 - lam_l * (2.0 * lam * sphi - 2.0 * G * (1.0 - sphi));
+            s1 = sa; s2 = sa; s3 = sc;
+            h1 = sa; h2 = sa; h3 = sc;
+            c1 = c2 = c12 = G; c3 = 2.0 * G; cII = lam;
+            b1 = b2 = G * (1.0 + sphi); b3 = -2.0 * G * (1.0 - sphi); bI = 2.0 * lam * sphi;
+            r = 1.0 / den_l;
+        } else if (br == 3) {
+            const double sa = lam * trE + 2.0 * G * e1 - lam_r * (2.0 * lam * sphi + 2.0 * G * (1.0 + sphi));
+            const double sc = lam * trE + G * (e2 + e3) - lam_r * (2.0 * lam * sphi - G * (1.0 - sphi));
+            s1 = sa; s2 = sc; s3 = sc;
+            h1 = sa; h2 = sc; h3 = sc;
+            c1 = 2.0 * G; c2 = c3 = c23 = G; cII = lam;
+            b1 = 2.0 * G * (1.0 + sphi); b2 = b3 = -G * (1.0 - sphi); bI = 2.0 * lam * sphi;
+            r = 1.0 / den_r;
+        } else {
+            const double sa = cb / (2.0 * sphi);
+            sI = sa;
+        }
+    }
+    // unordered first derivatives of l+ (dp), l- (dm), out-of-plane (dz)
+    double dp[3], dm[3];
+    if (rad != 0.0) {
+        dp[0] = (x - lm) / rad; dp[1] = (y - lm) / rad; dp[2] = 0.5 * gm / rad;
+    } else {
+        dp[0] = 1.0; dp[1] = 1.0; dp[2] = 0.0;
+    }
+    dm[0] = 1.0 - dp[0]; dm[1] = 1.0 - dp[1]; dm[2] = -dp[2];
+    double v1[3], v2[3], v3[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        v1[i] = o1 == 0 ? dp[i] : (o1 == 1 ? dm[i] : 0.0);
+        v2[i] = o2 == 0 ? dp[i] : (o2 == 1 ? dm[i] : 0.0);
+        v3[i] = o3 == 0 ? dp[i] : (o3 == 1 ? dm[i] : 0.0);
+    }
+    const double z1 = o1 == 2 ? 1.0 : 0.0, z2 = o2 == 2 ? 1.0 : 0.0, z3 = o3 == 2 ? 1.0 : 0.0;
+    // S: (s11, s22, s12, s33); the trial tensor is (x, y, gm/2, 0)
+    const double tt[3] = {x, y, 0.5 * gm};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        S[i] = s1 * v1[i] + s2 * v2[i] + s3 * v3[i] + (i < 2 ? sI : 0.0) + sT * tt[i];
+    S[3] = s1 * z1 + s2 * z2 + s3 * z3 + sI;
+    if (WITH_DS) {
+        // second derivative of l+ is Hp, of l- is -Hp, of the out-of-plane value 0
+        const double eta1 = o1 == 0 ? 1.0 : (o1 == 1 ? -1.0 : 0.0);
+        const double eta2 = o2 == 0 ? 1.0 : (o2 == 1 ? -1.0 : 0.0);
+        const double eta3 = o3 == 0 ? 1.0 : (o3 == 1 ? -1.0 : 0.0);
+        const double h = h1 * eta1 + h2 * eta2 + h3 * eta3;
+        const double irad = rad != 0.0 ? 1.0 / rad : 0.0;
+        double w[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) w[i] = b1 * v1[i] + b2 * v2[i] + b3 * v3[i] + (i < 2 ? bI : 0.0);
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = i; j < 3; ++j) {
+                const double dg = (i == j) ? (i < 2 ? 1.0 : 0.5) : 0.0;
+                const double Hp = (dg - dp[i] * dp[j] - dm[i] * dm[j]) * irad;
+                double v = h * Hp + bId * dg;
+                v += c1 * v1[i] * v1[j] + c2 * v2[i] * v2[j] + c3 * v3[i] * v3[j];
+                v += c12 * (v1[i] * v2[j] + v2[i] * v1[j]) + c23 * (v2[i] * v3[j] + v3[i] * v2[j]);
+                v += (i < 2 && j < 2) ? cII : 0.0;
+                v -= r * w[i] * w[j];
+                D6[q++] = v;
+            }
+        }
+    }
+    return br;
+}
+
+__host__ __device__ constexpr int sym3(int i, int j) { return i * 3 - (i * (i - 1)) / 2 + (j - i); }
+
+template <bool WITH_DS>
+__global__ __launch_bounds__(256) void k_mc2d_aos(int64_t n, const double *__restrict__ E,
+                                                  const double *__restrict__ cbar,
+                                                  const double *__restrict__ sphi,
+                                                  const double *__restrict__ shear,
+                                                  const double *__restrict__ lame,
+                                                  double *__restrict__ S, double *__restrict__ DS,
+                                                  int8_t *__restrict__ branch)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double eps[3] = {E[3 * p], E[3 * p + 1], E[3 * p + 2]};
+    double s[4], d[6];
+    const int br = mc2d_point<WITH_DS>(eps, cbar[p], sphi[p], shear[p], lame[p], s, d);
+    double2 *s2 = reinterpret_cast<double2 *>(S + 4 * p);
+    s2[0] = make_double2(s[0], s[1]);
+    s2[1] = make_double2(s[2], s[3]);
+    if (WITH_DS) {
+        double *o = DS + 9 * p;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int i = k % 3, j = k / 3;
+            o[k] = d[i <= j ? sym3(i, j) : sym3(j, i)];
+        }
+    }
+    if (branch) branch[p] = (int8_t)br;
+}
+
+// Davis strength reduction per point (reduction.m:25-58) for a scalar lambda
+// or one lambda per point.  Davis C's vector `if` is resolved on the host.
+__global__ void k_reduction(int64_t n, int davis, int c_all, const double *__restrict__ c0,
+                            const double *__restrict__ phi, const double *__restrict__ psi,
+                            const double *__restrict__ lam, int lam_stride,
+                            double *__restrict__ cbar, double *__restrict__ sphi)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+  {
+#pragma clang fp contract(off)
+    const double l = lam[lam_stride ? p : 0];
+    double c_l, phi_l;
+    if (davis == 'A') {
+        const double beta = cos(phi[p]) * cos(psi[p]) / (1.0 - sin(phi[p]) * sin(psi[p]));
+        c_l = beta * c0[p] / l;
+        phi_l = atan(beta * tan(phi[p]) / l);
+    } else if (davis == 'B') {
+        const double c01 = c0[p] / l;
+        const double phi1 = atan(tan(phi[p]) / l);
+        const double psi1 = atan(tan(psi[p]) / l);
+        const double beta = cos(phi1) * cos(psi1) / (1.0 - sin(phi1) * sin(psi1));
+        c_l = beta * c01;
+        phi_l = atan(beta * tan(phi1));
+    } else {
+        const double c01 = c0[p] / l;
+        const double phi1 = atan(tan(phi[p]) / l);
+        const double beta = c_all ? cos(phi1) * cos(psi[p]) / (1.0 - sin(phi1) * sin(psi[p])) : 1.0;
+        c_l = beta * c01;
+        phi_l = atan(beta * tan(phi1));
+    }
+    cbar[p] = 2.0 * c_l * cos(phi_l);
+    sphi[p] = sin(phi_l);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers (device pointers, library stream)
+// ---------------------------------------------------------------------------
+extern "C" int ssr_dev_constitutive_3D(int64_t n, const double *E, const double *c_bar,
+                                       const double *sin_phi, const double *shear,
+                                       const double *bulk, const double *lame, double *S,
+                                       double *DS, int8_t *branch)
+{
+    (void)bulk;
+    SSR_REQUIRE(n >= 0, "constitutive_3D: n_int must be >= 0");
+    if (n == 0) return SSR_OK;
+    SSR_REQUIRE(E && c_bar && sin_phi && shear && lame && S, "constitutive_3D: null argument");
+    const int th = 256;
+    if (DS)
+        hipLaunchKernelGGL(k_mc3d_aos<true>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n, E,
+                           c_bar, sin_phi, shear, lame, S, DS, branch);
+    else
+        hipLaunchKernelGGL(k_mc3d_aos<false>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n,
+                           E, c_bar, sin_phi, shear, lame, S, DS, branch);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+extern "C" int ssr_dev_constitutive_2D(int64_t n, const double *E, const double *c_bar,
+                                       const double *sin_phi, const double *shear,
+                                       const double *bulk, const double *lame, double *S,
+                                       double *DS, int8_t *branch)
+{
+    (void)bulk;
+    SSR_REQUIRE(n >= 0, "constitutive_2D: n_int must be >= 0");
+    if (n == 0) return SSR_OK;
+    SSR_REQUIRE(E && c_bar && sin_phi && shear && lame && S, "constitutive_2D: null argument");
+    const int th = 256;
+    if (DS)
+        hipLaunchKernelGGL(k_mc2d_aos<true>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n, E,
+                           c_bar, sin_phi, shear, lame, S, DS, branch);
+    else
+        hipLaunchKernelGGL(k_mc2d_aos<false>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n,
+                           E, c_bar, sin_phi, shear, lame, S, DS, branch);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+extern "C" int ssr_dev_reduction(int64_t n, int davis, const double *c0, const double *phi,
+                                 const double *psi, const double *lambda, int lambda_per_point,
+                                 int davis_c_all, double *c_bar, double *sin_phi)
+{
+    SSR_REQUIRE(davis == 'A' || davis == 'B' || davis == 'C', "Incorrect choice of the Davis approach.");
+    if (n == 0) return SSR_OK;
+    const int th = 256;
+    hipLaunchKernelGGL(k_reduction, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n, davis,
+                       davis_c_all, c0, phi, psi, lambda, lambda_per_point ? 1 : 0, c_bar, sin_phi);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}

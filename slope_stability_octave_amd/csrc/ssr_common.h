@@ -1,0 +1,42 @@
+// Shared helpers for the SSR hot-path HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#define SSR_OK 0
+#define SSR_ERR_ARG 1
+#define SSR_ERR_HIP 2
+#define SSR_ERR_ALLOC 3
+#define SSR_ERR_STATE 4
+
+extern "C" const char *ssr_last_error(void);
+void ssr_set_error(const char *fmt, ...);
+
+#define SSR_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            ssr_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
+                          __LINE__);                                                     \
+            return SSR_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+#define SSR_REQUIRE(cond, ...)          \
+    do {                                \
+        if (!(cond)) {                  \
+            ssr_set_error(__VA_ARGS__); \
+            return SSR_ERR_ARG;         \
+        }                               \
+    } while (0)
+
+// Grid size for a 1-D launch over n items; callers never pass n <= 0.
+static inline unsigned ssr_blocks(int64_t n, int threads)
+{
+    int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+hipStream_t ssr_stream(void);

@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+@pytest.fixture(scope="session")
+def golden3():
+    import numpy as np
+    return dict(np.load(os.path.join(ROOT, "tests", "golden", "golden_mc3d.npz")))
+
+
+@pytest.fixture(scope="session")
+def golden2():
+    import numpy as np
+    return dict(np.load(os.path.join(ROOT, "tests", "golden", "golden_mc2d.npz")))
