@@ -121,7 +121,7 @@ def ptr(a) -> ctypes.c_void_p:
     """Address of a numpy array or DeviceArray (None -> NULL)."""
     if a is None:
         return ctypes.c_void_p(0)
-    if isinstance(a, DeviceArray):
+    if isinstance(a, DeviceArray) or hasattr(a, "ptr"):
         return ctypes.c_void_p(a.ptr)
     if isinstance(a, np.ndarray):
         return ctypes.c_void_p(a.ctypes.data)

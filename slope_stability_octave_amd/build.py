@@ -24,6 +24,7 @@ FLAGS = [
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    "-munsafe-fp-atomics",
 ]
 
 

@@ -1,0 +1,163 @@
+"""Device-resident mirror of the reference's CONSTITUTIVE handle class
+(+CONSTITUTIVE_PROBLEM/CONSTITUTIVE.m).
+
+Same method names and meaning: ``reduction``, ``constitutive_problem_stress``,
+``constitutive_problem_stress_tangent``, ``build_F``, ``build_F_all``,
+``build_F_reduced``, ``build_F_and_DS_all``, ``build_F_and_DS_reduced``,
+``init_K_r_pattern``, ``build_K_tangent_QQ_vals``, ``build_K_r_QQ_vals``,
+``set_element_data``.  The strain-displacement matrix B is applied
+matrix-free from the element gradients (DPhi), so the constructor takes the
+element data instead of B.  Every array lives on the GPU; U, F and f are
+dim x n_n fields stored node-major (the layout of Octave's U(:)).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import assembly as A
+from . import constitutive_mex as CM
+from ._lib import DeviceArray
+
+
+class CONSTITUTIVE:
+    def __init__(self, elem, DPhi, c0, phi, psi, Davis_type, shear, bulk, lame, WEIGHT, n_strain, n_int, dim):
+        self.dim = int(dim)
+        self.n_strain = int(n_strain)
+        self.n_int = int(n_int)
+        self.Davis_type = Davis_type
+        self.elem = np.asarray(elem, dtype=np.int64)
+        self.n_p, self.n_e = self.elem.shape
+        self.n_q = self.n_int // self.n_e
+        self.n_n = int(self.elem.max()) + 1
+        self._c0, self._phi, self._psi = (np.asarray(x, dtype=np.float64) for x in (c0, phi, psi))
+        self.c0, self.phi, self.psi = (DeviceArray.from_host(x) for x in (self._c0, self._phi, self._psi))
+        self.shear, self.bulk, self.lame = (DeviceArray.from_host(np.asarray(x, dtype=np.float64))
+                                            for x in (shear, bulk, lame))
+        self.WEIGHT = DeviceArray.from_host(np.asarray(WEIGHT, dtype=np.float64).ravel())
+        # n_p x n_int, Octave column-major: the n_p values of a point contiguous
+        self.DPhi = [DeviceArray.from_host(np.asarray(d, dtype=np.float64).ravel(order="F")) for d in DPhi]
+        self.c_bar = DeviceArray(self.n_int)
+        self.sin_phi = DeviceArray(self.n_int)
+        self._lambda = DeviceArray(1)
+        ns = self.n_strain
+        self.s_rows = 6 if self.dim == 3 else 4
+        self.E = DeviceArray((self.n_int, ns))
+        self.S = DeviceArray((self.n_int, self.s_rows))
+        self.DS = DeviceArray((self.n_int, ns * ns))
+        self.pattern = None
+        self.pattern_initialized = False
+        self.profiler = None
+        self.timings = {}
+        self.counts = {}
+
+    def _t(self, name, t0):
+        self.timings[name] = self.timings.get(name, 0.0) + time.perf_counter() - t0
+        self.counts[name] = self.counts.get(name, 0) + 1
+
+    # -- material reduction (CONSTITUTIVE.m:178-200, reduction.m) -------------
+    def reduction(self, lam):
+        t0 = time.perf_counter()
+        c_all = True
+        if self.Davis_type == "C":
+            c_all = bool(np.all(np.arctan(np.tan(self._phi) / lam) > self._psi))
+        self._lambda.set(np.array([float(lam)]))
+        CM.reduction_dev(self.n_int, self.Davis_type, self.c0, self.phi, self.psi, self._lambda, False,
+                         self.c_bar, self.sin_phi, davis_c_all=c_all)
+        self._t("CONSTITUTIVE.reduction", t0)
+
+    # -- stress / tangent (CONSTITUTIVE.m:202-266) -----------------------------
+    def _strain(self, U):
+        self.pattern_or_raise().strain(self.n_q, self.DPhi, U, self.E)
+
+    def constitutive_problem_stress(self, U):
+        t0 = time.perf_counter()
+        self._strain(U)
+        CM.constitutive_dev(self.dim, self.n_int, self.E, self.c_bar, self.sin_phi, self.shear, self.bulk,
+                            self.lame, self.S, None)
+        self._t("CONSTITUTIVE.stress", t0)
+
+    def constitutive_problem_stress_tangent(self, U):
+        t0 = time.perf_counter()
+        self._strain(U)
+        CM.constitutive_dev(self.dim, self.n_int, self.E, self.c_bar, self.sin_phi, self.shear, self.bulk,
+                            self.lame, self.S, self.DS)
+        self._t("CONSTITUTIVE.stress_tangent", t0)
+
+    def build_F(self, F=None):
+        """F = B' * (WEIGHT .* S(1:n_strain,:)) (CONSTITUTIVE.m:268-285)."""
+        t0 = time.perf_counter()
+        if F is None:
+            F = DeviceArray((self.n_n, self.dim))
+        self.pattern_or_raise().force(self.n_q, self.DPhi, self.S, self.s_rows, self.WEIGHT, F)
+        self._t("CONSTITUTIVE.build_F", t0)
+        return F
+
+    def build_F_all(self, lam, U, F=None):
+        self.reduction(lam)
+        self.constitutive_problem_stress(U)
+        return self.build_F(F)
+
+    def build_F_reduced(self, U, F=None):
+        self.constitutive_problem_stress(U)
+        return self.build_F(F)
+
+    def build_F_and_DS_all(self, lam, U, F=None):
+        self.reduction(lam)
+        self.constitutive_problem_stress_tangent(U)
+        return self.build_F(F)
+
+    def build_F_and_DS_reduced(self, U, F=None):
+        self.constitutive_problem_stress_tangent(U)
+        return self.build_F(F)
+
+    # -- K_r(Q,Q) (CONSTITUTIVE.m:393-532) ------------------------------------
+    def pattern_or_raise(self):
+        if self.pattern is None:
+            raise RuntimeError("init_K_r_pattern(Q) must be called first")
+        return self.pattern
+
+    def init_K_r_pattern(self, Q):
+        """Node-block pattern over all nodes with the Dirichlet mask of Q and
+        the elastic values V_elast (CONSTITUTIVE.m:393-464)."""
+        t0 = time.perf_counter()
+        self.Q = np.asarray(Q, dtype=bool)
+        self.pattern = A.Pattern(self.elem, self.Q, self.dim)
+        P = self.pattern
+        self.mask = P.d_mask
+        ns = self.n_strain
+        DSe = DeviceArray((self.n_int, ns * ns))
+        A.ds_elastic_dev(self.dim, self.n_int, self.shear, self.bulk, DSe)
+        Kraw = DeviceArray(P.nnzb * P.bs)
+        P.assemble(self.n_q, self.DPhi, DSe, self.WEIGHT, Kraw)
+        # V_elast_QQ is the symmetrised elastic operator (CONSTITUTIVE.m:429)
+        self.K_elast = DeviceArray(P.nnzb * P.bs)
+        P.regularize(1.0, Kraw, Kraw, self.K_elast)
+        self.K_tang = DeviceArray(P.nnzb * P.bs)
+        self.K_r = DeviceArray(P.nnzb * P.bs)
+        self.n_Q = P.n_free
+        self.pattern_initialized = True
+        self._t("CONSTITUTIVE.init_K_r_pattern", t0)
+
+    def build_K_tangent_QQ_vals(self):
+        t0 = time.perf_counter()
+        P = self.pattern_or_raise()
+        P.assemble(self.n_q, self.DPhi, self.DS, self.WEIGHT, self.K_tang)
+        self._t("CONSTITUTIVE.build_K_tangent_QQ_vals", t0)
+        return self.K_tang
+
+    def build_K_r_QQ_vals(self, r):
+        """K_r = r K_elast + (1-r) K_tang, symmetrised, Dirichlet-masked; the
+        device block values (the pattern is fixed)."""
+        self.build_K_tangent_QQ_vals()
+        self.pattern.regularize(r, self.K_elast, self.K_tang, self.K_r)
+        return self.K_r
+
+    def build_K_r_QQ_vals_ref(self, r):
+        """(I, J, V) in the reference's order (1-based I, J over free DOFs)."""
+        K = self.build_K_r_QQ_vals(r)
+        P = self.pattern
+        V = DeviceArray(P.nnz_free)
+        P.extract_ref(K, V)
+        return V.get()

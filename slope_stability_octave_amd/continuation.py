@@ -1,0 +1,178 @@
+"""Indirect SSR continuation (the time-to-FoS harness).
+
+Mirrors +CONTINUATION/SSR_indirect_continuation.m:1-237 and
+init_phase_SSR_indirect_continuation.m:1-143: omega-stepping with the secant
+predictor U_ini = d_omega (U - U_old)/(omega - omega_old) + U (:119), retries
+with halved d_omega (:164-167), d_omega doubling when the lambda increment
+shrinks (:214-216) and the stop rules (:203-211, :220-227).  The factor of
+safety is lambda_hist[-1].
+
+``predictor`` (optional) replaces the secant extrapolation: a callable
+(history) -> U_ini or None; None falls back to the secant step.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from ._lib import DeviceArray
+from .linear_solvers import dot, lincomb
+from .newton import Workspace, newton, newton_ind_SSR
+
+
+class ContinuationError(RuntimeError):
+    pass
+
+
+def _umax(U):
+    u = U.get()
+    return float(np.sqrt((u * u).sum(axis=1)).max())
+
+
+def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max,
+                                         tol, r_min, K_elast, Q, f, cmb, solver, ws, log=None):
+    if not cmb.pattern_initialized:
+        cmb.init_K_r_pattern(Q)
+    mask = cmb.mask
+    U_ini = DeviceArray(f.shape, zero=True)
+    if log:
+        log(f"Init step 1: lambda={lambda_init:.6g}")
+    cmb.reduction(lambda_init)
+    its = []
+    U_it, flag_N, it_newt, _ = newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver,
+                                      ws, log)
+    its.append(it_newt)
+    if flag_N == 1:
+        raise ContinuationError("Initial choice of lambda seems to be too large.")
+    U1 = U_it
+    omega1 = dot(f, U1, mask)
+    d_lambda = d_lambda_init
+    lambda1 = lambda_init
+    solver.expand_deflation_basis(U_it)
+    while True:
+        lambda_it = lambda1 + d_lambda
+        cmb.reduction(lambda_it)
+        U_it, flag_N, it_newt, _ = newton(U1, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver,
+                                          ws, log)
+        its.append(it_newt)
+        if flag_N == 1:
+            d_lambda = d_lambda / 2
+        else:
+            U2 = U_it
+            omega2 = dot(f, U2, mask)
+            lambda2 = lambda_it
+            if (omega2 - omega1) / max(1.0, omega1) < 1e-5:
+                U1, lambda1, omega1 = U2, lambda2, omega2
+            else:
+                break
+        if d_lambda < d_lambda_min:
+            raise ContinuationError("It seems that the FoS is equal to lambda_init.")
+        if lambda1 > 10:
+            raise ContinuationError("It seems that the FoS is greater than 10.")
+    if log:
+        log(f"   lambda1 = {lambda1:.6g}, lambda2 = {lambda2:.6g}, omega1 = {omega1:.6g}, omega2 = {omega2:.6g}")
+    return U1, U2, omega1, omega2, lambda1, lambda2, its
+
+
+def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
+                              omega_max_stop, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
+                              linear_system_solver, log=None, predictor=None):
+    """Returns (U, lambda_hist, omega_hist, Umax_hist, stats)."""
+    t_total = time.perf_counter()
+    ws = Workspace(cmb.n_n, cmb.dim)
+    mask_ready = cmb.pattern_initialized or cmb.init_K_r_pattern(Q) or True  # noqa: F841
+    mask = cmb.mask
+    stats = dict(attempt_step=[], attempt_success=[], attempt_wall_time=[], attempt_newton_iterations=[],
+                 attempt_omega_target=[], attempt_lambda_before=[], attempt_lambda_after=[],
+                 step_lambda=[], step_omega=[], step_newton_iterations=[], newton_histories=[],
+                 predictor_used=[])
+    U_old, U, omega_old, omega, lambda_old, lam, init_its = init_phase_SSR_indirect_continuation(
+        lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
+        linear_system_solver.copy(), ws, log)
+    stats["init_newton_iterations"] = init_its
+    linear_system_solver.expand_deflation_basis(U_old)
+    lambda_hist = [lambda_old, lam]
+    omega_hist = [omega_old, omega]
+    Umax_hist = [_umax(U_old), _umax(U)]
+    d_omega = omega - omega_old
+    if omega_max_stop < omega + d_omega:
+        raise ContinuationError("Too small value of omega_max_stop. It is necessary to increase this input quantinty.")
+    step = 2
+    n_omega, n_omega_max = 0, 5
+    U_ini = DeviceArray(U.shape)
+    history = dict(lambda_hist=lambda_hist, omega_hist=omega_hist)
+    while True:
+        omega_it = min(omega + d_omega, omega_max_stop)
+        d_omega = omega_it - omega
+        if log:
+            log(f"Step {step + 1}: omega_target={omega_it:.6g}, d_omega={d_omega:.6g}")
+        used = False
+        if predictor is not None:
+            guess = predictor(dict(U=U, U_old=U_old, omega=omega, omega_old=omega_old, omega_it=omega_it,
+                                   lam=lam, history=history))
+            if guess is not None:
+                lincomb(1.0, guess, 0.0, guess, U_ini)
+                used = True
+        if not used:
+            a = d_omega / (omega - omega_old)
+            # U_ini = a (U - U_old) + U
+            lincomb(1.0 + a, U, -a, U_old, U_ini)
+        stats["predictor_used"].append(used)
+        t0 = time.perf_counter()
+        U_it, lambda_it, flag, it_newt, hist = newton_ind_SSR(
+            U_ini, omega_it, lam, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
+            linear_system_solver.copy(), ws, log)
+        wall = time.perf_counter() - t0
+        stats["attempt_step"].append(step + 1)
+        stats["attempt_success"].append(flag == 0)
+        stats["attempt_wall_time"].append(wall)
+        stats["attempt_newton_iterations"].append(it_newt)
+        stats["attempt_omega_target"].append(omega_it)
+        stats["attempt_lambda_before"].append(lam)
+        stats["attempt_lambda_after"].append(lambda_it if flag == 0 else float("nan"))
+        stats["newton_histories"].append(hist)
+        if flag == 1:
+            if log:
+                log(f"  retry: reducing d_omega from {d_omega:.6g} to {d_omega / 2:.6g}")
+            d_omega = d_omega / 2
+            n_omega += 1
+        else:
+            step += 1
+            U_old, U = U, U_it
+            linear_system_solver.expand_deflation_basis(U)
+            omega_old, omega = omega, omega_it
+            d_lambda = lambda_it - lam
+            lam = lambda_it
+            n_omega = 0
+            omega_hist.append(omega)
+            lambda_hist.append(lam)
+            Umax_hist.append(_umax(U))
+            stats["step_lambda"].append(lam)
+            stats["step_omega"].append(omega)
+            stats["step_newton_iterations"].append(it_newt)
+            scaled = d_lambda / d_omega * (omega_hist[-1] - omega_hist[0])
+            if log:
+                log(f"  accepted: lambda={lam:.6g}, d_lambda={d_lambda:.6g}, d_lambda_diff_scaled={scaled:.6g}, "
+                    f"omega={omega:.6g}, d_omega={d_omega:.6g}, U_max={Umax_hist[-1]:.6g}")
+            if scaled < d_lambda_diff_scaled_min:
+                if log:
+                    log("Stop: minimal increase of lambda achieved.")
+                break
+            if omega >= omega_max_stop:
+                if log:
+                    log("Stop: maximal omega achieved.")
+                break
+            if (lambda_hist[-1] - lambda_hist[-2]) < 0.9 * (lambda_hist[-2] - lambda_hist[-3]):
+                d_omega = 2 * d_omega
+        if n_omega >= n_omega_max:
+            if log:
+                log("Stop: impossible to increase omega.")
+            break
+        if step >= step_max:
+            if log:
+                log("Stop: maximal number of steps achieved.")
+            break
+    stats["total_wall_time"] = time.perf_counter() - t_total
+    return U, np.array(lambda_hist), np.array(omega_hist), np.array(Umax_hist), stats

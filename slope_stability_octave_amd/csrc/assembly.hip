@@ -1,0 +1,497 @@
+// Element-level operators for P2 elements on gfx950:
+//   strain gather   E = B U                 (CONSTITUTIVE.m:218,251)
+//   internal force  F = B' (WEIGHT .* S)    (CONSTITUTIVE.m:281)
+//   tangent values  sum_q B_q'(w_q D_q)B_q  (assemble_K_tangent_vals.c:85-171,
+//                                            assemble_K_tangent_vals_2D.c:88-160)
+//   K_r = r K_el + (1-r) K_t, symmetrised  (CONSTITUTIVE.m:515-532)
+//
+// B is never formed: each integration point carries the physical gradients of
+// the n_p basis functions (DPhi1..3, n_p x n_int, as produced by
+// elastic_stiffness_matrix_{2D,3D}.m), and the Voigt rows of B are applied
+// structurally (3 nonzeros per column in 3D, 2 in 2D).
+//
+// Tangent kernel: one 64-lane wavefront per element (3D; three elements per
+// wavefront in 2D).  The element's gradients, w*D for all points and, per
+// point, the products D*B_b of every node are staged in LDS; each lane owns
+// one node pair (a,b) with a <= b (55 pairs for a 10-node tetrahedron) and
+// accumulates its dim x dim block over the points.  Ke is symmetric, so the
+// lane adds its block at (a,b) and its transpose at (b,a).  Accumulation into
+// the global values uses hardware FP64 atomics.
+#include "ssr_common.h"
+
+#include <algorithm>
+
+struct ssr_pattern;
+const int32_t *ssr_pattern_d_row_ptr(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_col(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_emap(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_diag(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_elem(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_brow(const ssr_pattern *p);
+const int64_t *ssr_pattern_d_ref_pos(const ssr_pattern *p);
+const double *ssr_pattern_d_mask(const ssr_pattern *p);
+void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e,
+                       int64_t *nnzb, int64_t *nnz_free);
+
+namespace {
+
+template <int DIM> struct Voigt;
+// 3D: strain rows (e11,e22,e33,g12,g23,g13); column of node displacement
+// component c has entries (row, gradient component):
+//   c=x: (0,x) (3,y) (5,z);  c=y: (1,y) (3,x) (4,z);  c=z: (2,z) (4,y) (5,x)
+template <> struct Voigt<3> {
+    static constexpr int NS = 6, NNZ = 3;
+    __device__ static constexpr int row(int c, int k)
+    {
+        return c == 0 ? (k == 0 ? 0 : (k == 1 ? 3 : 5)) : c == 1 ? (k == 0 ? 1 : (k == 1 ? 3 : 4))
+                                                                  : (k == 0 ? 2 : (k == 1 ? 4 : 5));
+    }
+    __device__ static constexpr int grad(int c, int k)
+    {
+        return c == 0 ? (k == 0 ? 0 : (k == 1 ? 1 : 2)) : c == 1 ? (k == 0 ? 1 : (k == 1 ? 0 : 2))
+                                                                  : (k == 0 ? 2 : (k == 1 ? 1 : 0));
+    }
+};
+// 2D: rows (e11,e22,g12); c=x: (0,x) (2,y); c=y: (1,y) (2,x)
+template <> struct Voigt<2> {
+    static constexpr int NS = 3, NNZ = 2;
+    __device__ static constexpr int row(int c, int k) { return c == 0 ? (k == 0 ? 0 : 2) : (k == 0 ? 1 : 2); }
+    __device__ static constexpr int grad(int c, int k) { return c == 0 ? (k == 0 ? 0 : 1) : (k == 0 ? 1 : 0); }
+};
+
+__device__ __forceinline__ int symidx(int i, int j, int n)
+{
+    if (i > j) { int t = i; i = j; j = t; }
+    return i * n - (i * (i - 1)) / 2 + (j - i);
+}
+
+__device__ __forceinline__ void atomic_add_f64(double *p, double v)
+{
+    // hardware global_atomic_add_f64 (built with -munsafe-fp-atomics)
+    atomicAdd(p, v);
+}
+
+// ---------------------------------------------------------------------------
+// Tangent assembly
+// ---------------------------------------------------------------------------
+// DSL: 0 = MEX layout (ns*ns column-major per point), 1 = packed symmetric.
+// SCAT: 0 = node-block CSR via emap, 1 = reference int64 scatter map.
+template <int DIM, int NP, int NQ, int DSL, int SCAT>
+__global__ __launch_bounds__(64) void k_assemble(
+    int64_t n_e, const double *__restrict__ D1, const double *__restrict__ D2,
+    const double *__restrict__ D3, const double *__restrict__ DS, const double *__restrict__ W,
+    const int32_t *__restrict__ emap, const int64_t *__restrict__ smap, double *__restrict__ out)
+{
+    using V = Voigt<DIM>;
+    constexpr int NS = V::NS, NSYM = NS * (NS + 1) / 2, NPAIR = NP * (NP + 1) / 2;
+    constexpr int EPB = 64 / NPAIR;  // elements per wavefront
+    constexpr int NLD = DIM * NP;
+    __shared__ double sG[EPB][NQ][NP][DIM];
+    __shared__ double sD[EPB][NQ][NSYM];
+    __shared__ double sDB[EPB][NP][NS][DIM];
+
+    const int lane = threadIdx.x;
+    const int le = lane / NPAIR;          // element slot of this lane
+    const int pr = lane - le * NPAIR;     // pair index
+    // pair -> (a,b), a <= b, row-major over the upper triangle
+    int pa = 0, pb = 0;
+    {
+        int k = pr, a = 0;
+        while (k >= NP - a) { k -= NP - a; ++a; }
+        pa = a; pb = a + k;
+    }
+    const bool active = le < EPB;
+
+    for (int64_t e0 = (int64_t)blockIdx.x * EPB; e0 < n_e; e0 += (int64_t)gridDim.x * EPB) {
+        // stage gradients and weighted tangents
+        for (int t = lane; t < EPB * NQ * NP; t += 64) {
+            const int s = t / (NQ * NP), r = t - s * NQ * NP;
+            const int64_t e = e0 + s;
+            if (e < n_e) {
+                const int64_t g = e * NQ * NP + r;  // (point, node) flattened
+                const int q = r / NP, a = r - q * NP;
+                sG[s][q][a][0] = D1[g];
+                sG[s][q][a][1] = D2[g];
+                if (DIM == 3) sG[s][q][a][DIM - 1] = D3[g];
+            }
+        }
+        for (int t = lane; t < EPB * NQ * NSYM; t += 64) {
+            const int s = t / (NQ * NSYM), r = t - s * NQ * NSYM;
+            const int64_t e = e0 + s;
+            if (e < n_e) {
+                const int q = r / NSYM, k = r - q * NSYM;
+                // packed index k -> (i,j), i <= j
+                int i = 0, kk = k;
+                while (kk >= NS - i) { kk -= NS - i; ++i; }
+                const int j = i + kk;
+                const int64_t g = e * NQ + q;
+                const double w = W[g];
+                const double d = DSL == 0 ? DS[g * NS * NS + i + NS * j] : DS[g * NSYM + k];
+                sD[s][q][k] = w * d;
+            }
+        }
+        __syncthreads();
+
+        double acc[DIM][DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) acc[i][j] = 0.0;
+
+        for (int q = 0; q < NQ; ++q) {
+            // DB_b = (wD) B_b for every node b of every element slot
+            for (int t = lane; t < EPB * NP * NS * DIM; t += 64) {
+                const int s = t / (NP * NS * DIM);
+                int r = t - s * NP * NS * DIM;
+                const int b = r / (NS * DIM);
+                r -= b * NS * DIM;
+                const int row = r / DIM, c = r - row * DIM;
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < V::NNZ; ++k)
+                    v += sD[s][q][symidx(row, V::row(c, k), NS)] * sG[s][q][b][V::grad(c, k)];
+                sDB[s][b][row][c] = v;
+            }
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < V::NNZ; ++k)
+                            v += sG[le][q][pa][V::grad(i, k)] * sDB[le][pb][V::row(i, k)][j];
+                        acc[i][j] += v;
+                    }
+            }
+            __syncthreads();
+        }
+
+        const int64_t e = e0 + le;
+        if (active && e < n_e) {
+            if (SCAT == 0) {
+                const int32_t bab = emap[(e * NP + pa) * NP + pb];
+                double *o = out + (int64_t)bab * DIM * DIM;
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) atomic_add_f64(o + i * DIM + j, acc[i][j]);
+                if (pa != pb) {
+                    const int32_t bba = emap[(e * NP + pb) * NP + pa];
+                    double *o2 = out + (int64_t)bba * DIM * DIM;
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                        for (int j = 0; j < DIM; ++j) atomic_add_f64(o2 + j * DIM + i, acc[i][j]);
+                }
+            } else {
+                const int64_t *m = smap + e * (int64_t)NLD * NLD;
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) {
+                        const int ra = DIM * pa + i, cb = DIM * pb + j;
+                        const int64_t k1 = m[ra + NLD * cb];
+                        if (k1 > 0) atomic_add_f64(out + k1 - 1, acc[i][j]);
+                        if (pa != pb) {
+                            const int64_t k2 = m[cb + NLD * ra];
+                            if (k2 > 0) atomic_add_f64(out + k2 - 1, acc[i][j]);
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Strain gather: one thread per integration point.  U is dim x n_n
+// (node-major), E is point-major (ns per point).
+// ---------------------------------------------------------------------------
+template <int DIM, int NP, int NQ>
+__global__ __launch_bounds__(256) void k_strain(int64_t n_e, const int32_t *__restrict__ elem,
+                                                const double *__restrict__ D1,
+                                                const double *__restrict__ D2,
+                                                const double *__restrict__ D3,
+                                                const double *__restrict__ U, double *__restrict__ E)
+{
+    using V = Voigt<DIM>;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_e * NQ) return;
+    const int64_t e = g / NQ;
+    double eps[V::NS];
+#pragma unroll
+    for (int r = 0; r < V::NS; ++r) eps[r] = 0.0;
+    for (int a = 0; a < NP; ++a) {
+        const int64_t node = elem[e * NP + a];
+        double gr[DIM];
+        gr[0] = D1[g * NP + a];
+        gr[1] = D2[g * NP + a];
+        if (DIM == 3) gr[DIM - 1] = D3[g * NP + a];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            const double u = U[node * DIM + c];
+#pragma unroll
+            for (int k = 0; k < V::NNZ; ++k) eps[V::row(c, k)] += gr[V::grad(c, k)] * u;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < V::NS; ++r) E[g * V::NS + r] = eps[r];
+}
+
+// Internal force: one thread per element, F (dim x n_n) accumulated with
+// atomics; S has s_rows rows per point (6 in 3D; 4 in 2D where sigma33 is
+// not used), F must be zeroed first.
+template <int DIM, int NP, int NQ>
+__global__ __launch_bounds__(128) void k_force(int64_t n_e, const int32_t *__restrict__ elem,
+                                               const double *__restrict__ D1,
+                                               const double *__restrict__ D2,
+                                               const double *__restrict__ D3,
+                                               const double *__restrict__ S, int s_rows,
+                                               const double *__restrict__ W, double *__restrict__ F)
+{
+    using V = Voigt<DIM>;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_e) return;
+    double f[NP][DIM];
+#pragma unroll
+    for (int a = 0; a < NP; ++a)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) f[a][c] = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+        const int64_t g = e * NQ + q;
+        double ws[V::NS];
+#pragma unroll
+        for (int r = 0; r < V::NS; ++r) ws[r] = W[g] * S[g * s_rows + r];
+#pragma unroll
+        for (int a = 0; a < NP; ++a) {
+            double gr[DIM];
+            gr[0] = D1[g * NP + a];
+            gr[1] = D2[g * NP + a];
+            if (DIM == 3) gr[DIM - 1] = D3[g * NP + a];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < V::NNZ; ++k) v += gr[V::grad(c, k)] * ws[V::row(c, k)];
+                f[a][c] += v;
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+        const int64_t node = elem[e * NP + a];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) atomic_add_f64(F + node * DIM + c, f[a][c]);
+    }
+}
+
+// K_r = r K_el + (1-r) K_t, symmetrised through the transpose block, then
+// Dirichlet rows/columns masked (unit diagonal).  One thread per block entry.
+template <int DIM>
+__global__ void k_regularize(int64_t nnzb, const int32_t *__restrict__ brow,
+                             const int32_t *__restrict__ col, const int32_t *__restrict__ tblk,
+                             const double *__restrict__ mask, double r,
+                             const double *__restrict__ Kel, const double *__restrict__ Kt,
+                             double *__restrict__ Kr)
+{
+    constexpr int BS = DIM * DIM;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnzb * BS) return;
+    const int64_t b = t / BS;
+    const int ij = (int)(t - b * BS), i = ij / DIM, j = ij - (ij / DIM) * DIM;
+    const int64_t bt = tblk[b];
+    const int64_t tt = bt * BS + j * DIM + i;
+    const double v1 = r * Kel[t] + (1.0 - r) * Kt[t];
+    const double v2 = r * Kel[tt] + (1.0 - r) * Kt[tt];
+    double v = 0.5 * (v1 + v2);
+    const int64_t ro = (int64_t)brow[b] * DIM + i, co = (int64_t)col[b] * DIM + j;
+    if (mask[ro] == 0.0 || mask[co] == 0.0) v = (ro == co) ? 1.0 : 0.0;
+    Kr[t] = v;
+}
+
+__global__ void k_gather_ref(int64_t nnz, const int64_t *__restrict__ pos, const double *__restrict__ src,
+                             double *__restrict__ dst)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) dst[k] = src[pos[k]];
+}
+
+// Elastic DS per point: 2*DEV*shear + VOL*bulk with the operators of
+// CONSTITUTIVE.m:413-422, MEX layout.  There IOTA = [1;1;1;zeros(n_strain-3,1)],
+// so in 2D (n_strain = 3) VOL couples all three strain rows, shear included;
+// that operator is what the reference regularises with, and it is kept.
+__global__ void k_ds_elastic(int64_t n, int ns, const double *__restrict__ shear,
+                             const double *__restrict__ bulk, double *__restrict__ DS)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * ns * ns) return;
+    const int64_t p = t / (ns * ns);
+    const int k = (int)(t - p * ns * ns), i = k % ns, j = k / ns;
+    const int nn = ns == 6 ? 3 : 2;  // normal components of the Voigt identity
+    const double vol = (i < 3 && j < 3) ? 1.0 : 0.0;
+    const double idv = (i == j) ? (i < nn ? 1.0 : 0.5) : 0.0;
+    const double dev = idv - vol / 3.0;
+    DS[t] = 2.0 * dev * shear[p] + vol * bulk[p];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Entry points
+// ---------------------------------------------------------------------------
+static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, int scat,
+                           const double *D1, const double *D2, const double *D3, const double *DS,
+                           const double *W, const int32_t *emap, const int64_t *smap, double *out)
+{
+    if (n_e == 0) return SSR_OK;
+    const int64_t want = 4096;  // resident wavefronts; grid-stride beyond
+    if (dim == 3) {
+        SSR_REQUIRE(n_p == 10 && n_q == 11, "tangent assembly (3D): P2 tetrahedra with 11 points supported (got n_p=%lld, n_q=%d)",
+                    (long long)n_p, n_q);
+        const unsigned grid = (unsigned)std::min<int64_t>(n_e, want * 16);
+#define L3(DSL_, SC_) hipLaunchKernelGGL((k_assemble<3, 10, 11, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
+                                         n_e, D1, D2, D3, DS, W, emap, smap, out)
+        if (dsl == 0 && scat == 0) L3(0, 0);
+        else if (dsl == 0) L3(0, 1);
+        else if (scat == 0) L3(1, 0);
+        else L3(1, 1);
+#undef L3
+    } else {
+        SSR_REQUIRE(n_p == 6 && n_q == 7, "tangent assembly (2D): P2 triangles with 7 points supported (got n_p=%lld, n_q=%d)",
+                    (long long)n_p, n_q);
+        const int64_t groups = (n_e + 2) / 3;
+        const unsigned grid = (unsigned)std::min<int64_t>(groups, want * 16);
+#define L2(DSL_, SC_) hipLaunchKernelGGL((k_assemble<2, 6, 7, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
+                                         n_e, D1, D2, D3, DS, W, emap, smap, out)
+        if (dsl == 0 && scat == 0) L2(0, 0);
+        else if (dsl == 0) L2(0, 1);
+        else if (scat == 0) L2(1, 0);
+        else L2(1, 1);
+#undef L2
+    }
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+extern "C" {
+
+// Reference-layout tangent values on device pointers: V (nnz) is zeroed and
+// filled through the int64 scatter map (1-based, 0 = skip).
+int ssr_dev_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, int64_t n_e, int n_q,
+                                    const double *DPhi1, const double *DPhi2, const double *DPhi3,
+                                    const double *DS, const double *WEIGHT, const int64_t *smap,
+                                    int64_t nnz, double *V)
+{
+    SSR_REQUIRE(dim == 2 || dim == 3, "assemble_K_tangent_vals: dim must be 2 or 3");
+    SSR_REQUIRE(n_e * n_q == n_int, "assemble_K_tangent_vals: n_int (%lld) != n_e*n_q (%lld)",
+                (long long)n_int, (long long)(n_e * n_q));
+    SSR_REQUIRE(nnz >= 0, "assemble_K_tangent_vals: nnz_out must be >= 0");
+    if (nnz) SSR_HIP_CHECK(hipMemsetAsync(V, 0, sizeof(double) * nnz, ssr_stream()));
+    return assemble_launch(dim, n_p, n_e, n_q, 0, 1, DPhi1, DPhi2, DPhi3, DS, WEIGHT, nullptr, smap, V);
+}
+
+// Node-block values of sum_e Ke on the pattern (zeroed first).  ds_layout:
+// 0 = MEX layout DS (ns*ns per point), 1 = packed symmetric (ns(ns+1)/2).
+int ssr_dev_assemble_bsr(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                         const double *DPhi3, const double *DS, int ds_layout, const double *WEIGHT,
+                         double *Kvals)
+{
+    SSR_REQUIRE(P, "assemble_bsr: null pattern");
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    SSR_REQUIRE(ssr_pattern_d_emap(P), "assemble_bsr: pattern has no device tables");
+    SSR_HIP_CHECK(hipMemsetAsync(Kvals, 0, sizeof(double) * nnzb * dim * dim, ssr_stream()));
+    return assemble_launch(dim, n_p, n_e, n_q, ds_layout, 0, DPhi1, DPhi2, DPhi3, DS, WEIGHT,
+                           ssr_pattern_d_emap(P), nullptr, Kvals);
+}
+
+int ssr_dev_strain(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                   const double *DPhi3, const double *U, double *E)
+{
+    SSR_REQUIRE(P, "strain: null pattern");
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    const int64_t n = n_e * n_q;
+    const int th = 256;
+    if (dim == 3) {
+        SSR_REQUIRE(n_p == 10 && n_q == 11, "strain: P2 tetrahedra with 11 points supported");
+        hipLaunchKernelGGL((k_strain<3, 10, 11>), dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n_e,
+                           ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, U, E);
+    } else {
+        SSR_REQUIRE(n_p == 6 && n_q == 7, "strain: P2 triangles with 7 points supported");
+        hipLaunchKernelGGL((k_strain<2, 6, 7>), dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n_e,
+                           ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, U, E);
+    }
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+int ssr_dev_force(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                  const double *DPhi3, const double *S, int s_rows, const double *WEIGHT, double *F)
+{
+    SSR_REQUIRE(P, "force: null pattern");
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    SSR_HIP_CHECK(hipMemsetAsync(F, 0, sizeof(double) * dim * n_n, ssr_stream()));
+    const int th = 128;
+    if (dim == 3) {
+        SSR_REQUIRE(n_p == 10 && n_q == 11 && s_rows >= 6, "force: P2 tetrahedra with 11 points supported");
+        hipLaunchKernelGGL((k_force<3, 10, 11>), dim3(ssr_blocks(n_e, th)), dim3(th), 0, ssr_stream(), n_e,
+                           ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, S, s_rows, WEIGHT, F);
+    } else {
+        SSR_REQUIRE(n_p == 6 && n_q == 7 && s_rows >= 3, "force: P2 triangles with 7 points supported");
+        hipLaunchKernelGGL((k_force<2, 6, 7>), dim3(ssr_blocks(n_e, th)), dim3(th), 0, ssr_stream(), n_e,
+                           ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, S, s_rows, WEIGHT, F);
+    }
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+int ssr_dev_regularize(const ssr_pattern *P, double r, const double *Kel, const double *Kt, double *Kr)
+{
+    const int32_t *brow = ssr_pattern_d_brow(P);
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    const int64_t n = nnzb * dim * dim;
+    if (dim == 3)
+        hipLaunchKernelGGL(k_regularize<3>, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), nnzb, brow,
+                           ssr_pattern_d_col(P), ssr_pattern_d_tblk(P), ssr_pattern_d_mask(P), r, Kel, Kt, Kr);
+    else
+        hipLaunchKernelGGL(k_regularize<2>, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), nnzb, brow,
+                           ssr_pattern_d_col(P), ssr_pattern_d_tblk(P), ssr_pattern_d_mask(P), r, Kel, Kt, Kr);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+// V_ref (nnz_free) = block values gathered into the reference's order.
+int ssr_dev_extract_ref(const ssr_pattern *P, const double *Kvals, double *Vref)
+{
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    if (nnzf == 0) return SSR_OK;
+    hipLaunchKernelGGL(k_gather_ref, dim3(ssr_blocks(nnzf, 256)), dim3(256), 0, ssr_stream(), nnzf,
+                       ssr_pattern_d_ref_pos(P), Kvals, Vref);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+int ssr_dev_ds_elastic(int dim, int64_t n_int, const double *shear, const double *bulk, double *DS)
+{
+    const int ns = dim == 3 ? 6 : 3;
+    const int64_t n = n_int * ns * ns;
+    if (n == 0) return SSR_OK;
+    hipLaunchKernelGGL(k_ds_elastic, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n_int, ns, shear,
+                       bulk, DS);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+}  // extern "C"

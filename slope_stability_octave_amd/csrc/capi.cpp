@@ -212,3 +212,41 @@ int ssr_constitutive_problem_2D_S(int64_t n_int, int64_t rows_E, const double *E
 }
 
 }  // extern "C"
+
+extern "C" int ssr_dev_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, int64_t n_e, int n_q,
+                                               const double *DPhi1, const double *DPhi2,
+                                               const double *DPhi3, const double *DS,
+                                               const double *WEIGHT, const int64_t *smap, int64_t nnz,
+                                               double *V);
+
+// Host-pointer drop-in for assemble_K_tangent_vals{,_2D} (MEX boundary).
+extern "C" int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, int64_t n_e, int n_q,
+                                           const double *DPhi1, const double *DPhi2, const double *DPhi3,
+                                           const double *DS, const double *WEIGHT,
+                                           const int64_t *scatter_map, int64_t nnz_out, double *V)
+{
+    SSR_REQUIRE(dim == 2 || dim == 3, "assemble_K_tangent_vals: dim must be 2 or 3");
+    SSR_REQUIRE(scatter_map, "assemble_K_tangent_vals: scatter_map must be int64.");
+    SSR_REQUIRE(n_e * n_q == n_int, "assemble_K_tangent_vals: n_int != n_e*n_q");
+    if (nnz_out == 0) return SSR_OK;
+    SSR_REQUIRE(V && DPhi1 && DPhi2 && DS && WEIGHT && (dim == 2 || DPhi3), "assemble_K_tangent_vals: null argument");
+    const int ns = dim == 3 ? 6 : 3, nld = dim * (int)n_p;
+    DevBuf dP, dD, dW, dM, dV;
+    int rc;
+    if ((rc = dP.alloc(sizeof(double) * dim * n_p * n_int))) return rc;
+    if ((rc = dD.alloc(sizeof(double) * ns * ns * n_int))) return rc;
+    if ((rc = dW.alloc(sizeof(double) * n_int))) return rc;
+    if ((rc = dM.alloc(sizeof(int64_t) * nld * nld * n_e))) return rc;
+    if ((rc = dV.alloc(sizeof(double) * nnz_out))) return rc;
+    const double *src[3] = {DPhi1, DPhi2, DPhi3};
+    for (int k = 0; k < dim; ++k)
+        if ((rc = ssr_htod(dP.d() + k * n_p * n_int, src[k], sizeof(double) * n_p * n_int))) return rc;
+    if ((rc = ssr_htod(dD.p, DS, sizeof(double) * ns * ns * n_int))) return rc;
+    if ((rc = ssr_htod(dW.p, WEIGHT, sizeof(double) * n_int))) return rc;
+    if ((rc = ssr_htod(dM.p, scatter_map, sizeof(int64_t) * nld * nld * n_e))) return rc;
+    rc = ssr_dev_assemble_K_tangent_vals(dim, n_p, n_int, n_e, n_q, dP.d(), dP.d() + n_p * n_int,
+                                         dim == 3 ? dP.d() + 2 * n_p * n_int : nullptr, dD.d(), dW.d(),
+                                         (const int64_t *)dM.p, nnz_out, dV.d());
+    if (rc) return rc;
+    return ssr_dtoh(V, dV.p, sizeof(double) * nnz_out);
+}

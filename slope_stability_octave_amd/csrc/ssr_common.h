@@ -40,3 +40,11 @@ static inline unsigned ssr_blocks(int64_t n, int threads)
 }
 
 hipStream_t ssr_stream(void);
+
+// Preconditioner interface used by the Krylov solvers (solver.cpp): block
+// Jacobi and smoothed-aggregation AMG implement it.
+struct ssr_precond {
+    virtual ~ssr_precond() {}
+    virtual int setup(const double *Kvals) = 0;       // operator values (device)
+    virtual int apply(const double *x, double *y) = 0;  // y = M^{-1} x (device)
+};

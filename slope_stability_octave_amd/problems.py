@@ -1,0 +1,100 @@
+"""Problem setup and the time-to-FoS driver on the GPU.
+
+``slope_2d`` / ``slope_3d`` build the homogeneous or layered slope problems of
+the reference drivers (slope_stability_2D_homo_SSR.m:27-110,
+slope_stability_3D_homo_SSR.m:27-110 with the structured generator):
+P2 mesh, gradients, per-point materials (heterogenous_materials.m), gravity
+load f_V = [0; -gamma(; 0)] (vector_volume_{2D,3D}.m).
+
+``run_ssr`` runs the indirect continuation on the GPU with the DFGMRES
+(or DCG) solver and returns the factor of safety, histories and timings.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import fem, mesh
+from ._lib import DeviceArray
+
+# continuation / Newton / linear-solver parameters of slope_stability_2D_homo_SSR.m:114-137
+PARAMS_2D = dict(lambda_init=0.9, d_lambda_init=0.05, d_lambda_min=1e-5, d_lambda_diff_scaled_min=0.001,
+                 omega_max_stop=7e7, step_max=100, it_newt_max=30, it_damp_max=10, tol=1e-4, r_min=1e-4,
+                 linear_tol=1e-1, linear_maxit=100, deflation_tol=1e-3)
+# slope_stability_3D_homo_SSR.m:118-140
+PARAMS_3D = dict(lambda_init=0.9, d_lambda_init=0.1, d_lambda_min=1e-3, d_lambda_diff_scaled_min=0.001,
+                 omega_max_stop=7e7, step_max=100, it_newt_max=100, it_damp_max=10, tol=1e-4, r_min=1e-4,
+                 linear_tol=1e-1, linear_maxit=100, deflation_tol=1e-3)
+
+HOMO_2D = [dict(c0=6, phi=45, psi=0, young=40000, poisson=0.3, gamma_sat=20, gamma_unsat=20)]
+
+
+def _finish(coord, elem, Q, dim, mats, mat_id):
+    DPhi, WEIGHT, HatP, WF = fem.gradients(coord, elem, dim)
+    n_q = WF.size
+    m = fem.materials_per_point(mat_id, n_q, mats)
+    n_int = WEIGHT.size
+    f_int = np.zeros((dim, n_int))
+    f_int[1] = -m["gamma"]
+    f = fem.volume_force(elem, coord.shape[1], HatP, WEIGHT, f_int)
+    return dict(coord=coord, elem=elem, Q=Q, dim=dim, DPhi=DPhi, WEIGHT=WEIGHT, HatP=HatP, n_q=n_q,
+                mats=m, f=f, n_int=n_int, n_dof_free=int(Q.sum()))
+
+
+def slope_2d(h=0.5, x1=15.0, x2=10.0, x3=15.0, y1=10.0, y2=10.0, materials=None):
+    """2D homogeneous slope (10 m foundation, 10 m high 1:1 slope, 15 m crest
+    and toe berms by default)."""
+    coord, elem, Q = mesh.mesh_P2_2D(h, x1, x2, x3, y1, y2)
+    mats = materials or HOMO_2D
+    return _finish(coord, elem, Q, 2, mats, np.zeros(elem.shape[1], dtype=np.int64))
+
+
+def slope_3d(counts, x1, x2, x3, y1, y2, z, materials, layers=None):
+    """3D slope from mesh_P2_3D with cell counts (nx1, nx2, nx3, ny1, ny2, nz).
+    ``layers``: list of y-levels splitting the body into horizontal layers
+    (material id 0 at the bottom), assigned by element centroid."""
+    coord, elem, Q = mesh.mesh_P2_3D(*counts, x1, x2, x3, y1, y2, z)
+    mat_id = np.zeros(elem.shape[1], dtype=np.int64)
+    if layers:
+        yc = coord[1, elem[:4]].mean(axis=0)
+        for lv in sorted(layers):
+            mat_id += (yc > lv)
+    return _finish(coord, elem, Q, 3, materials, mat_id)
+
+
+def make_constitutive(prob, Davis_type="B"):
+    from .constitutive import CONSTITUTIVE
+    m = prob["mats"]
+    dim = prob["dim"]
+    cmb = CONSTITUTIVE(prob["elem"], prob["DPhi"], m["c0"], m["phi"], m["psi"], Davis_type, m["shear"], m["bulk"],
+                       m["lame"], prob["WEIGHT"], 6 if dim == 3 else 3, prob["n_int"], dim)
+    cmb.init_K_r_pattern(prob["Q"])
+    return cmb
+
+
+def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", log=None, predictor=None,
+            amg_options=None):
+    from .continuation import SSR_indirect_continuation
+    from .linear_solvers import DCG, DFGMRES
+    p = dict(PARAMS_2D if prob["dim"] == 2 else PARAMS_3D)
+    p.update(params or {})
+    t0 = time.perf_counter()
+    cmb = make_constitutive(prob)
+    cls = DFGMRES if solver == "dfgmres" else DCG
+    if preconditioner == "amg":
+        amg_options = dict(amg_options or {})
+        amg_options.setdefault("coord", prob["coord"])
+    lin = cls(cmb.pattern, p["linear_tol"], p["linear_maxit"], p["deflation_tol"], preconditioner,
+              amg_options=amg_options)
+    f = DeviceArray.from_host(prob["f"].T)
+    t_setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    U, lam_h, om_h, umax_h, stats = SSR_indirect_continuation(
+        p["lambda_init"], p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
+        p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], None, prob["Q"], f, cmb,
+        lin, log=log, predictor=predictor)
+    t_run = time.perf_counter() - t1
+    return dict(FoS=float(lam_h[-1]), lambda_hist=lam_h, omega_hist=om_h, Umax_hist=umax_h, stats=stats,
+                U=U, setup_s=t_setup, time_to_fos_s=t_run, timings=dict(cmb.timings), counts=dict(cmb.counts),
+                newton_iterations=int(sum(stats["attempt_newton_iterations"])))

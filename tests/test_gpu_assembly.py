@@ -1,0 +1,129 @@
+"""HIP tangent assembly, strain gather and internal force vs the oracle.
+
+Tolerance: tangent values within 1e-11 relative, measured normwise
+(max |V - V_ref| <= 1e-11 * max |V_ref|); strains/forces within 1e-12 of the
+largest entry.
+"""
+import numpy as np
+import pytest
+
+from oracle import fe_oracle as FO
+from oracle import oracle as O
+from slope_stability_octave_amd import assembly as A
+from slope_stability_octave_amd import fem, mesh
+from slope_stability_octave_amd import workloads as W
+from slope_stability_octave_amd._lib import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+TOL_K = 1e-11
+
+
+def problem(dim, size="small", seed=0):
+    if dim == 2:
+        coord, elem, Q = mesh.mesh_P2_2D(2.5 if size == "small" else 1.0, 5, 5, 5, 5, 5)
+    elif size == "small":
+        coord, elem, Q = mesh.mesh_P2_3D(1, 1, 1, 1, 1, 1, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0)
+    else:
+        coord, elem, Q = mesh.mesh_P2_3D(2, 2, 3, 2, 2, 3, 2.0, 2.0, 3.0, 2.0, 2.0, 3.0)
+    DPhi, WEIGHT, HatP, WF = fem.gradients(coord, elem, dim)
+    n_int = WEIGHT.size
+    G, K, L = W.moduli()
+    rng = np.random.default_rng(seed)
+    ns = 6 if dim == 3 else 3
+    E = rng.normal(0, 2e-3, size=(ns, n_int))
+    cb, sp = O.reduction(np.full(n_int, 15.0), np.full(n_int, np.pi / 6), np.zeros(n_int), 1.3, "B")
+    f = O.mc3d if dim == 3 else O.mc2d
+    S, DS, br = f(E, cb, sp, np.full(n_int, G), np.full(n_int, K), np.full(n_int, L))
+    R = FO.RefPattern(elem, Q, dim, DPhi, WEIGHT, np.full(n_int, G), np.full(n_int, K))
+    return dict(coord=coord, elem=elem, Q=Q, DPhi=DPhi, WEIGHT=WEIGHT, n_q=WF.size, DS=DS, S=S, R=R,
+                shear=np.full(n_int, G), bulk=np.full(n_int, K))
+
+
+def relK(V, V0):
+    return np.abs(V - V0).max() / np.abs(V0).max()
+
+
+@pytest.mark.parametrize("dim,size", [(3, "small"), (2, "small"), (3, "medium"), (2, "medium")])
+def test_mex_dropin_scatter_map(dim, size):
+    p = problem(dim, size)
+    n_n = p["coord"].shape[1]
+    R = p["R"]
+    smap = R.scatter_map(p["elem"], n_n)
+    V0 = O.assemble(dim, p["DPhi"], p["DS"], p["WEIGHT"], smap, p["n_q"], R.nnz)
+    if dim == 3:
+        V = A.assemble_K_tangent_vals(*p["DPhi"], p["DS"], p["WEIGHT"], smap, np.int32(p["n_q"]), np.int32(R.nnz))
+    else:
+        V = A.assemble_K_tangent_vals_2D(*p["DPhi"], p["DS"], p["WEIGHT"], smap, np.int32(p["n_q"]), np.int32(R.nnz))
+    assert relK(V, V0) <= TOL_K
+
+
+def test_mex_dropin_rejects_non_int64_map():
+    p = problem(3)
+    with pytest.raises(TypeError, match="int64"):
+        A.assemble_K_tangent_vals(*p["DPhi"], p["DS"], p["WEIGHT"], np.zeros((900, 1), dtype=np.int32), 11, 10)
+
+
+@pytest.mark.parametrize("dim,size", [(3, "small"), (2, "small"), (3, "medium")])
+def test_block_assembly_regularize_extract(dim, size):
+    p = problem(dim, size)
+    R = p["R"]
+    P = A.Pattern(p["elem"], p["Q"], dim)
+    dD = [DeviceArray.from_host(x.ravel(order="F")) for x in p["DPhi"]]
+    dW = DeviceArray.from_host(p["WEIGHT"])
+    dDS = DeviceArray.from_host(np.asfortranarray(p["DS"]).T.copy())  # point-major
+    Kt = DeviceArray(P.nnzb * P.bs)
+    P.assemble(p["n_q"], dD, dDS, dW, Kt)
+    Vt = DeviceArray(P.nnz_free)
+    P.extract_ref(Kt, Vt)
+    V0 = R.global_tangent(p["DS"], p["WEIGHT"])
+    assert relK(Vt.get(), V0) <= TOL_K
+    # elastic operator and the regularised, symmetrised K_r
+    n_int = p["WEIGHT"].size
+    ns = 6 if dim == 3 else 3
+    dDSe = DeviceArray((n_int, ns * ns))
+    A.ds_elastic_dev(dim, n_int, DeviceArray.from_host(p["shear"]), DeviceArray.from_host(p["bulk"]), dDSe)
+    Kel = DeviceArray(P.nnzb * P.bs)
+    P.assemble(p["n_q"], dD, dDSe, dW, Kel)
+    r = 0.37
+    Kr = DeviceArray(P.nnzb * P.bs)
+    P.regularize(r, Kel, Kt, Kr)
+    Vr = DeviceArray(P.nnz_free)
+    P.extract_ref(Kr, Vr)
+    Vr0 = R.build_K_r(r, V0)
+    assert relK(Vr.get(), Vr0) <= TOL_K
+    # Dirichlet rows/columns masked with a unit diagonal, free part symmetric
+    Kh = Kr.get().reshape(-1, dim, dim)
+    rows = np.repeat(np.arange(P.n_n), np.diff(P.row_ptr))
+    mask = P.mask.reshape(-1, dim)
+    col = P.col
+    for b in np.flatnonzero((mask[rows] == 0).any(axis=1))[:200]:
+        for i in range(dim):
+            if mask[rows[b], i] == 0:
+                expect = np.zeros(dim)
+                if rows[b] == col[b]:
+                    expect[i] = 1.0
+                assert np.array_equal(Kh[b, i], expect)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_strain_and_force(dim):
+    p = problem(dim, "medium")
+    n_n = p["coord"].shape[1]
+    P = A.Pattern(p["elem"], p["Q"], dim)
+    rng = np.random.default_rng(9)
+    U = rng.normal(size=(dim, n_n))
+    dU = DeviceArray.from_host(U.T)  # node-major dim x n_n == (n_n, dim) C-order
+    dD = [DeviceArray.from_host(x.ravel(order="F")) for x in p["DPhi"]]
+    ns = 6 if dim == 3 else 3
+    n_int = p["WEIGHT"].size
+    dE = DeviceArray((n_int, ns))
+    P.strain(p["n_q"], dD, dU, dE)
+    E0 = O.strain(dim, p["elem"], p["DPhi"], U, p["n_q"])
+    assert np.abs(dE.get().T - E0).max() <= 1e-12 * np.abs(E0).max()
+    srows = 6 if dim == 3 else 4
+    dS = DeviceArray.from_host(np.asfortranarray(p["S"]).T.copy())
+    dF = DeviceArray((n_n, dim))
+    P.force(p["n_q"], dD, dS, srows, DeviceArray.from_host(p["WEIGHT"]), dF)
+    F0 = O.force(dim, p["elem"], p["DPhi"], p["S"], p["WEIGHT"], p["n_q"], n_n)
+    assert np.abs(dF.get().T - F0).max() <= 1e-12 * np.abs(F0).max()

@@ -1,0 +1,65 @@
+"""CONSTITUTIVE on the GPU vs its CPU restatement, and the SSR factor of
+safety end to end.
+
+Tolerances: internal forces within 1e-12 of the largest entry; K_r values
+within 1e-11 relative (normwise); factor of safety |dFoS| <= 1e-4 against the
+CPU oracle path (FP64, direct linear solves) on the same mesh.
+"""
+import numpy as np
+import pytest
+
+from oracle import newton_path as NP
+from slope_stability_octave_amd import problems
+from slope_stability_octave_amd._lib import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def coarse2d():
+    prob = problems.slope_2d(h=2.0)
+    cpu = NP.ConstitutiveCPU(prob["coord"], prob["elem"], prob["Q"], 2, prob["DPhi"], prob["WEIGHT"], prob["mats"])
+    return prob, cpu
+
+
+def test_build_F_and_K_r_match_cpu(coarse2d):
+    prob, cpu = coarse2d
+    cmb = problems.make_constitutive(prob)
+    rng = np.random.default_rng(0)
+    U = rng.normal(0, 5e-3, size=(2, prob["coord"].shape[1])) * prob["Q"]
+    dU = DeviceArray.from_host(U.T)
+    for lam in (0.8, 1.3):
+        F = cmb.build_F_and_DS_all(lam, dU).get().T
+        F0 = cpu.build_F_and_DS_all(lam, U)
+        assert np.abs(F - F0).max() <= 1e-12 * np.abs(F0).max()
+        Fs = cmb.build_F_all(lam + 1e-7, dU).get().T
+        Fs0 = cpu.build_F_all(lam + 1e-7, U)
+        assert np.abs(Fs - Fs0).max() <= 1e-12 * np.abs(Fs0).max()
+        for r in (1e-4, 0.3):
+            V = cmb.build_K_r_QQ_vals_ref(r)
+            V0 = cpu.build_K_r(r).tocoo()
+            Vref = cpu.R.build_K_r(r, __import__("oracle").oracle.assemble(
+                2, prob["DPhi"], cpu.DS, prob["WEIGHT"], cpu.smap, prob["n_q"], cpu.R.nnz))
+            assert np.abs(V - Vref).max() <= 1e-11 * np.abs(Vref).max()
+            assert V0.nnz == V.size
+
+
+def test_fos_matches_cpu_oracle_coarse_2d(coarse2d):
+    """Same continuation parameters on both sides; the GPU Krylov tolerance is
+    tightened so both paths solve the Newton systems to the same accuracy
+    (with the reference's 1e-1 the continuation may take one more or one
+    fewer step, which moves lambda by the last increment)."""
+    prob, cpu = coarse2d
+    p = dict(problems.PARAMS_2D)
+    p["lambda_init"] = 1.0
+    p["linear_tol"] = 1e-10
+    p["linear_maxit"] = 1500
+    res = problems.run_ssr(prob, p)
+    U0, lh0, oh0 = NP.SSR_indirect_continuation(
+        1.0, p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
+        p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], prob["f"], cpu)
+    print("GPU FoS", res["FoS"], "CPU FoS", lh0[-1], "steps", len(res["lambda_hist"]), len(lh0))
+    assert abs(res["FoS"] - lh0[-1]) <= 1e-4
+    # the lambda-omega curve is monotone (slope_stability_2D_homo_SSR.m:229-234)
+    assert np.all(np.diff(res["lambda_hist"]) >= -1e-10)
+    assert np.all(np.diff(res["omega_hist"]) >= -1e-10)
