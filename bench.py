@@ -55,7 +55,7 @@ def cpu_baseline(d, cb, sp, budget_s=12.0, sample=1_320_000):
         O.mc3d(E, *mats, use_ref=(kind == "reference"))
         reps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 200:
+        if el >= budget_s or reps >= 2000:
             break
     rate = reps * n / el / 1e9
     return {
@@ -68,6 +68,45 @@ def cpu_baseline(d, cb, sp, budget_s=12.0, sample=1_320_000):
     }
 
 
+def run_fos(args):
+    """Time-to-FoS of the indirect SSR continuation on the GPU (plain FP64
+    form) for the 2D homogeneous slope of the reference's 2D SSR driver."""
+    from slope_stability_octave_amd import problems
+    from slope_stability_octave_amd import _lib
+
+    _lib.require_gpu()
+    prob = problems.slope_2d(h=args.h)
+    params = dict(problems.PARAMS_2D)
+    params["lambda_init"] = args.lambda_init
+    params["omega_max_stop"] = args.omega_max
+    if args.linear_tol is not None:
+        params["linear_tol"] = args.linear_tol
+    log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
+    res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log)
+    out = {
+        "metric": "SSR time-to-FoS (plain GPU form, FP64)",
+        "value": res["time_to_fos_s"],
+        "unit": "s",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D homogeneous slope SSR, P2 triangles h={args.h}",
+                   "free_dof": prob["n_dof_free"], "int_points": prob["n_int"],
+                   "lambda_init": args.lambda_init, "omega_max_stop": args.omega_max,
+                   "preconditioner": args.precond},
+        "FoS": res["FoS"],
+        "steps": len(res["lambda_hist"]),
+        "newton_iterations": res["newton_iterations"],
+        "lambda_hist": [float(x) for x in res["lambda_hist"]],
+        "omega_hist": [float(x) for x in res["omega_hist"]],
+        "setup_s": res["setup_s"],
+        "timings_s": res["timings"],
+        "counts": res["counts"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,12 +114,27 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d"])
+    ap.add_argument("--h", type=float, default=0.5)
+    ap.add_argument("--lambda-init", type=float, default=1.0)
+    ap.add_argument("--omega-max", type=float, default=7e7)
+    ap.add_argument("--precond", default="amg", choices=["amg", "block_jacobi"])
+    ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--variant", type=int, default=None, help="3D constitutive kernel variant (0 direct, 1 LDS)")
+    ap.add_argument("--linear-tol", type=float, default=None)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain device copy of known size first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the hot path is measured on one "
               "MI355X; the multi-GPU decomposition is a design note, not a launch mode)",
               file=sys.stderr)
         sys.exit(2)
+    if args.variant is not None:
+        from slope_stability_octave_amd import _lib as L
+        L.set_tuning(L.TUNE_CONSTITUTIVE_VARIANT, args.variant)
+    if args.workload == "fos2d":
+        return run_fos(args)
 
     from slope_stability_octave_amd import _lib
     from slope_stability_octave_amd import constitutive_mex as cm
@@ -98,6 +152,17 @@ def main():
     dG, dK, dL = (DeviceArray.from_host(d[k]) for k in ("shear", "bulk", "lame"))
     dcb, dsp = DeviceArray(n), DeviceArray(n)
     dS, dD = DeviceArray((n, 6)), DeviceArray((n, 36))
+
+    if args.copy_calibration:
+        # plain copy of 36*n doubles (algorithmic traffic 2*8*36*n bytes): the
+        # reference point for scaling FETCH_SIZE / WRITE_SIZE counter readings
+        from slope_stability_octave_amd import linear_solvers as LS
+        src = dD.view(0, (36 * n,))
+        dst = DeviceArray(36 * n)
+        for _ in range(2):
+            LS.axpby(1.0, src, 0.0, dst)
+        _lib.synchronize()
+        del dst
 
     tim = EventTimer()
     tim.start()

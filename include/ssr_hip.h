@@ -91,6 +91,112 @@ int ssr_dev_reduction(int64_t n_int, int davis, const double *c0, const double *
                       const double *psi, const double *lambda, int lambda_per_point,
                       int davis_c_all, double *c_bar, double *sin_phi);
 
+/* ---- tuning ------------------------------------------------------------- */
+/* key 0: 3D constitutive kernel variant (0 direct point-major stores,
+ *        1 LDS-staged coalesced, default); key 1: lanes per block row in the
+ *        node-block SpMV (4, 8 = default, 16).  No reference counterpart. */
+int ssr_set_tuning(int key, int value);
+int ssr_get_tuning(int key, int *value);
+
+/* ---- tangent assembly ---------------------------------------------------- */
+
+/* V_tang = assemble_K_tangent_vals(DPhi1, DPhi2, DPhi3, DS, WEIGHT, scatter_map, n_q, nnz_out)
+ * replaces slope_stability/+ASSEMBLY/mex/assemble_K_tangent_vals.c:38-172 (dim = 3,
+ * P2 tetrahedra, 11 points) and assemble_K_tangent_vals_2D.c:31-165 (dim = 2,
+ * DPhi3 = NULL, P2 triangles, 7 points).  DPhi* are n_p x n_int, DS is ns^2 x n_int,
+ * scatter_map is int64 (dim n_p)^2 x n_e, 1-based, 0 = skip.  V is zero-filled. */
+int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, int64_t n_e, int n_q,
+                                const double *DPhi1, const double *DPhi2, const double *DPhi3,
+                                const double *DS, const double *WEIGHT, const int64_t *scatter_map,
+                                int64_t nnz_out, double *V);
+int ssr_dev_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, int64_t n_e, int n_q,
+                                    const double *DPhi1, const double *DPhi2, const double *DPhi3,
+                                    const double *DS, const double *WEIGHT, const int64_t *smap,
+                                    int64_t nnz, double *V);
+
+/* Fixed sparsity pattern: replaces CONSTITUTIVE.init_K_r_pattern (CONSTITUTIVE.m:393-464)
+ * and build_scatter_map (:582-635).  Node-block CSR over all nodes with dim x dim
+ * blocks; Dirichlet DOFs (Q == 0) masked.  elem: n_p x n_e (0-based), Q: dim x n_n. */
+typedef struct ssr_pattern ssr_pattern;
+int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int64_t *elem,
+                       const uint8_t *Q, int upload_device, ssr_pattern **out);
+int ssr_pattern_destroy(ssr_pattern *p);
+/* info[8] = dim, n_n, n_p, n_e, n_blocks, n_free, nnz_free, on_device */
+int ssr_pattern_info(const ssr_pattern *p, int64_t *info);
+/* host tables: 0 row_ptr, 1 col, 2 element block map, 3 transpose block, 4 diagonal
+ * block, 5 reference-order positions (int64), 6 free mask (uint8) */
+int ssr_pattern_get(const ssr_pattern *p, int which, void *dst);
+/* device tables: 0 row_ptr, 1 col, 2 emap, 3 tblk, 4 diag, 5 elem, 6 ref_pos, 7 mask, 8 brow */
+int ssr_pattern_dev_ptr(const ssr_pattern *p, int which, void **out);
+
+/* sum_e sum_q B'(w D)B into the block values (zeroed first); ds_layout 0 = MEX
+ * layout (ns^2 per point), 1 = packed symmetric (ns(ns+1)/2 per point).
+ * Replaces CONSTITUTIVE.build_K_tangent_QQ_vals (CONSTITUTIVE.m:487-501). */
+int ssr_dev_assemble_bsr(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                         const double *DPhi3, const double *DS, int ds_layout, const double *WEIGHT,
+                         double *Kvals);
+/* E = B*U (CONSTITUTIVE.m:218,251); U node-major dim x n_n, E ns x n_int point-major. */
+int ssr_dev_strain(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                   const double *DPhi3, const double *U, double *E);
+/* F = B'*(WEIGHT.*S(1:ns,:)) (CONSTITUTIVE.m:268-285); S has s_rows rows per point. */
+int ssr_dev_force(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                  const double *DPhi3, const double *S, int s_rows, const double *WEIGHT, double *F);
+/* K_r = r K_el + (1-r) K_t, (K_r + K_r')/2, Dirichlet masked (CONSTITUTIVE.m:515-532). */
+int ssr_dev_regularize(const ssr_pattern *P, double r, const double *Kel, const double *Kt, double *Kr);
+/* block values -> V in the reference's (ref_I, ref_J) order (CONSTITUTIVE.m:445). */
+int ssr_dev_extract_ref(const ssr_pattern *P, const double *Kvals, double *Vref);
+/* DS_elast = 2 DEV shear + VOL bulk per point (CONSTITUTIVE.m:413-422), MEX layout. */
+int ssr_dev_ds_elastic(int dim, int64_t n_int, const double *shear, const double *bulk, double *DS);
+
+/* ---- sparse / dense kernels of the Krylov path --------------------------- */
+/* y = alpha K x + beta y on the node-block matrix (the librsb SpMV of newton_ind_SSR.m:91) */
+int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col,
+                     const double *val, const double *x, double *y, double alpha, double beta);
+int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result);
+int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result);
+int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h);
+int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const double *h, double alpha,
+                        double beta, double *u);
+int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y);
+int ssr_dev_lincomb(int64_t n, double a, const double *x, double b, const double *y, double *z);
+int ssr_dev_mul(int64_t n, const double *m, double *x);
+int ssr_dev_block_jacobi_setup(int dim, int64_t n_n, const int32_t *diag, const double *val, double *inv);
+int ssr_dev_block_jacobi_apply(int dim, int64_t n_n, const double *inv, const double *x,
+                               const double *mask, double *y);
+int ssr_dev_csr_spmv(int64_t n_rows, const int64_t *rp, const int32_t *ci, const double *v,
+                     const double *x, double *y, double alpha, double beta);
+int ssr_dev_ewmul(int64_t n, const double *a, const double *b, double *z);
+int ssr_dev_cheb_step(int64_t n, double c2, double c3, const double *dinv, const double *r, double *d,
+                      double *x);
+int ssr_dev_dense_gemv(int n, const double *M, const double *x, double *y);
+
+/* ---- deflated Krylov solvers ---------------------------------------------- */
+/* Replaces the DFGMRES class (+LINEAR_SOLVERS/DFGMRES.m:77-221) with dfgmres_solver.m,
+ * dcg_solver.m and A_orthogonalize.m.  precond_type 0 = node-block Jacobi;
+ * ssr_solver_set_amg installs smoothed-aggregation AMG (the BoomerAMG role of
+ * hypre_boomeramg_mex.cpp: setup_ijv / update_values / apply). */
+typedef struct ssr_solver ssr_solver;
+int ssr_solver_create(const ssr_pattern *P, int maxit, double tol, double defl_tol, int precond_type,
+                      ssr_solver **out);
+int ssr_solver_destroy(ssr_solver *s);
+/* operator values (device, borrowed); setup_precond = 1 rebuilds/refreshes M */
+int ssr_solver_set_matrix(ssr_solver *s, const double *Kvals, int setup_precond);
+/* near_null: n_null columns over all dim*n_n DOFs (host, column-major);
+ * opts: NULL or double[8] {theta, max_levels, max_coarse, cheb_degree, cheb_lower,
+ * cheb_upper, refresh_every, rebuild_every} */
+int ssr_solver_set_amg(ssr_solver *s, const double *near_null, int n_null, const double *opts);
+int ssr_solver_deflation_size(const ssr_solver *s, int *k);
+int ssr_solver_clear_deflation(ssr_solver *s);
+int ssr_solver_expand_deflation(ssr_solver *s, const double *v);       /* DFGMRES.m:163-170 */
+int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src); /* DFGMRES.m:204-221 */
+int ssr_solver_get_deflation(const ssr_solver *s, int c, double *out);
+int ssr_solver_A_orthogonalize(ssr_solver *s);                         /* A_orthogonalize.m */
+int ssr_solver_dfgmres(ssr_solver *s, const double *b, double *x, int *iters, double *relres);
+int ssr_solver_dcg(ssr_solver *s, const double *b, double *x, int *iters, double *relres);
+int ssr_solver_precond_apply(ssr_solver *s, const double *x, double *y);
+/* out[4] = total iterations, solves, preconditioner applications, SpMVs */
+int ssr_solver_stats(const ssr_solver *s, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,8 @@ SIGNATURES = {
     "ssr_dev_constitutive_3D": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_constitutive_2D": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_reduction": [_i64, _int, _vp, _vp, _vp, _vp, _int, _int, _vp, _vp],
+    "ssr_set_tuning": [_int, _int],
+    "ssr_get_tuning": [_int, _vp],
 }
 _RESTYPE = {"ssr_last_error": ctypes.c_char_p, "ssr_get_stream": _vp}
 
@@ -183,6 +185,20 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+TUNE_CONSTITUTIVE_VARIANT = 0
+TUNE_SPMV_GROUP = 1
+
+
+def set_tuning(key: int, value: int):
+    call("ssr_set_tuning", int(key), int(value))
+
+
+def get_tuning(key: int) -> int:
+    v = ctypes.c_int(0)
+    call("ssr_get_tuning", int(key), ctypes.byref(v))
+    return v.value
 
 
 def synchronize():

@@ -8,7 +8,7 @@ import numpy as np
 from ._lib import call, ptr
 
 DEFAULTS = dict(theta=0.0, max_levels=12, max_coarse=600, cheb_degree=2, cheb_lower=1.0 / 30.0, cheb_upper=1.1,
-                refresh_every=1, rebuild_every=0)
+                refresh_every=0, rebuild_every=0)
 
 
 def rigid_body_modes(coord):

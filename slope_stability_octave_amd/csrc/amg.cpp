@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -180,7 +182,8 @@ struct Level {
     HCsr P, R;                     // prolongation to this level from the next coarser / restriction
     std::vector<int64_t> node_ptr; // dofs of node i: node_ptr[i]..node_ptr[i+1]
     std::vector<double> dinv;
-    double rho = 1.0;
+    double rho = 1.0;    // power-iteration estimate of rho(D^-1 A) (prolongator smoothing)
+    double rho_up = 1.0; // Gershgorin bound of rho(D^-1 A) (Chebyshev interval)
     DCsr dA, dP, dR;
     double *d_dinv = nullptr, *d_x = nullptr, *d_b = nullptr, *d_r = nullptr, *d_d = nullptr;
     ~Level()
@@ -192,6 +195,18 @@ struct Level {
         (void)hipFree(d_d);
     }
 };
+
+// max_i sum_j |a_ij| |dinv_i|: an upper bound of the spectral radius of D^-1 A
+double gershgorin_rho(const HCsr &A, const std::vector<double> &dinv)
+{
+    double m = 0.0;
+    for (int64_t r = 0; r < A.nr; ++r) {
+        double s = 0.0;
+        for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) s += fabs(A.v[k]);
+        m = std::max(m, s * fabs(dinv[r]));
+    }
+    return m > 0 ? m : 1.0;
+}
 
 double power_rho(const HCsr &A, const std::vector<double> &dinv, int its)
 {
@@ -469,7 +484,8 @@ struct Amg : ssr_precond {
         for (int lev = 0;; ++lev) {
             Level &lv = *L[lev];
             diag_inv(lv.A, lv.dinv);
-            lv.rho = power_rho(lv.A, lv.dinv, 12);
+            lv.rho = power_rho(lv.A, lv.dinv, 30);
+            lv.rho_up = gershgorin_rho(lv.A, lv.dinv);
             if (lv.A.nr <= opt.max_coarse || lev + 1 >= opt.max_levels) break;
             int32_t n_agg = 0;
             std::vector<int32_t> agg = aggregate(lv, n_agg);
@@ -530,6 +546,12 @@ struct Amg : ssr_precond {
         }
         if ((rc = build_coarse())) return rc;
         built = true;
+        if (getenv("SSR_AMG_VERBOSE")) {
+            for (size_t l = 0; l < L.size(); ++l)
+                fprintf(stderr, "[amg] level %zu: n=%lld nnz=%lld nodes=%lld rho=%.3g rho_up=%.3g\n", l,
+                        (long long)L[l]->A.nr, (long long)L[l]->A.ci.size(), (long long)L[l]->node_ptr.size() - 1,
+                        L[l]->rho, L[l]->rho_up);
+        }
         return SSR_OK;
     }
 
@@ -542,7 +564,8 @@ struct Amg : ssr_precond {
         for (size_t l = 0; l < L.size(); ++l) {
             Level &lv = *L[l];
             diag_inv(lv.A, lv.dinv);
-            lv.rho = power_rho(lv.A, lv.dinv, 12);
+            lv.rho = power_rho(lv.A, lv.dinv, 30);
+            lv.rho_up = gershgorin_rho(lv.A, lv.dinv);
             if ((rc = dupload(&lv.d_dinv, lv.dinv))) return rc;
             if (l + 1 < L.size()) {
                 Level &c = *L[l + 1];
@@ -606,13 +629,18 @@ struct Amg : ssr_precond {
 
     int setup(const double *Kvals) override
     {
+        // The finest level always applies the current operator (Kvals on the
+        // device); the coarse hierarchy is rebuilt / refreshed on the host only
+        // when the schedule asks for it.
         Kdev = Kvals;
-        Kh.resize((size_t)nnzb * dim * dim);
-        SSR_HIP_CHECK(hipMemcpy(Kh.data(), Kvals, sizeof(double) * Kh.size(), hipMemcpyDeviceToHost));
         const bool rebuild = !built || (opt.rebuild_every > 0 && n_setup % opt.rebuild_every == 0);
+        const bool refresh_now = !rebuild && opt.refresh_every > 0 && n_setup % opt.refresh_every == 0;
         int rc = SSR_OK;
-        if (rebuild) rc = build();
-        else if (opt.refresh_every > 0 && n_setup % opt.refresh_every == 0) rc = refresh();
+        if (rebuild || refresh_now) {
+            Kh.resize((size_t)nnzb * dim * dim);
+            SSR_HIP_CHECK(hipMemcpy(Kh.data(), Kvals, sizeof(double) * Kh.size(), hipMemcpyDeviceToHost));
+            rc = rebuild ? build() : refresh();
+        }
         n_setup++;
         return rc;
     }
@@ -628,7 +656,9 @@ struct Amg : ssr_precond {
     {
         Level &lv = *L[l];
         const int64_t m = lv.A.nr;
-        const double up = opt.cheb_upper * lv.rho, lo = opt.cheb_lower * lv.rho;
+        // interval [lower, upper] * rho_up with rho_up a guaranteed upper bound, so the
+        // polynomial never amplifies the highest modes
+        const double up = opt.cheb_upper * lv.rho_up, lo = opt.cheb_lower * lv.rho_up;
         const double theta = 0.5 * (up + lo), delta = 0.5 * (up - lo);
         const double sigma = theta / delta;
         double rho_k = 1.0 / sigma;

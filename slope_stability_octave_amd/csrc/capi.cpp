@@ -250,3 +250,24 @@ extern "C" int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, 
     if (rc) return rc;
     return ssr_dtoh(V, dV.p, sizeof(double) * nnz_out);
 }
+
+static int g_tuning[SSR_TUNE_COUNT] = {1, 8, 0, 0, 0, 0, 0, 0};
+
+int ssr_tuning(int key) { return (key >= 0 && key < SSR_TUNE_COUNT) ? g_tuning[key] : 0; }
+
+// Set a kernel-variant / launch-shape knob (see SSR_TUNE_* in ssr_common.h).
+extern "C" int ssr_set_tuning(int key, int value)
+{
+    SSR_REQUIRE(key >= 0 && key < SSR_TUNE_COUNT, "set_tuning: unknown key %d", key);
+    if (key == SSR_TUNE_SPMV_GROUP)
+        SSR_REQUIRE(value == 4 || value == 8 || value == 16, "set_tuning: SpMV group must be 4, 8 or 16");
+    g_tuning[key] = value;
+    return SSR_OK;
+}
+
+extern "C" int ssr_get_tuning(int key, int *value)
+{
+    SSR_REQUIRE(key >= 0 && key < SSR_TUNE_COUNT && value, "get_tuning: unknown key %d", key);
+    *value = g_tuning[key];
+    return SSR_OK;
+}

@@ -282,6 +282,71 @@ __global__ __launch_bounds__(256) void k_mc3d_aos(int64_t n, const double *__res
     if (branch) branch[p] = (int8_t)br;
 }
 
+// Same contract, one 64-lane wavefront per block, with every global access of
+// the point-major arrays coalesced through LDS: the wave loads its 64x6 strain
+// block as one contiguous span, and writes S (64x6) and DS (64x36) as
+// contiguous spans of 16-byte stores.  DS is staged as the 21 unique entries
+// per point (stride 21 doubles spreads the LDS banks) and expanded to the full
+// column-major 6x6 block on the way out.
+template <bool WITH_DS>
+__global__ __launch_bounds__(64) void k_mc3d_aos_lds(int64_t n, const double *__restrict__ E,
+                                                     const double *__restrict__ cbar,
+                                                     const double *__restrict__ sphi,
+                                                     const double *__restrict__ shear,
+                                                     const double *__restrict__ lame,
+                                                     double *__restrict__ S, double *__restrict__ DS,
+                                                     int8_t *__restrict__ branch)
+{
+    constexpr int W = 64;
+    __shared__ double sbuf[W * 21];
+    const int lane = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * W;
+    const int cnt = (int)((n - p0) < W ? (n - p0) : W);
+    const int64_t p = p0 + lane;
+    {
+        const double2 *Eg = reinterpret_cast<const double2 *>(E + 6 * p0);
+        double2 *s2 = reinterpret_cast<double2 *>(sbuf);
+        for (int t = lane; t < cnt * 3; t += W) s2[t] = Eg[t];
+    }
+    __syncthreads();
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    double cb = 1.0, sp = 0.5, G = 1.0, L = 1.0;
+    if (lane < cnt) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) eps[i] = sbuf[lane * 6 + i];
+        cb = cbar[p];
+        sp = sphi[p];
+        G = shear[p];
+        L = lame[p];
+    }
+    double s[6], d[21];
+    const int br = mc3d_point<WITH_DS>(eps, cb, sp, G, L, s, d);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
+    __syncthreads();
+    {
+        const double2 *s2 = reinterpret_cast<const double2 *>(sbuf);
+        double2 *Sg = reinterpret_cast<double2 *>(S + 6 * p0);
+        for (int t = lane; t < cnt * 3; t += W) Sg[t] = s2[t];
+    }
+    if (WITH_DS) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 21; ++k) sbuf[lane * 21 + k] = d[k];
+        __syncthreads();
+        double2 *Dg = reinterpret_cast<double2 *>(DS + 36 * p0);
+        for (int t = lane; t < cnt * 18; t += W) {
+            const int o = 2 * t, q = o / 36, c = o - 36 * q;  // c even: c, c+1 in the same point
+            const int i0 = c % 6, j0 = c / 6, i1 = (c + 1) % 6, j1 = (c + 1) / 6;
+            const double v0 = sbuf[q * 21 + (i0 <= j0 ? sym6(i0, j0) : sym6(j0, i0))];
+            const double v1 = sbuf[q * 21 + (i1 <= j1 ? sym6(i1, j1) : sym6(j1, i1))];
+            Dg[t] = make_double2(v0, v1);
+        }
+    }
+    if (branch && lane < cnt) branch[p] = (int8_t)br;
+}
+
 // ---------------------------------------------------------------------------
 // 2D (plane strain)
 // ---------------------------------------------------------------------------
@@ -492,13 +557,22 @@ extern "C" int ssr_dev_constitutive_3D(int64_t n, const double *E, const double 
     SSR_REQUIRE(n >= 0, "constitutive_3D: n_int must be >= 0");
     if (n == 0) return SSR_OK;
     SSR_REQUIRE(E && c_bar && sin_phi && shear && lame && S, "constitutive_3D: null argument");
-    const int th = 256;
-    if (DS)
-        hipLaunchKernelGGL(k_mc3d_aos<true>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n, E,
-                           c_bar, sin_phi, shear, lame, S, DS, branch);
-    else
-        hipLaunchKernelGGL(k_mc3d_aos<false>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n,
-                           E, c_bar, sin_phi, shear, lame, S, DS, branch);
+    if (ssr_tuning(SSR_TUNE_CONSTITUTIVE_VARIANT) == 0) {
+        const int th = 256;
+        if (DS)
+            hipLaunchKernelGGL(k_mc3d_aos<true>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n, E,
+                               c_bar, sin_phi, shear, lame, S, DS, branch);
+        else
+            hipLaunchKernelGGL(k_mc3d_aos<false>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n,
+                               E, c_bar, sin_phi, shear, lame, S, DS, branch);
+    } else {
+        if (DS)
+            hipLaunchKernelGGL(k_mc3d_aos_lds<true>, dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n, E,
+                               c_bar, sin_phi, shear, lame, S, DS, branch);
+        else
+            hipLaunchKernelGGL(k_mc3d_aos_lds<false>, dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
+                               E, c_bar, sin_phi, shear, lame, S, DS, branch);
+    }
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

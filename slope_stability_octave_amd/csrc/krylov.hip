@@ -232,14 +232,21 @@ int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int3
                      const double *x, double *y, double alpha, double beta)
 {
     if (n_rows == 0) return SSR_OK;
-    constexpr int G = 8;
+    const int G = ssr_tuning(SSR_TUNE_SPMV_GROUP);
     const int64_t threads = n_rows * G;
-    if (dim == 3)
-        hipLaunchKernelGGL((k_spmv_bsr<3, G>), dim3(ssr_blocks(threads, 256)), dim3(256), 0, ssr_stream(), n_rows,
-                           row_ptr, col, val, x, y, alpha, beta);
-    else
-        hipLaunchKernelGGL((k_spmv_bsr<2, G>), dim3(ssr_blocks(threads, 256)), dim3(256), 0, ssr_stream(), n_rows,
-                           row_ptr, col, val, x, y, alpha, beta);
+    const dim3 grid(ssr_blocks(threads, 256)), blk(256);
+#define SPMV(D_, G_) hipLaunchKernelGGL((k_spmv_bsr<D_, G_>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, \
+                                        x, y, alpha, beta)
+    if (dim == 3) {
+        if (G == 4) SPMV(3, 4);
+        else if (G == 16) SPMV(3, 16);
+        else SPMV(3, 8);
+    } else {
+        if (G == 4) SPMV(2, 4);
+        else if (G == 16) SPMV(2, 16);
+        else SPMV(2, 8);
+    }
+#undef SPMV
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

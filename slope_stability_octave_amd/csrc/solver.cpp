@@ -392,10 +392,17 @@ int ssr_solver_dfgmres(ssr_solver *s, const double *b, double *x, int *iters, do
         // CGS2
         const int k = j + 1;
         for (int pass = 0; pass < 2; ++pass) {
-            if ((rc = ssr_dev_multidot(n, k, V, n, u, s->hdev.p))) return rc;
+            // all coefficients first (classical Gram-Schmidt), in chunks of 128 columns
+            for (int c0 = 0; c0 < k; c0 += 128) {
+                const int kk = k - c0 < 128 ? k - c0 : 128;
+                if ((rc = ssr_dev_multidot(n, kk, V + (int64_t)c0 * n, n, u, s->hdev.p + c0))) return rc;
+            }
             std::vector<double> &hv = pass == 0 ? h : h2;
             SSR_HIP_CHECK(hipMemcpyAsync(hv.data(), s->hdev.p, sizeof(double) * k, hipMemcpyDeviceToHost, ssr_stream()));
-            if ((rc = ssr_dev_gemv_update(n, k, V, n, s->hdev.p, 1.0, -1.0, u))) return rc;
+            for (int c0 = 0; c0 < k; c0 += 128) {
+                const int kk = k - c0 < 128 ? k - c0 : 128;
+                if ((rc = ssr_dev_gemv_update(n, kk, V + (int64_t)c0 * n, n, s->hdev.p + c0, 1.0, -1.0, u))) return rc;
+            }
         }
         SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
         for (int i = 0; i < k; ++i) h[i] += h2[i];

@@ -48,3 +48,9 @@ struct ssr_precond {
     virtual int setup(const double *Kvals) = 0;       // operator values (device)
     virtual int apply(const double *x, double *y) = 0;  // y = M^{-1} x (device)
 };
+
+// Kernel-variant / launch-shape knobs (set by the autotuner or the caller).
+#define SSR_TUNE_CONSTITUTIVE_VARIANT 0 /* 0 direct point-major stores, 1 LDS-staged (default) */
+#define SSR_TUNE_SPMV_GROUP 1           /* lanes per block row in the node-block SpMV: 4, 8, 16 */
+#define SSR_TUNE_COUNT 8
+int ssr_tuning(int key);

@@ -10,6 +10,7 @@ with Dirichlet entries zero.  The solver loops run in libssr_hip.so.
 from __future__ import annotations
 
 import ctypes
+import time
 
 import numpy as np
 
@@ -46,6 +47,10 @@ _lib.extend_signatures({
     "ssr_dev_mul": [_i64, _vp, _vp],
     "ssr_dev_block_jacobi_setup": [_int, _i64, _vp, _vp, _vp],
     "ssr_dev_block_jacobi_apply": [_int, _i64, _vp, _vp, _vp, _vp],
+    "ssr_dev_csr_spmv": [_i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
+    "ssr_dev_ewmul": [_i64, _vp, _vp, _vp],
+    "ssr_dev_cheb_step": [_i64, _dbl, _dbl, _vp, _vp, _vp, _vp],
+    "ssr_dev_dense_gemv": [_int, _vp, _vp, _vp],
 })
 
 PRECONDITIONERS = {"block_jacobi": 0}
@@ -87,6 +92,8 @@ class DFGMRES:
         self.amg_options = amg_options
         self.verbose = verbose
         self.profiler = None
+        self.timings = {}
+        self.counters = {}
         h = _vp(0)
         call("ssr_solver_create", pattern.handle, self.max_iterations, self.tolerance,
              self.tolerance_deflation_basis, PRECONDITIONERS.get(preconditioner, 0), ctypes.byref(h))
@@ -106,10 +113,16 @@ class DFGMRES:
         except Exception:
             pass
 
+    def _t(self, name, t0):
+        if self.timings is not None:
+            self.timings[name] = self.timings.get(name, 0.0) + time.perf_counter() - t0
+
     # -- preconditioner -------------------------------------------------------
     def setup_preconditioner(self, K):
+        t0 = time.perf_counter()
         self._K = K
         call("ssr_solver_set_matrix", self.handle, ptr(K), 1)
+        self._t("DFGMRES.update_preconditioner", t0)
 
     def update_preconditioner_values(self, K):
         self.setup_preconditioner(K)
@@ -129,8 +142,10 @@ class DFGMRES:
         call("ssr_solver_expand_deflation", self.handle, ptr(v))
 
     def A_orthogonalize(self, K):
+        t0 = time.perf_counter()
         self.set_operator(K)
         call("ssr_solver_A_orthogonalize", self.handle)
+        self._t("DFGMRES.A_orthogonalize", t0)
 
     def deflation_basis(self):
         """Host copy of the basis, columns over all DOFs."""
@@ -149,11 +164,14 @@ class DFGMRES:
                          self.preconditioner_name, self.verbose, self.amg_options)
         call("ssr_solver_copy_deflation", new.handle, self.handle)
         new.profiler = self.profiler
+        new.timings = self.timings  # shared, like the reference's profiler handle
+        new.counters = self.counters
         return new
 
     # -- solve ----------------------------------------------------------------
     def solve(self, K, b, x=None):
         """Return (x, iterations) for K x = b (device arrays)."""
+        t0 = time.perf_counter()
         if K is not self._K:
             self.set_operator(K)
         if x is None:
@@ -164,6 +182,10 @@ class DFGMRES:
         call(name, self.handle, ptr(b), ptr(x), ctypes.byref(it), ctypes.byref(rr))
         self.last_iterations = it.value
         self.last_relres = rr.value
+        self._t("DFGMRES.solve", t0)
+        if self.counters is not None:
+            self.counters["DFGMRES.solve.n_iters"] = self.counters.get("DFGMRES.solve.n_iters", 0) + it.value
+            self.counters["DFGMRES.solve.calls"] = self.counters.get("DFGMRES.solve.calls", 0) + 1
         return x, it.value
 
     def stats(self):

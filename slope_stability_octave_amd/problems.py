@@ -95,6 +95,10 @@ def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", 
         p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], None, prob["Q"], f, cmb,
         lin, log=log, predictor=predictor)
     t_run = time.perf_counter() - t1
+    timings = dict(cmb.timings)
+    timings.update(lin.timings)
+    counts = dict(cmb.counts)
+    counts.update(lin.counters)
     return dict(FoS=float(lam_h[-1]), lambda_hist=lam_h, omega_hist=om_h, Umax_hist=umax_h, stats=stats,
-                U=U, setup_s=t_setup, time_to_fos_s=t_run, timings=dict(cmb.timings), counts=dict(cmb.counts),
+                U=U, setup_s=t_setup, time_to_fos_s=t_run, timings=timings, counts=counts,
                 newton_iterations=int(sum(stats["attempt_newton_iterations"])))
