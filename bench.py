@@ -82,7 +82,8 @@ def run_fos(args):
     if args.linear_tol is not None:
         params["linear_tol"] = args.linear_tol
     log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
-    res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log)
+    amg_opts = {"rebuild_every": args.amg_rebuild_every} if args.amg_rebuild_every is not None else None
+    res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts)
     out = {
         "metric": "SSR time-to-FoS (plain GPU form, FP64)",
         "value": res["time_to_fos_s"],
@@ -122,6 +123,8 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--variant", type=int, default=None, help="3D constitutive kernel variant (0 direct, 1 LDS)")
     ap.add_argument("--linear-tol", type=float, default=None)
+    ap.add_argument("--amg-rebuild-every", type=int, default=None,
+                    help="rebuild the AMG hierarchy every n-th Newton iteration (0: once per continuation step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size first (counter calibration)")
     args = ap.parse_args()
@@ -183,6 +186,12 @@ def main():
     wall_ms = (time.perf_counter() - tw0) * 1e3 / args.steps
     print(f"[bench] {args.steps} steps: {kern_ms:.3f} ms/step (events), {wall_ms:.3f} ms/step (wall)",
           file=sys.stderr, flush=True)
+    # stress-only variant (the S evaluations of the line search), 136 B/point
+    tim.start()
+    for _ in range(5):
+        cm.constitutive_dev(3, n, dE, dcb, dsp, dG, dK, dL, dS, None)
+    tim.stop()
+    s_only_ms = tim.elapsed_ms() / 5
 
     # parity spot check on a slice against the oracle (not timed)
     parity = None
@@ -246,6 +255,9 @@ def main():
             "wall_ms_per_step": wall_ms,
             "reduction_ms": red_ms,
             "reduction_gpts": n / (red_ms * 1e-3) / 1e9,
+            "stress_only_ms": s_only_ms,
+            "stress_only_gpts": n / (s_only_ms * 1e-3) / 1e9,
+            "stress_only_hbm_frac": 136 * n / (s_only_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "parity": parity,
         },
     }

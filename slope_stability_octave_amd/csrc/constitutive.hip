@@ -288,7 +288,19 @@ __global__ __launch_bounds__(256) void k_mc3d_aos(int64_t n, const double *__res
 // contiguous spans of 16-byte stores.  DS is staged as the 21 unique entries
 // per point (stride 21 doubles spreads the LDS banks) and expanded to the full
 // column-major 6x6 block on the way out.
-template <bool WITH_DS>
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+// 16-byte store; NT selects the non-temporal (streaming) form, which does not
+// keep the written lines in the caches.
+template <bool NT>
+__device__ __forceinline__ void store2(double *p, double a, double b)
+{
+    dvec2 v = {a, b};
+    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<dvec2 *>(p));
+    else *reinterpret_cast<dvec2 *>(p) = v;
+}
+
+template <bool WITH_DS, bool NT = false>
 __global__ __launch_bounds__(64) void k_mc3d_aos_lds(int64_t n, const double *__restrict__ E,
                                                      const double *__restrict__ cbar,
                                                      const double *__restrict__ sphi,
@@ -326,22 +338,21 @@ __global__ __launch_bounds__(64) void k_mc3d_aos_lds(int64_t n, const double *__
     for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
     __syncthreads();
     {
-        const double2 *s2 = reinterpret_cast<const double2 *>(sbuf);
-        double2 *Sg = reinterpret_cast<double2 *>(S + 6 * p0);
-        for (int t = lane; t < cnt * 3; t += W) Sg[t] = s2[t];
+        double *Sg = S + 6 * p0;
+        for (int t = lane; t < cnt * 3; t += W) store2<NT>(Sg + 2 * t, sbuf[2 * t], sbuf[2 * t + 1]);
     }
     if (WITH_DS) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 21; ++k) sbuf[lane * 21 + k] = d[k];
         __syncthreads();
-        double2 *Dg = reinterpret_cast<double2 *>(DS + 36 * p0);
+        double *Dg = DS + 36 * p0;
         for (int t = lane; t < cnt * 18; t += W) {
             const int o = 2 * t, q = o / 36, c = o - 36 * q;  // c even: c, c+1 in the same point
             const int i0 = c % 6, j0 = c / 6, i1 = (c + 1) % 6, j1 = (c + 1) / 6;
             const double v0 = sbuf[q * 21 + (i0 <= j0 ? sym6(i0, j0) : sym6(j0, i0))];
             const double v1 = sbuf[q * 21 + (i1 <= j1 ? sym6(i1, j1) : sym6(j1, i1))];
-            Dg[t] = make_double2(v0, v1);
+            store2<NT>(Dg + o, v0, v1);
         }
     }
     if (branch && lane < cnt) branch[p] = (int8_t)br;
@@ -565,12 +576,19 @@ extern "C" int ssr_dev_constitutive_3D(int64_t n, const double *E, const double 
         else
             hipLaunchKernelGGL(k_mc3d_aos<false>, dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n,
                                E, c_bar, sin_phi, shear, lame, S, DS, branch);
+    } else if (ssr_tuning(SSR_TUNE_CONSTITUTIVE_VARIANT) == 1) {
+        if (DS)
+            hipLaunchKernelGGL((k_mc3d_aos_lds<true, false>), dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
+                               E, c_bar, sin_phi, shear, lame, S, DS, branch);
+        else
+            hipLaunchKernelGGL((k_mc3d_aos_lds<false, false>), dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
+                               E, c_bar, sin_phi, shear, lame, S, DS, branch);
     } else {
         if (DS)
-            hipLaunchKernelGGL(k_mc3d_aos_lds<true>, dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n, E,
-                               c_bar, sin_phi, shear, lame, S, DS, branch);
+            hipLaunchKernelGGL((k_mc3d_aos_lds<true, true>), dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
+                               E, c_bar, sin_phi, shear, lame, S, DS, branch);
         else
-            hipLaunchKernelGGL(k_mc3d_aos_lds<false>, dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
+            hipLaunchKernelGGL((k_mc3d_aos_lds<false, true>), dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
                                E, c_bar, sin_phi, shear, lame, S, DS, branch);
     }
     SSR_HIP_CHECK(hipGetLastError());
