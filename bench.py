@@ -70,15 +70,29 @@ def cpu_baseline(d, cb, sp, budget_s=12.0, sample=1_320_000):
 
 def run_fos(args):
     """Time-to-FoS of the indirect SSR continuation on the GPU (plain FP64
-    form) for the 2D homogeneous slope of the reference's 2D SSR driver."""
+    form): the 2D homogeneous slope of slope_stability_2D_homo_SSR.m or the 3D
+    homogeneous slope of slope_stability_3D_homo_SSR.m on its SSR_homo_ada_L1
+    mesh (continuation stopped at omega = 3000 as in the recorded run)."""
     from slope_stability_octave_amd import problems
     from slope_stability_octave_amd import _lib
 
     _lib.require_gpu()
-    prob = problems.slope_2d(h=args.h)
-    params = dict(problems.PARAMS_2D)
-    params["lambda_init"] = args.lambda_init
-    params["omega_max_stop"] = args.omega_max
+    if args.workload == "fos3d":
+        import numpy as np
+        mesh_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                 "mesh_SSR_homo_ada_L1.npz")
+        with np.load(mesh_file) as z:
+            prob = problems.slope_3d_mesh(dict(z))
+        params = dict(problems.PARAMS_3D_NOTEBOOK)
+        if args.omega_max != 7e7:
+            params["omega_max_stop"] = args.omega_max
+        name = "3D homogeneous slope SSR, P2 tetrahedra, mesh SSR_homo_ada_L1 (25591 nodes, 16615 elements)"
+    else:
+        prob = problems.slope_2d(h=args.h)
+        params = dict(problems.PARAMS_2D)
+        params["lambda_init"] = args.lambda_init
+        params["omega_max_stop"] = args.omega_max
+        name = f"2D homogeneous slope SSR, P2 triangles h={args.h}"
     if args.linear_tol is not None:
         params["linear_tol"] = args.linear_tol
     log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
@@ -92,9 +106,9 @@ def run_fos(args):
         "n_gpus": 1,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"2D homogeneous slope SSR, P2 triangles h={args.h}",
+        "config": {"workload": name,
                    "free_dof": prob["n_dof_free"], "int_points": prob["n_int"],
-                   "lambda_init": args.lambda_init, "omega_max_stop": args.omega_max,
+                   "lambda_init": params["lambda_init"], "omega_max_stop": params["omega_max_stop"],
                    "preconditioner": args.precond},
         "FoS": res["FoS"],
         "steps": len(res["lambda_hist"]),
@@ -115,7 +129,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d"])
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d"])
     ap.add_argument("--h", type=float, default=0.5)
     ap.add_argument("--lambda-init", type=float, default=1.0)
     ap.add_argument("--omega-max", type=float, default=7e7)
@@ -127,6 +141,8 @@ def main():
                     help="rebuild the AMG hierarchy every n-th Newton iteration (0: once per continuation step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size first (counter calibration)")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="also time write-only, copy and read-only streaming kernels of the DS size")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the hot path is measured on one "
@@ -136,7 +152,7 @@ def main():
     if args.variant is not None:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_CONSTITUTIVE_VARIANT, args.variant)
-    if args.workload == "fos2d":
+    if args.workload in ("fos2d", "fos3d"):
         return run_fos(args)
 
     from slope_stability_octave_amd import _lib
@@ -192,6 +208,28 @@ def main():
         cm.constitutive_dev(3, n, dE, dcb, dsp, dG, dK, dL, dS, None)
     tim.stop()
     s_only_ms = tim.elapsed_ms() / 5
+
+    # streaming ceilings of this device with plain kernels over the DS buffer
+    # (36n doubles): write-only fill, copy (read + write) and read-only dot
+    probe = None
+    if args.hbm_probe:
+        from slope_stability_octave_amd import linear_solvers as LS
+        nb = 36 * n
+        src = dD.view(0, (nb,))
+        dst = DeviceArray(nb)
+        res = {}
+        for name, fn, nbytes in (("write", lambda: LS.fill(dst, 1.0), 8 * nb),
+                                 ("copy", lambda: LS.axpby(1.0, src, 0.0, dst), 16 * nb),
+                                 ("read", lambda: LS.dot(src, src), 8 * nb)):
+            fn()
+            _lib.synchronize()
+            tim.start()
+            for _ in range(5):
+                fn()
+            tim.stop()
+            res[name + "_GBs"] = nbytes / (tim.elapsed_ms() / 5 * 1e-3) / 1e9
+        del dst
+        probe = res
 
     # parity spot check on a slice against the oracle (not timed)
     parity = None
@@ -259,6 +297,7 @@ def main():
             "stress_only_gpts": n / (s_only_ms * 1e-3) / 1e9,
             "stress_only_hbm_frac": 136 * n / (s_only_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "parity": parity,
+            "hbm_probe": probe,
         },
     }
     print(json.dumps(out))

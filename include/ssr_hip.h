@@ -160,6 +160,8 @@ int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const dou
 int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y);
 int ssr_dev_lincomb(int64_t n, double a, const double *x, double b, const double *y, double *z);
 int ssr_dev_mul(int64_t n, const double *m, double *x);
+/* x = v (write-only fill; clears buffers without a memset node in captured graphs) */
+int ssr_dev_fill(int64_t n, double v, double *x);
 int ssr_dev_block_jacobi_setup(int dim, int64_t n_n, const int32_t *diag, const double *val, double *inv);
 int ssr_dev_block_jacobi_apply(int dim, int64_t n_n, const double *inv, const double *x,
                                const double *mask, double *y);

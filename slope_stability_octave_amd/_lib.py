@@ -189,6 +189,7 @@ class DeviceArray:
 
 TUNE_CONSTITUTIVE_VARIANT = 0
 TUNE_SPMV_GROUP = 1
+TUNE_PERSIST_WAVES = 2
 
 
 def set_tuning(key: int, value: int):

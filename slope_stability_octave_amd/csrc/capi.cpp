@@ -24,6 +24,19 @@ void ssr_set_error(const char *fmt, ...)
 
 hipStream_t ssr_stream(void) { return g_stream; }
 
+int ssr_cu_count(void)
+{
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] <= 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        cached[dev] = c;
+    }
+    return cached[dev];
+}
+
 extern "C" {
 
 const char *ssr_last_error(void) { return g_err; }
@@ -251,7 +264,7 @@ extern "C" int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, 
     return ssr_dtoh(V, dV.p, sizeof(double) * nnz_out);
 }
 
-static int g_tuning[SSR_TUNE_COUNT] = {1, 8, 0, 0, 0, 0, 0, 0};
+static int g_tuning[SSR_TUNE_COUNT] = {2, 8, 0, 0, 0, 0, 0, 0};
 
 int ssr_tuning(int key) { return (key >= 0 && key < SSR_TUNE_COUNT) ? g_tuning[key] : 0; }
 

@@ -47,10 +47,11 @@ __host__ __device__ __forceinline__ double dsq3(int i, int j, const double t[6])
 
 // Branch-reduced coefficients for one 3D point.
 struct Coef3 {
-    double sP1, sP2, sP3, sI, sT;             // S = sP1 P1 + sP2 P2 + sP3 P3 + sI iota + sT t
+    double sP1, sP3, sI, sT;                  // S = sP1 P1 + sP3 P3 + sI iota + sT t
     double a, b;                              // DS += a dT2 + b Id
-    double c11, c22, c33, c13, cII, c1I, c3I, cIT, c1T, c3T;  // symmetric C over (P1,P2,P3,iota,t)
-    bool use1, use2, use3;
+    double c11, c33, c13, cII, c1I, c3I, cIT, c1T, c3T;  // symmetric C over (P1,P3,iota,t)
+    double g1, g3;                            // 1/((e_k-e_m)(e_k-e_n)) of the projections in use
+    bool use1, use3;
 };
 
 // Returns branch code 0..4 (elastic, smooth, left edge, right edge, apex).
@@ -92,7 +93,7 @@ __host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, 
     const double lam_s = f_tr / den_s;
     if (lam_s <= g_sl && lam_s <= g_sr) {
         const double g1 = 1.0 / ((e1 - e2) * (e1 - e3));
-        const double g2 = 1.0 / ((e2 - e1) * (e2 - e3));
+        const double g2 = 1.0 / ((e2 - e1) * (e2 - e3));  // used in A2 only
         const double g3 = 1.0 / ((e3 - e1) * (e3 - e2));
         const double s1 = lam * I1 + 2.0 * G * e1 - lam_s * (2.0 * lam * spsi + 2.0 * G * (1.0 + spsi));
         const double s2 = lam * I1 + 2.0 * G * e2 - lam_s * 2.0 * lam * spsi;
@@ -103,17 +104,20 @@ __host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, 
         const double k3a = 2.0 * e3 - e1 - e2, k3b = e1 - e2;
         const double r = 1.0 / den_s;
         const double al1 = 2.0 * G * (1.0 + sphi), al3 = -2.0 * G * (1.0 - sphi), alI = 2.0 * lam * sphi;
-        k.sP1 = s1; k.sP2 = s2; k.sP3 = s3;
+        // P2 = iota - P1 - P3 (the eigenprojections sum to the identity), so
+        // the P2 terms of S and of c22 P2 (x) P2 fold into the P1/P3/iota ones.
+        const double c22 = -A1 * k1b - A2 * k2a + A3 * k3b + 2.0 * G;
+        k.sP1 = s1 - s2; k.sP3 = s3 - s2; k.sI = s2;
         k.a = A1 + A2 + A3;
         k.b = -(A1 * (e2 + e3) + A2 * (e1 + e3) + A3 * (e1 + e2));
-        k.c11 = -A1 * k1a - A2 * k2b - A3 * k3b + 2.0 * G - r * al1 * al1;
-        k.c22 = -A1 * k1b - A2 * k2a + A3 * k3b + 2.0 * G;
-        k.c33 = A1 * k1b + A2 * k2b - A3 * k3a + 2.0 * G - r * al3 * al3;
-        k.c13 = -r * al1 * al3;
-        k.cII = lam - r * alI * alI;
-        k.c1I = -r * al1 * alI;
-        k.c3I = -r * al3 * alI;
-        k.use1 = k.use2 = k.use3 = true;
+        k.c11 = -A1 * k1a - A2 * k2b - A3 * k3b + 2.0 * G - r * al1 * al1 + c22;
+        k.c33 = A1 * k1b + A2 * k2b - A3 * k3a + 2.0 * G - r * al3 * al3 + c22;
+        k.c13 = -r * al1 * al3 + c22;
+        k.cII = lam - r * alI * alI + c22;
+        k.c1I = -r * al1 * alI - c22;
+        k.c3I = -r * al3 * alI - c22;
+        k.use1 = k.use3 = true;
+        k.g1 = g1; k.g3 = g3;
         return 1;
     }
     if (g_sl < g_sr) {
@@ -139,6 +143,7 @@ __host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, 
             k.cIT = -A;
             k.c3T = A;
             k.use3 = true;
+            k.g3 = g3;
             return 2;
         }
     }
@@ -165,6 +170,7 @@ __host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, 
             k.cIT = -A;
             k.c1T = A;
             k.use1 = true;
+            k.g1 = g1;
             return 3;
         }
     }
@@ -172,11 +178,11 @@ __host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, 
     return 4;
 }
 
-// Eigenprojection (T^2 - (e_m+e_n) T + e_m e_n I) / ((e_k-e_m)(e_k-e_n)).
-__host__ __device__ __forceinline__ void proj3(const double t2[6], const double t[6], double ek, double em,
+// Eigenprojection (T^2 - (e_m+e_n) T + e_m e_n I) * g, g = 1/((e_k-e_m)(e_k-e_n))
+// (the inverse gap already formed for the branch coefficients).
+__host__ __device__ __forceinline__ void proj3(const double t2[6], const double t[6], double g, double em,
                                       double en, bool use, double P[6])
 {
-    const double g = 1.0 / ((ek - em) * (ek - en));
     const double sm = em + en, pm = em * en;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -203,24 +209,22 @@ __host__ __device__ __forceinline__ int mc3d_point(const double eps[6], double c
     t2[3] = t[0] * t[3] + t[1] * t[3] + t[4] * t[5];
     t2[4] = t[3] * t[5] + t[1] * t[4] + t[2] * t[4];
     t2[5] = t[0] * t[5] + t[3] * t[4] + t[2] * t[5];
-    double P1[6], P2[6], P3[6];
-    proj3(t2, t, e1, e2, e3, k.use1, P1);
-    proj3(t2, t, e2, e1, e3, k.use2, P2);
-    proj3(t2, t, e3, e1, e2, k.use3, P3);
+    double P1[6], P3[6];
+    proj3(t2, t, k.g1, e2, e3, k.use1, P1);
+    proj3(t2, t, k.g3, e1, e2, k.use3, P3);
 
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         const double io = i < 3 ? 1.0 : 0.0;
-        S[i] = k.sP1 * P1[i] + k.sP2 * P2[i] + k.sP3 * P3[i] + k.sI * io + k.sT * t[i];
+        S[i] = k.sP1 * P1[i] + k.sP3 * P3[i] + k.sI * io + k.sT * t[i];
     }
     if (WITH_DS) {
-        // W_u(j) = sum_v C_uv v(j) for u in (P1,P2,P3,iota,t)
-        double W1[6], W2[6], W3[6], WI[6], WT[6];
+        // W_u(j) = sum_v C_uv v(j) for u in (P1,P3,iota,t)
+        double W1[6], W3[6], WI[6], WT[6];
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
             const double io = j < 3 ? 1.0 : 0.0;
             W1[j] = k.c11 * P1[j] + k.c13 * P3[j] + k.c1I * io + k.c1T * t[j];
-            W2[j] = k.c22 * P2[j];
             W3[j] = k.c13 * P1[j] + k.c33 * P3[j] + k.c3I * io + k.c3T * t[j];
             WI[j] = k.c1I * P1[j] + k.c3I * P3[j] + k.cII * io + k.cIT * t[j];
             WT[j] = k.c1T * P1[j] + k.c3T * P3[j] + k.cIT * io;
@@ -231,7 +235,7 @@ __host__ __device__ __forceinline__ int mc3d_point(const double eps[6], double c
 #pragma unroll
             for (int j = i; j < 6; ++j) {
                 const double id = (i == j) ? (i < 3 ? 1.0 : 0.5) : 0.0;
-                double v = P1[i] * W1[j] + P2[i] * W2[j] + P3[i] * W3[j] + t[i] * WT[j];
+                double v = P1[i] * W1[j] + P3[i] * W3[j] + t[i] * WT[j];
                 if (i < 3) v += WI[j];
                 v += k.a * dsq3(i, j, t) + k.b * id;
                 D21[q++] = v;
@@ -310,7 +314,7 @@ __global__ __launch_bounds__(64) void k_mc3d_aos_lds(int64_t n, const double *__
                                                      int8_t *__restrict__ branch)
 {
     constexpr int W = 64;
-    __shared__ double sbuf[W * 21];
+    __shared__ double sbuf[WITH_DS ? W * 21 : W * 6];
     const int lane = threadIdx.x;
     const int64_t p0 = (int64_t)blockIdx.x * W;
     const int cnt = (int)((n - p0) < W ? (n - p0) : W);
@@ -356,6 +360,310 @@ __global__ __launch_bounds__(64) void k_mc3d_aos_lds(int64_t n, const double *__
         }
     }
     if (branch && lane < cnt) branch[p] = (int8_t)br;
+}
+
+// Same as k_mc3d_aos_lds with half the LDS: DS is staged and written out in
+// two phases of 32 points (5.4 kB per wavefront instead of 10.75 kB), which
+// roughly doubles the wavefronts an LDS-limited compute unit can hold.
+template <bool WITH_DS, bool NT, int MINW = 1>
+__global__ __launch_bounds__(64, MINW) void k_mc3d_aos_half(int64_t n, const double *__restrict__ E,
+                                                      const double *__restrict__ cbar,
+                                                      const double *__restrict__ sphi,
+                                                      const double *__restrict__ shear,
+                                                      const double *__restrict__ lame,
+                                                      double *__restrict__ S, double *__restrict__ DS,
+                                                      int8_t *__restrict__ branch)
+{
+    constexpr int W = 64, H = 32;
+    __shared__ double sbuf[H * 21];  // >= W*6 for the E / S staging
+    const int lane = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * W;
+    const int cnt = (int)((n - p0) < W ? (n - p0) : W);
+    const int64_t p = p0 + lane;
+    {
+        const double2 *Eg = reinterpret_cast<const double2 *>(E + 6 * p0);
+        double2 *s2 = reinterpret_cast<double2 *>(sbuf);
+        for (int t = lane; t < cnt * 3; t += W) s2[t] = Eg[t];
+    }
+    __syncthreads();
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    double cb = 1.0, sp = 0.5, G = 1.0, L = 1.0;
+    if (lane < cnt) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) eps[i] = sbuf[lane * 6 + i];
+        cb = cbar[p];
+        sp = sphi[p];
+        G = shear[p];
+        L = lame[p];
+    }
+    double s[6], d[21];
+    const int br = mc3d_point<WITH_DS>(eps, cb, sp, G, L, s, d);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
+    __syncthreads();
+    {
+        double *Sg = S + 6 * p0;
+        for (int t = lane; t < cnt * 3; t += W) store2<NT>(Sg + 2 * t, sbuf[2 * t], sbuf[2 * t + 1]);
+    }
+    if (WITH_DS) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ch = cnt - h * H < H ? (cnt - h * H > 0 ? cnt - h * H : 0) : H;
+            __syncthreads();
+            if (lane >= h * H && lane < h * H + H) {
+#pragma unroll
+                for (int k = 0; k < 21; ++k) sbuf[(lane - h * H) * 21 + k] = d[k];
+            }
+            __syncthreads();
+            double *Dg = DS + 36 * (p0 + h * H);
+            for (int t = lane; t < ch * 18; t += W) {
+                const int o = 2 * t, q = o / 36, c = o - 36 * q;
+                const int i0 = c % 6, j0 = c / 6, i1 = (c + 1) % 6, j1 = (c + 1) / 6;
+                const double v0 = sbuf[q * 21 + (i0 <= j0 ? sym6(i0, j0) : sym6(j0, i0))];
+                const double v1 = sbuf[q * 21 + (i1 <= j1 ? sym6(i1, j1) : sym6(j1, i1))];
+                store2<NT>(Dg + o, v0, v1);
+            }
+        }
+    }
+    if (branch && lane < cnt) branch[p] = (int8_t)br;
+}
+
+// Half-staged form whose LDS rows hold each point's full column-major 6x6
+// block (36 doubles) instead of the 21 packed entries: the symmetric
+// expansion happens once per lane at compile-time offsets, and the write-out
+// is a plain 16-byte copy of the contiguous span (no per-element index
+// arithmetic, which dominated the VALU count of the packed form).
+template <bool WITH_DS, bool NT>
+__global__ __launch_bounds__(64) void k_mc3d_x36(int64_t n, const double *__restrict__ E,
+                                                 const double *__restrict__ cbar,
+                                                 const double *__restrict__ sphi,
+                                                 const double *__restrict__ shear,
+                                                 const double *__restrict__ lame,
+                                                 double *__restrict__ S, double *__restrict__ DS,
+                                                 int8_t *__restrict__ branch)
+{
+    constexpr int W = 64, H = 32;
+    __shared__ __align__(16) double sbuf[WITH_DS ? H * 36 : W * 6];  // H*36 >= W*6
+    const int lane = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * W;
+    const int cnt = (int)((n - p0) < W ? (n - p0) : W);
+    const int64_t p = p0 + lane;
+    {
+        const double2 *Eg = reinterpret_cast<const double2 *>(E + 6 * p0);
+        double2 *s2 = reinterpret_cast<double2 *>(sbuf);
+        for (int t = lane; t < cnt * 3; t += W) s2[t] = Eg[t];
+    }
+    __syncthreads();
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    double cb = 1.0, sp = 0.5, G = 1.0, L = 1.0;
+    if (lane < cnt) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) eps[i] = sbuf[lane * 6 + i];
+        cb = cbar[p];
+        sp = sphi[p];
+        G = shear[p];
+        L = lame[p];
+    }
+    double s[6], d[21];
+    const int br = mc3d_point<WITH_DS>(eps, cb, sp, G, L, s, d);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
+    __syncthreads();
+    {
+        double *Sg = S + 6 * p0;
+        for (int t = lane; t < cnt * 3; t += W) store2<NT>(Sg + 2 * t, sbuf[2 * t], sbuf[2 * t + 1]);
+    }
+    if (WITH_DS) {
+        const dvec2 *src = reinterpret_cast<const dvec2 *>(sbuf);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ch = cnt - h * H < H ? (cnt - h * H > 0 ? cnt - h * H : 0) : H;
+            __syncthreads();
+            if (lane >= h * H && lane < h * H + H) {
+                double *row = sbuf + (lane - h * H) * 36;
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) row[i + 6 * j] = d[i <= j ? sym6(i, j) : sym6(j, i)];
+            }
+            __syncthreads();
+            double *Dg = DS + 36 * (p0 + h * H);
+            const int m = ch * 18;
+#pragma unroll
+            for (int k = 0; k < H * 18 / W; ++k) {
+                const int t = lane + W * k;
+                if (t < m) {
+                    const dvec2 v = src[t];
+                    store2<NT>(Dg + 2 * t, v.x, v.y);
+                }
+            }
+        }
+    }
+    if (branch && lane < cnt) branch[p] = (int8_t)br;
+}
+
+// k_mc3d_x36 with the inputs read straight into registers: every load of
+// the wavefront (each lane's 48-byte strain row as three 16-byte loads, and
+// the four material values) is issued up front, so a single memory latency
+// precedes the arithmetic and no LDS round trip or barrier is spent on E.
+// DS is staged in phases of H points.
+template <bool WITH_DS, bool NT, int H>
+__global__ __launch_bounds__(64) void k_mc3d_x36d(int64_t n, const double *__restrict__ E,
+                                                  const double *__restrict__ cbar,
+                                                  const double *__restrict__ sphi,
+                                                  const double *__restrict__ shear,
+                                                  const double *__restrict__ lame,
+                                                  double *__restrict__ S, double *__restrict__ DS,
+                                                  int8_t *__restrict__ branch)
+{
+    constexpr int W = 64;
+    static_assert(W % H == 0, "phase size");
+    __shared__ __align__(16) double sbuf[WITH_DS ? (H * 36 > W * 6 ? H * 36 : W * 6) : W * 6];
+    const int lane = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * W;
+    const int cnt = (int)((n - p0) < W ? (n - p0) : W);
+    const int64_t p = p0 + lane;
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    double cb = 1.0, sp = 0.5, G = 1.0, L = 1.0;
+    if (lane < cnt) {
+        const double2 *e2 = reinterpret_cast<const double2 *>(E + 6 * p);
+        const double2 a = e2[0], b = e2[1], c = e2[2];
+        cb = cbar[p];
+        sp = sphi[p];
+        G = shear[p];
+        L = lame[p];
+        eps[0] = a.x; eps[1] = a.y; eps[2] = b.x; eps[3] = b.y; eps[4] = c.x; eps[5] = c.y;
+    }
+    double s[6], d[21];
+    const int br = mc3d_point<WITH_DS>(eps, cb, sp, G, L, s, d);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
+    __syncthreads();
+    {
+        double *Sg = S + 6 * p0;
+        for (int t = lane; t < cnt * 3; t += W) store2<NT>(Sg + 2 * t, sbuf[2 * t], sbuf[2 * t + 1]);
+    }
+    if (WITH_DS) {
+        const dvec2 *src = reinterpret_cast<const dvec2 *>(sbuf);
+#pragma unroll
+        for (int h = 0; h < W / H; ++h) {
+            const int ch = cnt - h * H < H ? (cnt - h * H > 0 ? cnt - h * H : 0) : H;
+            __syncthreads();
+            if (lane >= h * H && lane < h * H + H) {
+                double *row = sbuf + (lane - h * H) * 36;
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) row[i + 6 * j] = d[i <= j ? sym6(i, j) : sym6(j, i)];
+            }
+            __syncthreads();
+            double *Dg = DS + 36 * (p0 + h * H);
+            const int m = ch * 18;
+#pragma unroll
+            for (int k = 0; k < (H * 18 + W - 1) / W; ++k) {
+                const int t = lane + W * k;
+                if (t < m) {
+                    const dvec2 v = src[t];
+                    store2<NT>(Dg + 2 * t, v.x, v.y);
+                }
+            }
+        }
+    }
+    if (branch && lane < cnt) branch[p] = (int8_t)br;
+}
+
+// Persistent, software-pipelined form of k_mc3d_aos_half: each wavefront
+// walks 64-point tiles with the grid as stride and issues the loads of its
+// next tile (the contiguous strain span and the four material values) before
+// the arithmetic of the current one.  The load latency and the draining of
+// the previous tile's stores then overlap the return mapping instead of
+// waiting for other wavefronts to cover them.
+template <bool WITH_DS, bool NT, int MINW>
+__global__ __launch_bounds__(64, MINW) void k_mc3d_pipe(int64_t n, int64_t ntiles, const double *__restrict__ E,
+                                                        const double *__restrict__ cbar,
+                                                        const double *__restrict__ sphi,
+                                                        const double *__restrict__ shear,
+                                                        const double *__restrict__ lame,
+                                                        double *__restrict__ S, double *__restrict__ DS,
+                                                        int8_t *__restrict__ branch)
+{
+    constexpr int W = 64, H = 32;
+    __shared__ double sbuf[H * 21];  // >= 3*W double2 of the strain span
+    const int lane = threadIdx.x;
+    int64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    double2 pe0, pe1, pe2;
+    double pcb, psp, pG, pL;
+    auto fetch = [&](int64_t tl) {
+        const int64_t q0 = tl * W;
+        const int c = (int)((n - q0) < W ? (n - q0) : W);
+        const int m = 3 * c;
+        const double2 *Eg = reinterpret_cast<const double2 *>(E + 6 * q0);
+        const double2 z = make_double2(0.0, 0.0);
+        pe0 = lane < m ? Eg[lane] : z;
+        pe1 = lane + W < m ? Eg[lane + W] : z;
+        pe2 = lane + 2 * W < m ? Eg[lane + 2 * W] : z;
+        if (lane < c) {
+            const int64_t q = q0 + lane;
+            pcb = cbar[q];
+            psp = sphi[q];
+            pG = shear[q];
+            pL = lame[q];
+        } else {
+            pcb = 1.0; psp = 0.5; pG = 1.0; pL = 1.0;
+        }
+    };
+    fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int64_t p0 = tile * W;
+        const int cnt = (int)((n - p0) < W ? (n - p0) : W);
+        const int64_t p = p0 + lane;
+        double2 *s2 = reinterpret_cast<double2 *>(sbuf);
+        __syncthreads();
+        s2[lane] = pe0;
+        s2[lane + W] = pe1;
+        s2[lane + 2 * W] = pe2;
+        __syncthreads();
+        double eps[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) eps[i] = sbuf[lane * 6 + i];
+        const double cb = pcb, sp = psp, G = pG, L = pL;
+        const int64_t nxt = tile + gridDim.x;
+        if (nxt < ntiles) fetch(nxt);
+        double s[6], d[21];
+        const int br = mc3d_point<WITH_DS>(eps, cb, sp, G, L, s, d);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
+        __syncthreads();
+        {
+            double *Sg = S + 6 * p0;
+            for (int t = lane; t < cnt * 3; t += W) store2<NT>(Sg + 2 * t, sbuf[2 * t], sbuf[2 * t + 1]);
+        }
+        if (WITH_DS) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int ch = cnt - h * H < H ? (cnt - h * H > 0 ? cnt - h * H : 0) : H;
+                __syncthreads();
+                if (lane >= h * H && lane < h * H + H) {
+#pragma unroll
+                    for (int k = 0; k < 21; ++k) sbuf[(lane - h * H) * 21 + k] = d[k];
+                }
+                __syncthreads();
+                double *Dg = DS + 36 * (p0 + h * H);
+                for (int t = lane; t < ch * 18; t += W) {
+                    const int o = 2 * t, q = o / 36, c = o - 36 * q;
+                    const int i0 = c % 6, j0 = c / 6, i1 = (c + 1) % 6, j1 = (c + 1) / 6;
+                    const double v0 = sbuf[q * 21 + (i0 <= j0 ? sym6(i0, j0) : sym6(j0, i0))];
+                    const double v1 = sbuf[q * 21 + (i1 <= j1 ? sym6(i1, j1) : sym6(j1, i1))];
+                    store2<NT>(Dg + o, v0, v1);
+                }
+            }
+        }
+        if (branch && lane < cnt) branch[p] = (int8_t)br;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -584,12 +892,49 @@ extern "C" int ssr_dev_constitutive_3D(int64_t n, const double *E, const double 
             hipLaunchKernelGGL((k_mc3d_aos_lds<false, false>), dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
                                E, c_bar, sin_phi, shear, lame, S, DS, branch);
     } else {
-        if (DS)
-            hipLaunchKernelGGL((k_mc3d_aos_lds<true, true>), dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
-                               E, c_bar, sin_phi, shear, lame, S, DS, branch);
-        else
-            hipLaunchKernelGGL((k_mc3d_aos_lds<false, true>), dim3(ssr_blocks(n, 64)), dim3(64), 0, ssr_stream(), n,
-                               E, c_bar, sin_phi, shear, lame, S, DS, branch);
+        const int v = ssr_tuning(SSR_TUNE_CONSTITUTIVE_VARIANT);
+        const dim3 grid(ssr_blocks(n, 64)), blk(64);
+#define MC3(K_) hipLaunchKernelGGL(K_, grid, blk, 0, ssr_stream(), n, E, c_bar, sin_phi, shear, lame, S, DS, branch)
+        const int64_t ntiles = (n + 63) / 64;
+        const int wpc = ssr_tuning(SSR_TUNE_PERSIST_WAVES);
+        auto pgrid = [&](int minw) {
+            const int64_t g = (int64_t)ssr_cu_count() * (wpc > 0 ? wpc : 4 * minw);
+            return dim3((unsigned)(g < ntiles ? g : ntiles));
+        };
+#define MCP(K_, MW_) hipLaunchKernelGGL(K_, pgrid(MW_), blk, 0, ssr_stream(), n, ntiles, E, c_bar, sin_phi, shear, lame, S, DS, branch)
+        if (v == 2) {
+            if (DS) MC3((k_mc3d_aos_lds<true, true>));
+            else MC3((k_mc3d_aos_lds<false, true>));
+        } else if (v == 3) {
+            if (DS) MC3((k_mc3d_aos_half<true, false>));
+            else MC3((k_mc3d_aos_half<false, false>));
+        } else if (v == 4) {
+            if (DS) MC3((k_mc3d_aos_half<true, true>));
+            else MC3((k_mc3d_aos_half<false, true>));
+        } else if (v == 5) {
+            if (DS) MCP((k_mc3d_pipe<true, true, 1>), 4);
+            else MCP((k_mc3d_pipe<false, true, 1>), 4);
+        } else if (v == 6) {
+            if (DS) MCP((k_mc3d_pipe<true, true, 5>), 5);
+            else MCP((k_mc3d_pipe<false, true, 5>), 5);
+        } else if (v == 7) {
+            if (DS) MC3((k_mc3d_aos_half<true, true, 5>));
+            else MC3((k_mc3d_aos_half<false, true, 5>));
+        } else if (v == 8) {
+            if (DS) MC3((k_mc3d_x36<true, false>));
+            else MC3((k_mc3d_x36<false, false>));
+        } else if (v == 9) {
+            if (DS) MC3((k_mc3d_x36<true, true>));
+            else MC3((k_mc3d_x36<false, true>));
+        } else if (v == 10) {
+            if (DS) MC3((k_mc3d_x36d<true, true, 32>));
+            else MC3((k_mc3d_x36d<false, true, 32>));
+        } else {
+            if (DS) MC3((k_mc3d_x36d<true, true, 16>));
+            else MC3((k_mc3d_x36d<false, true, 16>));
+        }
+#undef MCP
+#undef MC3
     }
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;

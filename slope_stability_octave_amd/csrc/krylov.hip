@@ -158,6 +158,13 @@ __global__ void k_lincomb(int64_t n, double a, const double *__restrict__ x, dou
     if (i < n) z[i] = a * x[i] + b * y[i];
 }
 
+// x = v (write-only; also the graph-safe way to clear a buffer)
+__global__ void k_fill(int64_t n, double v, double *__restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
 __global__ void k_mul(int64_t n, const double *__restrict__ m, double *__restrict__ x)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -310,6 +317,14 @@ int ssr_dev_lincomb(int64_t n, double a, const double *x, double b, const double
 {
     if (n == 0) return SSR_OK;
     hipLaunchKernelGGL(k_lincomb, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, a, x, b, y, z);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+int ssr_dev_fill(int64_t n, double v, double *x)
+{
+    if (n == 0) return SSR_OK;
+    hipLaunchKernelGGL(k_fill, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, v, x);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

@@ -40,6 +40,7 @@ static inline unsigned ssr_blocks(int64_t n, int threads)
 }
 
 hipStream_t ssr_stream(void);
+int ssr_cu_count(void);  // compute units of the current device (cached)
 
 // Preconditioner interface used by the Krylov solvers (solver.cpp): block
 // Jacobi and smoothed-aggregation AMG implement it.
@@ -50,7 +51,8 @@ struct ssr_precond {
 };
 
 // Kernel-variant / launch-shape knobs (set by the autotuner or the caller).
-#define SSR_TUNE_CONSTITUTIVE_VARIANT 0 /* 0 direct point-major stores, 1 LDS-staged (default) */
+#define SSR_TUNE_CONSTITUTIVE_VARIANT 0 /* 3D return-mapping kernel form, see constitutive.hip */
 #define SSR_TUNE_SPMV_GROUP 1           /* lanes per block row in the node-block SpMV: 4, 8, 16 */
+#define SSR_TUNE_PERSIST_WAVES 2        /* wavefronts per compute unit of persistent grids (0: auto) */
 #define SSR_TUNE_COUNT 8
 int ssr_tuning(int key);

@@ -45,6 +45,7 @@ _lib.extend_signatures({
     "ssr_dev_axpby": [_i64, _dbl, _vp, _dbl, _vp],
     "ssr_dev_lincomb": [_i64, _dbl, _vp, _dbl, _vp, _vp],
     "ssr_dev_mul": [_i64, _vp, _vp],
+    "ssr_dev_fill": [_i64, _dbl, _vp],
     "ssr_dev_block_jacobi_setup": [_int, _i64, _vp, _vp, _vp],
     "ssr_dev_block_jacobi_apply": [_int, _i64, _vp, _vp, _vp, _vp],
     "ssr_dev_csr_spmv": [_i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
@@ -65,6 +66,11 @@ def dot(a, b, mask=None) -> float:
 def lincomb(a, x, b, y, z):
     """z = a*x + b*y on device."""
     call("ssr_dev_lincomb", int(np.prod(z.shape)), float(a), ptr(x), float(b), ptr(y), ptr(z))
+
+
+def fill(x, v=0.0):
+    """x[:] = v on the device (write-only kernel)."""
+    call("ssr_dev_fill", int(np.prod(x.shape)), float(v), ptr(x))
 
 
 def axpby(a, x, b, y):

@@ -178,3 +178,30 @@ def mesh_P2_box(nx, ny, nz, Lx, Ly, Lz):
     Q[1] = coord[1] > 0
     Q[2] = (coord[2] > 0) & (coord[1] > 0) & (coord[2] < Lz)
     return coord, elem, Q
+
+
+def load_mesh_P2(data, boundary_type=0):
+    """Mesh in the HDF5 layout of the reference's meshes (datasets node, elem,
+    face, boundary, material; 0-based), as read by load_mesh_P2.m:25-57:
+    Dirichlet mask from the boundary labels (1,2 -> x; 3,4 -> z; 5 -> y, or
+    all components when boundary_type), then the axis permutation [1 3 2]
+    that makes the second coordinate vertical.  ``data`` is a mapping of the
+    five arrays as stored in the file (row-major: node n_n x 3, elem n_e x 10,
+    face n_f x 6).  Returns coord (3 x n_n), elem (10 x n_e), surf, Q (3 x n_n),
+    material (n_e)."""
+    node = np.asarray(data["node"], dtype=np.float64).T
+    elem = np.asarray(data["elem"]).astype(np.int64).T
+    face = np.asarray(data["face"]).astype(np.int64).T
+    boundary = np.asarray(data["boundary"]).ravel()
+    material = np.asarray(data["material"]).ravel().astype(np.int64)
+    Q = np.ones(node.shape, dtype=bool)
+    for lab, comp in ((1, 0), (2, 0), (3, 1), (4, 1)):
+        Q[comp, np.unique(face[:, boundary == lab])] = False
+    f5 = np.unique(face[:, boundary == 5])
+    if boundary_type:
+        Q[:, f5] = False
+    else:
+        Q[2, f5] = False
+    coord = node[[0, 2, 1], :]
+    Q = Q[[0, 2, 1], :]
+    return coord, elem, face, Q, material

@@ -102,3 +102,16 @@ def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", 
     return dict(FoS=float(lam_h[-1]), lambda_hist=lam_h, omega_hist=om_h, Umax_hist=umax_h, stats=stats,
                 U=U, setup_s=t_setup, time_to_fos_s=t_run, timings=timings, counts=counts,
                 newton_iterations=int(sum(stats["attempt_newton_iterations"])))
+
+
+# slope_stability_3D_homo_SSR.m:118-140 (recorded run: ss/slope_stability_3D_homo_SSR.ipynb:366-410)
+PARAMS_3D_NOTEBOOK = dict(PARAMS_3D, omega_max_stop=3000.0)
+
+
+def slope_3d_mesh(data, materials=None, mat_id=None):
+    """3D problem on a mesh in the reference's HDF5 layout (see
+    mesh.load_mesh_P2), homogeneous unless ``mat_id`` is given."""
+    coord, elem, surf, Q, material = mesh.load_mesh_P2(data)
+    mats = materials or HOMO_2D
+    mid = np.zeros(elem.shape[1], dtype=np.int64) if mat_id is None else np.asarray(mat_id)
+    return _finish(coord, elem, Q, 3, mats, mid)
