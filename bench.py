@@ -97,9 +97,13 @@ def run_fos(args):
         params["linear_tol"] = args.linear_tol
     log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
     amg_opts = {"rebuild_every": args.amg_rebuild_every} if args.amg_rebuild_every is not None else None
-    res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts)
+    if args.amg_theta is not None:
+        amg_opts = dict(amg_opts or {}, theta=args.amg_theta)
+    res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts,
+                           precision=args.precision)
     out = {
-        "metric": "SSR time-to-FoS (plain GPU form, FP64)",
+        "metric": "SSR time-to-FoS (%s)" % ("plain GPU form, FP64" if args.precision == 64
+                                             else "FP32 Krylov/AMG + FP64 refinement"),
         "value": res["time_to_fos_s"],
         "unit": "s",
         "higher_is_better": False,
@@ -109,7 +113,8 @@ def run_fos(args):
         "config": {"workload": name,
                    "free_dof": prob["n_dof_free"], "int_points": prob["n_int"],
                    "lambda_init": params["lambda_init"], "omega_max_stop": params["omega_max_stop"],
-                   "preconditioner": args.precond},
+                   "preconditioner": args.precond, "krylov_precision": args.precision,
+                   "linear_tol": params["linear_tol"]},
         "FoS": res["FoS"],
         "steps": len(res["lambda_hist"]),
         "newton_iterations": res["newton_iterations"],
@@ -141,6 +146,9 @@ def main():
                     help="rebuild the AMG hierarchy every n-th Newton iteration (0: once per continuation step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size first (counter calibration)")
+    ap.add_argument("--precision", type=int, default=64, choices=[32, 64],
+                    help="Krylov/preconditioner arithmetic of the FoS workloads (32: float with double refinement)")
+    ap.add_argument("--amg-theta", type=float, default=None, help="AMG strength threshold (FoS workloads)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="also time write-only, copy and read-only streaming kernels of the DS size")
     args = ap.parse_args()

@@ -183,6 +183,14 @@ int ssr_solver_create(const ssr_pattern *P, int maxit, double tol, double defl_t
 int ssr_solver_destroy(ssr_solver *s);
 /* operator values (device, borrowed); setup_precond = 1 rebuilds/refreshes M */
 int ssr_solver_set_matrix(ssr_solver *s, const double *Kvals, int setup_precond);
+/* Arithmetic of the Krylov iterations: 64 = double throughout (the reference's
+ * FP64 DFGMRES); 32 = float Arnoldi, preconditioner and deflation projector on
+ * float copies of K and W, with x0, the true residual and x += Z y in double
+ * and repeated float solves on the double residual until the relative
+ * residual meets the tolerance.  No reference counterpart (the reference runs
+ * FP64 only); call before set_matrix. */
+int ssr_solver_set_precision(ssr_solver *s, int bits);
+int ssr_solver_get_precision(const ssr_solver *s, int *bits);
 /* near_null: n_null columns over all dim*n_n DOFs (host, column-major);
  * opts: NULL or double[8] {theta, max_levels, max_coarse, cheb_degree, cheb_lower,
  * cheb_upper, refresh_every, rebuild_every} */

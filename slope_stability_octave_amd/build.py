@@ -2,7 +2,8 @@
 
 All HIP sources under csrc/ are compiled into one shared library next to this
 file.  Flags: -O3, gfx950 only, packed-FP32 instructions disabled (the compiler
-otherwise forms them from plain float code).
+otherwise forms them from plain float code); the host drivers (*.cpp) are
+built with OpenMP for the threaded AMG setup.
 """
 from __future__ import annotations
 
@@ -39,14 +40,15 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(CSRC, os.path.basename(src) + ".o")
         cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", src, "-o", obj, "-I", CSRC]
         if src.endswith(".cpp"):
-            cmd += ["-x", "hip"]
-            cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-x", "hip", "-c", src, "-o", obj, "-I", CSRC]
+            # host-side drivers (AMG setup uses OpenMP threads)
+            cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-fopenmp", "-x", "hip", "-c", src, "-o", obj,
+                                                                   "-I", CSRC]
         if verbose:
             print(" ".join(cmd), file=sys.stderr)
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     for o in objs:

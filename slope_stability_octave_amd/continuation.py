@@ -8,7 +8,11 @@ shrinks (:214-216) and the stop rules (:203-211, :220-227).  The factor of
 safety is lambda_hist[-1].
 
 ``predictor`` (optional) replaces the secant extrapolation: a callable
-(history) -> U_ini or None; None falls back to the secant step.
+(state) -> U_ini, (U_ini, lambda_ini) or None; None falls back to the secant
+step.  ``state`` holds the last converged fields (``U_hist``, newest first,
+at most four), the histories and the target omega.  ``recorder`` (optional)
+is called after every accepted step as recorder(state, U_new, lambda_new)
+with the state the step started from (training-data capture).
 """
 from __future__ import annotations
 
@@ -78,7 +82,7 @@ def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_mi
 
 def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
                               omega_max_stop, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
-                              linear_system_solver, log=None, predictor=None):
+                              linear_system_solver, log=None, predictor=None, recorder=None):
     """Returns (U, lambda_hist, omega_hist, Umax_hist, stats)."""
     t_total = time.perf_counter()
     ws = Workspace(cmb.n_n, cmb.dim)
@@ -103,16 +107,22 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
     n_omega, n_omega_max = 0, 5
     U_ini = DeviceArray(U.shape)
     history = dict(lambda_hist=lambda_hist, omega_hist=omega_hist)
+    U_hist = [U, U_old]  # converged fields, newest first (at most four kept)
     while True:
         omega_it = min(omega + d_omega, omega_max_stop)
         d_omega = omega_it - omega
         if log:
             log(f"Step {step + 1}: omega_target={omega_it:.6g}, d_omega={d_omega:.6g}")
         used = False
+        lambda_ini = lam
+        state = dict(U=U, U_old=U_old, omega=omega, omega_old=omega_old, omega_it=omega_it, lam=lam,
+                     history=history, U_hist=list(U_hist), lambda_hist=list(lambda_hist),
+                     omega_hist=list(omega_hist))
         if predictor is not None:
-            guess = predictor(dict(U=U, U_old=U_old, omega=omega, omega_old=omega_old, omega_it=omega_it,
-                                   lam=lam, history=history))
+            guess = predictor(state)
             if guess is not None:
+                if isinstance(guess, tuple):
+                    guess, lambda_ini = guess
                 lincomb(1.0, guess, 0.0, guess, U_ini)
                 used = True
         if not used:
@@ -122,7 +132,7 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
         stats["predictor_used"].append(used)
         t0 = time.perf_counter()
         U_it, lambda_it, flag, it_newt, hist = newton_ind_SSR(
-            U_ini, omega_it, lam, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
+            U_ini, omega_it, lambda_ini, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
             linear_system_solver.copy(), ws, log)
         wall = time.perf_counter() - t0
         stats["attempt_step"].append(step + 1)
@@ -139,8 +149,11 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
             d_omega = d_omega / 2
             n_omega += 1
         else:
+            if recorder is not None:
+                recorder(state, U_it, lambda_it)
             step += 1
             U_old, U = U, U_it
+            U_hist = [U_it] + U_hist[:3]
             linear_system_solver.expand_deflation_basis(U)
             omega_old, omega = omega, omega_it
             d_lambda = lambda_it - lam

@@ -23,10 +23,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <omp.h>
+
 #include <algorithm>
+#include <chrono>
 #include <memory>
 #include <vector>
 
+#include "linalg.h"
 #include "ssr_common.h"
 
 struct ssr_pattern;
@@ -39,21 +43,26 @@ void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_
 
 extern "C" {
 int ssr_pattern_get(const ssr_pattern *p, int which, void *dst);
-int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const double *val,
-                     const double *x, double *y, double alpha, double beta);
-int ssr_dev_csr_spmv(int64_t n_rows, const int64_t *rp, const int32_t *ci, const double *v, const double *x,
-                     double *y, double alpha, double beta);
-int ssr_dev_ewmul(int64_t n, const double *a, const double *b, double *z);
-int ssr_dev_cheb_step(int64_t n, double c2, double c3, const double *dinv, const double *r, double *d, double *x);
-int ssr_dev_dense_gemv(int n, const double *M, const double *x, double *y);
-int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y);
-int ssr_dev_mul(int64_t n, const double *m, double *x);
 int ssr_solver_set_precond(ssr_solver *s, ssr_precond *M);
 const ssr_pattern *ssr_solver_pattern(const ssr_solver *s);
 }
 
 
 namespace {
+
+// Wall-clock phase totals of the host setup (printed with SSR_AMG_VERBOSE).
+struct PhaseClock {
+    double t[12] = {0};
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    void lap(int k)
+    {
+        const auto t1 = std::chrono::steady_clock::now();
+        t[k] += std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+    }
+};
+const char *kPhaseNames[12] = {"download", "fine_csr", "rho", "aggregate", "tentative", "A*T",
+                               "P", "R=P'", "RAP", "upload", "coarse_inv", "other"};
 
 struct HCsr {
     int64_t nr = 0, nc = 0;
@@ -82,37 +91,63 @@ HCsr transpose(const HCsr &A)
     return T;
 }
 
-// C = A*B, Gustavson with a dense accumulator; columns sorted per row.
+// C = A*B, Gustavson with a dense accumulator per thread; columns sorted per
+// row.  Rows are independent, so the result does not depend on the thread
+// count.
 HCsr spgemm(const HCsr &A, const HCsr &B)
 {
     HCsr C;
     C.nr = A.nr;
     C.nc = B.nc;
     C.rp.assign(C.nr + 1, 0);
-    std::vector<int64_t> mark(B.nc, -1);
-    std::vector<double> acc(B.nc, 0.0);
-    std::vector<int32_t> cols;
-    for (int64_t r = 0; r < A.nr; ++r) {
-        cols.clear();
-        for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) {
-            const int32_t j = A.ci[k];
-            const double a = A.v[k];
-            for (int64_t t = B.rp[j]; t < B.rp[j + 1]; ++t) {
-                const int32_t c = B.ci[t];
-                if (mark[c] != r) {
-                    mark[c] = r;
-                    acc[c] = 0.0;
-                    cols.push_back(c);
+#pragma omp parallel
+    {
+        std::vector<int64_t> mark(B.nc, -1);
+#pragma omp for schedule(dynamic, 512)
+        for (int64_t r = 0; r < A.nr; ++r) {
+            int64_t c = 0;
+            for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) {
+                const int32_t j = A.ci[k];
+                for (int64_t t = B.rp[j]; t < B.rp[j + 1]; ++t)
+                    if (mark[B.ci[t]] != r) {
+                        mark[B.ci[t]] = r;
+                        c++;
+                    }
+            }
+            C.rp[r + 1] = c;
+        }
+    }
+    for (int64_t r = 0; r < C.nr; ++r) C.rp[r + 1] += C.rp[r];
+    C.ci.resize(C.rp[C.nr]);
+    C.v.resize(C.rp[C.nr]);
+#pragma omp parallel
+    {
+        std::vector<int64_t> mark(B.nc, -1);
+        std::vector<double> acc(B.nc, 0.0);
+        std::vector<int32_t> cols;
+#pragma omp for schedule(dynamic, 512)
+        for (int64_t r = 0; r < A.nr; ++r) {
+            cols.clear();
+            for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) {
+                const int32_t j = A.ci[k];
+                const double a = A.v[k];
+                for (int64_t t = B.rp[j]; t < B.rp[j + 1]; ++t) {
+                    const int32_t c = B.ci[t];
+                    if (mark[c] != r) {
+                        mark[c] = r;
+                        acc[c] = 0.0;
+                        cols.push_back(c);
+                    }
+                    acc[c] += a * B.v[t];
                 }
-                acc[c] += a * B.v[t];
+            }
+            std::sort(cols.begin(), cols.end());
+            int64_t o = C.rp[r];
+            for (int32_t c : cols) {
+                C.ci[o] = c;
+                C.v[o++] = acc[c];
             }
         }
-        std::sort(cols.begin(), cols.end());
-        for (int32_t c : cols) {
-            C.ci.push_back(c);
-            C.v.push_back(acc[c]);
-        }
-        C.rp[r + 1] = (int64_t)C.ci.size();
     }
     return C;
 }
@@ -134,6 +169,7 @@ void spgemm_numeric(const HCsr &A, const HCsr &B, HCsr &C)
 
 void spmv_host(const HCsr &A, const double *x, double *y)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < A.nr; ++r) {
         double s = 0.0;
         for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) s += A.v[k] * x[A.ci[k]];
@@ -151,11 +187,12 @@ int dupload(T **d, const std::vector<T> &h)
     return SSR_OK;
 }
 
+template <class T>
 struct DCsr {
     int64_t nr = 0;
     int64_t *rp = nullptr;
     int32_t *ci = nullptr;
-    double *v = nullptr;
+    T *v = nullptr;
     ~DCsr()
     {
         (void)hipFree(rp);
@@ -166,15 +203,29 @@ struct DCsr {
     {
         nr = A.nr;
         int rc;
-        if ((rc = dupload(&rp, A.rp)) || (rc = dupload(&ci, A.ci)) || (rc = dupload(&v, A.v))) return rc;
-        return SSR_OK;
+        if ((rc = dupload(&rp, A.rp)) || (rc = dupload(&ci, A.ci))) return rc;
+        std::vector<T> vt(A.v.begin(), A.v.end());
+        return dupload(&v, vt);
     }
-    int upload_values(const HCsr &A)
+    int spmv(const T *x, T *y, double a, double b, const T *z = nullptr) const
     {
-        SSR_HIP_CHECK(hipMemcpy(v, A.v.data(), sizeof(double) * A.v.size(), hipMemcpyHostToDevice));
-        return SSR_OK;
+        return ssr::spmv_csr<T>(nr, rp, ci, v, x, y, a, b, z);
     }
-    int spmv(const double *x, double *y, double a, double b) const { return ssr_dev_csr_spmv(nr, rp, ci, v, x, y, a, b); }
+};
+
+// Device copy of one level in precision T.
+template <class T>
+struct DevLevel {
+    DCsr<T> A, P, R;
+    T *dinv = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;
+    ~DevLevel()
+    {
+        (void)hipFree(dinv);
+        (void)hipFree(x);
+        (void)hipFree(b);
+        (void)hipFree(r);
+        (void)hipFree(d);
+    }
 };
 
 struct Level {
@@ -184,17 +235,12 @@ struct Level {
     std::vector<double> dinv;
     double rho = 1.0;    // power-iteration estimate of rho(D^-1 A) (prolongator smoothing)
     double rho_up = 1.0; // Gershgorin bound of rho(D^-1 A) (Chebyshev interval)
-    DCsr dA, dP, dR;
-    double *d_dinv = nullptr, *d_x = nullptr, *d_b = nullptr, *d_r = nullptr, *d_d = nullptr;
-    ~Level()
-    {
-        (void)hipFree(d_dinv);
-        (void)hipFree(d_x);
-        (void)hipFree(d_b);
-        (void)hipFree(d_r);
-        (void)hipFree(d_d);
-    }
+    DevLevel<double> d64;
+    DevLevel<float> d32;
+    template <class T> DevLevel<T> &dev();
 };
+template <> DevLevel<double> &Level::dev<double>() { return d64; }
+template <> DevLevel<float> &Level::dev<float>() { return d32; }
 
 // max_i sum_j |a_ij| |dinv_i|: an upper bound of the spectral radius of D^-1 A
 double gershgorin_rho(const HCsr &A, const std::vector<double> &dinv)
@@ -257,8 +303,11 @@ struct Amg : ssr_precond {
     std::vector<std::unique_ptr<Level>> L;
     std::vector<double> coarse_inv;
     double *d_coarse_inv = nullptr;
+    float *d_coarse_inv32 = nullptr;
     int64_t n_coarse = 0;
     const double *Kdev = nullptr;
+    const float *Kdev32 = nullptr;
+    bool want32 = false;  // keep a float copy of the device hierarchy
     std::vector<double> Kh;
     int64_t n_setup = 0;
     bool built = false;
@@ -280,7 +329,11 @@ struct Amg : ssr_precond {
             for (int64_t i = 0; i < n; ++i)
                 if (!mask[i]) B[(size_t)c * n + i] = 0.0;
     }
-    ~Amg() { (void)hipFree(d_coarse_inv); }
+    ~Amg()
+    {
+        (void)hipFree(d_coarse_inv);
+        (void)hipFree(d_coarse_inv32);
+    }
 
     void fine_from_bsr(HCsr &A)
     {
@@ -458,24 +511,47 @@ struct Amg : ssr_precond {
             for (int c = 0; c < kl; ++c) Bc[(size_t)c * nc + r] = bcv[(size_t)r * kl + c];
     }
 
+    template <class T>
     int alloc_level_vectors(Level &lv)
     {
         const int64_t m = lv.A.nr > 0 ? lv.A.nr : 1;
-        double **ps[4] = {&lv.d_x, &lv.d_b, &lv.d_r, &lv.d_d};
+        DevLevel<T> &dl = lv.dev<T>();
+        T **ps[4] = {&dl.x, &dl.b, &dl.r, &dl.d};
         for (auto p : ps) {
             (void)hipFree(*p);
             *p = nullptr;
-            SSR_HIP_CHECK(hipMalloc((void **)p, sizeof(double) * m));
+            SSR_HIP_CHECK(hipMalloc((void **)p, sizeof(T) * m));
         }
         return SSR_OK;
     }
 
+    // Device copy of level l (operator except on the finest level, P, R,
+    // D^-1, work vectors) in precision T.
+    template <class T>
+    int upload_level(size_t l)
+    {
+        Level &lv = *L[l];
+        DevLevel<T> &dl = lv.dev<T>();
+        int rc;
+        if (l > 0 && (rc = dl.A.upload(lv.A))) return rc;
+        if (l + 1 < L.size()) {
+            if ((rc = dl.P.upload(lv.P)) || (rc = dl.R.upload(lv.R))) return rc;
+        }
+        std::vector<T> di(lv.dinv.begin(), lv.dinv.end());
+        if ((rc = dupload(&dl.dinv, di))) return rc;
+        return alloc_level_vectors<T>(lv);
+    }
+
+    PhaseClock clk;
+
     int build()
     {
+        clk.lap(11);
         L.clear();
         L.emplace_back(new Level());
         Level &f = *L[0];
         fine_from_bsr(f.A);
+        clk.lap(1);
         f.node_ptr.resize(n_n + 1);
         for (int64_t i = 0; i <= n_n; ++i) f.node_ptr[i] = i * dim;
         std::vector<double> Bl = B;
@@ -486,17 +562,21 @@ struct Amg : ssr_precond {
             diag_inv(lv.A, lv.dinv);
             lv.rho = power_rho(lv.A, lv.dinv, 30);
             lv.rho_up = gershgorin_rho(lv.A, lv.dinv);
+            clk.lap(2);
             if (lv.A.nr <= opt.max_coarse || lev + 1 >= opt.max_levels) break;
             int32_t n_agg = 0;
             std::vector<int32_t> agg = aggregate(lv, n_agg);
+            clk.lap(3);
             if (n_agg == 0) break;
             HCsr T;
             std::vector<double> Bc;
             std::vector<int64_t> cnode;
             tentative(lv, Bl, kl, agg, n_agg, T, Bc, cnode);
+            clk.lap(4);
             if (T.nc == 0 || T.nc >= lv.A.nr * 0.9) break;
             // P = T - omega D^-1 A T, omega = 4/(3 rho)
             HCsr AT = spgemm(lv.A, T);
+            clk.lap(5);
             const double om = 4.0 / (3.0 * lv.rho);
             HCsr P;
             {
@@ -523,11 +603,14 @@ struct Amg : ssr_precond {
                     P.rp[r + 1] = (int64_t)P.ci.size();
                 }
             }
+            clk.lap(6);
             HCsr R = transpose(P);
+            clk.lap(7);
             HCsr AP = spgemm(lv.A, P);
             L.emplace_back(new Level());
             Level &c = *L[lev + 1];
             c.A = spgemm(R, AP);
+            clk.lap(8);
             c.node_ptr = cnode;
             lv.P = std::move(P);
             lv.R = std::move(R);
@@ -536,21 +619,21 @@ struct Amg : ssr_precond {
         }
         // device upload
         for (size_t l = 0; l < L.size(); ++l) {
-            Level &lv = *L[l];
-            if (l > 0 && (rc = lv.dA.upload(lv.A))) return rc;
-            if (l + 1 < L.size()) {
-                if ((rc = lv.dP.upload(lv.P)) || (rc = lv.dR.upload(lv.R))) return rc;
-            }
-            if ((rc = dupload(&lv.d_dinv, lv.dinv))) return rc;
-            if ((rc = alloc_level_vectors(lv))) return rc;
+            if ((rc = upload_level<double>(l))) return rc;
+            if (want32 && (rc = upload_level<float>(l))) return rc;
         }
+        clk.lap(9);
         if ((rc = build_coarse())) return rc;
+        clk.lap(10);
         built = true;
         if (getenv("SSR_AMG_VERBOSE")) {
             for (size_t l = 0; l < L.size(); ++l)
                 fprintf(stderr, "[amg] level %zu: n=%lld nnz=%lld nodes=%lld rho=%.3g rho_up=%.3g\n", l,
                         (long long)L[l]->A.nr, (long long)L[l]->A.ci.size(), (long long)L[l]->node_ptr.size() - 1,
                         L[l]->rho, L[l]->rho_up);
+            fprintf(stderr, "[amg] setup totals (s):");
+            for (int q = 0; q < 12; ++q) fprintf(stderr, " %s=%.3f", kPhaseNames[q], clk.t[q]);
+            fprintf(stderr, "\n");
         }
         return SSR_OK;
     }
@@ -566,14 +649,16 @@ struct Amg : ssr_precond {
             diag_inv(lv.A, lv.dinv);
             lv.rho = power_rho(lv.A, lv.dinv, 30);
             lv.rho_up = gershgorin_rho(lv.A, lv.dinv);
-            if ((rc = dupload(&lv.d_dinv, lv.dinv))) return rc;
             if (l + 1 < L.size()) {
                 Level &c = *L[l + 1];
                 HCsr AP = spgemm(lv.A, lv.P);
                 HCsr Ac = spgemm(lv.R, AP);
                 c.A = std::move(Ac);
-                if ((rc = c.dA.upload(c.A))) return rc;
             }
+        }
+        for (size_t l = 0; l < L.size(); ++l) {
+            if ((rc = upload_level<double>(l))) return rc;
+            if (want32 && (rc = upload_level<float>(l))) return rc;
         }
         return build_coarse();
     }
@@ -624,37 +709,60 @@ struct Amg : ssr_precond {
             }
         }
         coarse_inv.swap(I);
-        return dupload(&d_coarse_inv, coarse_inv);
+        int rc = dupload(&d_coarse_inv, coarse_inv);
+        if (rc || !want32) return rc;
+        std::vector<float> ci32(coarse_inv.begin(), coarse_inv.end());
+        return dupload(&d_coarse_inv32, ci32);
     }
 
-    int setup(const double *Kvals) override
+    int setup(const double *Kvals, const float *K32) override
     {
-        // The finest level always applies the current operator (Kvals on the
-        // device); the coarse hierarchy is rebuilt / refreshed on the host only
-        // when the schedule asks for it.
+        // The finest level always applies the current operator (Kvals / K32
+        // on the device); the coarse hierarchy is rebuilt / refreshed on the
+        // host only when the schedule asks for it.
         Kdev = Kvals;
+        Kdev32 = K32;
+        const bool need32 = K32 != nullptr && !want32;
+        if (K32) want32 = true;
         const bool rebuild = !built || (opt.rebuild_every > 0 && n_setup % opt.rebuild_every == 0);
         const bool refresh_now = !rebuild && opt.refresh_every > 0 && n_setup % opt.refresh_every == 0;
         int rc = SSR_OK;
         if (rebuild || refresh_now) {
+            clk.lap(11);
             Kh.resize((size_t)nnzb * dim * dim);
             SSR_HIP_CHECK(hipMemcpy(Kh.data(), Kvals, sizeof(double) * Kh.size(), hipMemcpyDeviceToHost));
+            clk.lap(0);
             rc = rebuild ? build() : refresh();
+        } else if (need32) {
+            // first float use of an existing hierarchy
+            for (size_t l = 0; l < L.size() && !rc; ++l) rc = upload_level<float>(l);
+            if (!rc) {
+                std::vector<float> ci32(coarse_inv.begin(), coarse_inv.end());
+                rc = dupload(&d_coarse_inv32, ci32);
+            }
         }
         n_setup++;
         return rc;
     }
 
-    int level_spmv(size_t l, const double *x, double *y, double a, double b)
+    template <class T> const T *fine_vals() const;
+    template <class T> const T *coarse_inverse() const;
+
+    template <class T>
+    int level_spmv(size_t l, const T *x, T *y, double a, double b, const T *z)
     {
-        if (l == 0) return ssr_dev_spmv_bsr(dim, n_n, ssr_pattern_d_row_ptr(Pat), ssr_pattern_d_col(Pat), Kdev, x, y, a, b);
-        return L[l]->dA.spmv(x, y, a, b);
+        if (l == 0)
+            return ssr::spmv_bsr<T>(dim, n_n, ssr_pattern_d_row_ptr(Pat), ssr_pattern_d_col(Pat), fine_vals<T>(), x, y,
+                                    a, b, z);
+        return L[l]->dev<T>().A.spmv(x, y, a, b, z);
     }
 
     // Chebyshev smoothing of A x = b on level l (x updated), D^-1 scaled.
-    int smooth(size_t l, const double *b, double *x, bool x_zero)
+    template <class T>
+    int smooth(size_t l, const T *b, T *x, bool x_zero)
     {
         Level &lv = *L[l];
+        DevLevel<T> &dl = lv.dev<T>();
         const int64_t m = lv.A.nr;
         // interval [lower, upper] * rho_up with rho_up a guaranteed upper bound, so the
         // polynomial never amplifies the highest modes
@@ -663,44 +771,39 @@ struct Amg : ssr_precond {
         const double sigma = theta / delta;
         double rho_k = 1.0 / sigma;
         int rc;
-        // r = b - A x
-        if (x_zero) SSR_HIP_CHECK(hipMemcpyAsync(lv.d_r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, ssr_stream()));
-        else {
-            SSR_HIP_CHECK(hipMemcpyAsync(lv.d_r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, ssr_stream()));
-            if ((rc = level_spmv(l, x, lv.d_r, -1.0, 1.0))) return rc;
+        // d = (1/theta) D^-1 r ; x += d   (r = b when x = 0)
+        if (x_zero) {
+            if ((rc = ssr::cheb_first<T>(m, 1.0 / theta, dl.dinv, b, dl.d, x))) return rc;
+        } else {
+            if ((rc = level_spmv<T>(l, x, dl.r, -1.0, 1.0, b))) return rc;
+            if ((rc = ssr::cheb_step<T>(m, 0.0, 1.0 / theta, dl.dinv, dl.r, dl.d, x))) return rc;
         }
-        // d = (1/theta) D^-1 r ; x += d
-        SSR_HIP_CHECK(hipMemsetAsync(lv.d_d, 0, sizeof(double) * m, ssr_stream()));
-        if ((rc = ssr_dev_cheb_step(m, 0.0, 1.0 / theta, lv.d_dinv, lv.d_r, lv.d_d, x))) return rc;
         for (int it = 1; it < opt.cheb_degree; ++it) {
             const double rho_new = 1.0 / (2.0 * sigma - rho_k);
-            // r = b - A x
-            SSR_HIP_CHECK(hipMemcpyAsync(lv.d_r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, ssr_stream()));
-            if ((rc = level_spmv(l, x, lv.d_r, -1.0, 1.0))) return rc;
-            if ((rc = ssr_dev_cheb_step(m, rho_new * rho_k, 2.0 * rho_new / delta, lv.d_dinv, lv.d_r, lv.d_d, x)))
+            if ((rc = level_spmv<T>(l, x, dl.r, -1.0, 1.0, b))) return rc;  // r = b - A x
+            if ((rc = ssr::cheb_step<T>(m, rho_new * rho_k, 2.0 * rho_new / delta, dl.dinv, dl.r, dl.d, x)))
                 return rc;
             rho_k = rho_new;
         }
         return SSR_OK;
     }
 
-    int vcycle(size_t l, const double *b, double *x)
+    template <class T>
+    int vcycle(size_t l, const T *b, T *x)
     {
         int rc;
         Level &lv = *L[l];
-        const int64_t m = lv.A.nr;
-        if (l + 1 == L.size()) return ssr_dev_dense_gemv((int)n_coarse, d_coarse_inv, b, x);
-        SSR_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * m, ssr_stream()));
-        if ((rc = smooth(l, b, x, true))) return rc;
+        DevLevel<T> &dl = lv.dev<T>();
+        if (l + 1 == L.size()) return ssr::dense_gemv<T>((int)n_coarse, coarse_inverse<T>(), b, x);
+        if ((rc = smooth<T>(l, b, x, true))) return rc;
         // r = b - A x ; bc = R r
-        SSR_HIP_CHECK(hipMemcpyAsync(lv.d_r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, ssr_stream()));
-        if ((rc = level_spmv(l, x, lv.d_r, -1.0, 1.0))) return rc;
-        Level &c = *L[l + 1];
-        if ((rc = lv.dR.spmv(lv.d_r, c.d_b, 1.0, 0.0))) return rc;
-        if ((rc = vcycle(l + 1, c.d_b, c.d_x))) return rc;
+        if ((rc = level_spmv<T>(l, x, dl.r, -1.0, 1.0, b))) return rc;
+        DevLevel<T> &dc = L[l + 1]->dev<T>();
+        if ((rc = dl.R.spmv(dl.r, dc.b, 1.0, 0.0))) return rc;
+        if ((rc = vcycle<T>(l + 1, dc.b, dc.x))) return rc;
         // x += P xc
-        if ((rc = lv.dP.spmv(c.d_x, x, 1.0, 1.0))) return rc;
-        return smooth(l, b, x, false);
+        if ((rc = dl.P.spmv(dc.x, x, 1.0, 1.0))) return rc;
+        return smooth<T>(l, b, x, false);
     }
 
     int apply(const double *x, double *y) override
@@ -709,9 +812,23 @@ struct Amg : ssr_precond {
             ssr_set_error("AMG: apply before setup");
             return SSR_ERR_STATE;
         }
-        return vcycle(0, x, y);
+        return vcycle<double>(0, x, y);
+    }
+
+    int apply(const float *x, float *y) override
+    {
+        if (!built || !want32 || !Kdev32) {
+            ssr_set_error("AMG: float apply before a mixed-precision setup");
+            return SSR_ERR_STATE;
+        }
+        return vcycle<float>(0, x, y);
     }
 };
+
+template <> const double *Amg::fine_vals<double>() const { return Kdev; }
+template <> const float *Amg::fine_vals<float>() const { return Kdev32; }
+template <> const double *Amg::coarse_inverse<double>() const { return d_coarse_inv; }
+template <> const float *Amg::coarse_inverse<float>() const { return d_coarse_inv32; }
 
 extern "C" {
 

@@ -509,8 +509,8 @@ __global__ __launch_bounds__(64) void k_mc3d_x36(int64_t n, const double *__rest
 // the four material values) is issued up front, so a single memory latency
 // precedes the arithmetic and no LDS round trip or barrier is spent on E.
 // DS is staged in phases of H points.
-template <bool WITH_DS, bool NT, int H>
-__global__ __launch_bounds__(64) void k_mc3d_x36d(int64_t n, const double *__restrict__ E,
+template <bool WITH_DS, bool NT, int H, int MINW = 1, bool SDIRECT = false>
+__global__ __launch_bounds__(64, MINW) void k_mc3d_x36d(int64_t n, const double *__restrict__ E,
                                                   const double *__restrict__ cbar,
                                                   const double *__restrict__ sphi,
                                                   const double *__restrict__ shear,
@@ -538,10 +538,18 @@ __global__ __launch_bounds__(64) void k_mc3d_x36d(int64_t n, const double *__res
     }
     double s[6], d[21];
     const int br = mc3d_point<WITH_DS>(eps, cb, sp, G, L, s, d);
+    if (SDIRECT) {
+        // each lane's 48-byte stress row as three 16-byte stores
+        if (lane < cnt) {
+            double *Sg = S + 6 * p;
+            store2<NT>(Sg, s[0], s[1]);
+            store2<NT>(Sg + 2, s[2], s[3]);
+            store2<NT>(Sg + 4, s[4], s[5]);
+        }
+    } else {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
-    __syncthreads();
-    {
+        for (int i = 0; i < 6; ++i) sbuf[lane * 6 + i] = s[i];
+        __syncthreads();
         double *Sg = S + 6 * p0;
         for (int t = lane; t < cnt * 3; t += W) store2<NT>(Sg + 2 * t, sbuf[2 * t], sbuf[2 * t + 1]);
     }
@@ -929,9 +937,15 @@ extern "C" int ssr_dev_constitutive_3D(int64_t n, const double *E, const double 
         } else if (v == 10) {
             if (DS) MC3((k_mc3d_x36d<true, true, 32>));
             else MC3((k_mc3d_x36d<false, true, 32>));
-        } else {
+        } else if (v == 11) {
             if (DS) MC3((k_mc3d_x36d<true, true, 16>));
             else MC3((k_mc3d_x36d<false, true, 16>));
+        } else if (v == 12) {
+            if (DS) MC3((k_mc3d_x36d<true, true, 16, 5>));
+            else MC3((k_mc3d_x36d<false, true, 16, 5>));
+        } else {
+            if (DS) MC3((k_mc3d_x36d<true, true, 16, 1, true>));
+            else MC3((k_mc3d_x36d<false, true, 16, 1, true>));
         }
 #undef MCP
 #undef MC3

@@ -1,9 +1,15 @@
-// Sparse and dense linear algebra for the Krylov solvers on gfx950:
-//   node-block CSR SpMV (dim x dim blocks), masked BLAS-1, tall-skinny
-//   multi-dot V'u and update u += V h, node-block Jacobi preconditioner.
+// Sparse and dense linear algebra of the Krylov solvers and the AMG V-cycle
+// on gfx950, for double and float operands:
+//   node-block CSR SpMV (dim x dim blocks), scalar CSR SpMV, masked BLAS-1,
+//   tall-skinny multi-dot V'u and update u += V h, node-block Jacobi,
+//   Chebyshev smoothing steps, dense coarse-level product, precision
+//   conversion.
 //
 // Reductions are two-stage with a fixed grid (deterministic: no atomics), a
-// 64-lane wavefront reduction by cross-lane shuffles then LDS across waves.
+// 64-lane wavefront reduction by cross-lane shuffles then LDS across waves,
+// always accumulated in double.  Results of reductions stay on the device so
+// that a caller can chain them without host round trips.
+#include "linalg.h"
 #include "ssr_common.h"
 
 namespace {
@@ -33,31 +39,30 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 }
 
 // ---------------------------------------------------------------------------
-// SpMV y = K x on node-block CSR.  A group of G lanes handles one block row:
-// lanes stride over the row's blocks, each forming dim partial outputs, then
-// the group reduces with shuffles.  Block values are row-major dim x dim.
+// SpMV y = alpha K x + beta z on node-block CSR.  A group of G lanes handles
+// one block row: lanes stride over the row's blocks, each forming dim partial
+// outputs, then the group reduces with shuffles.  Block values row-major.
 // ---------------------------------------------------------------------------
-template <int DIM, int G>
+template <int DIM, int G, class T>
 __global__ __launch_bounds__(256) void k_spmv_bsr(int64_t n_rows, const int32_t *__restrict__ row_ptr,
-                                                  const int32_t *__restrict__ col,
-                                                  const double *__restrict__ val,
-                                                  const double *__restrict__ x, double *__restrict__ y,
-                                                  double alpha, double beta)
+                                                  const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                  const T *__restrict__ x, T *y, double alpha, double beta,
+                                                  const T *z)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t r = gid / G;
     const int l = (int)(gid % G);
-    double acc[DIM];
+    T acc[DIM];
 #pragma unroll
-    for (int i = 0; i < DIM; ++i) acc[i] = 0.0;
+    for (int i = 0; i < DIM; ++i) acc[i] = T(0);
     if (r < n_rows) {
         const int32_t b0 = row_ptr[r], b1 = row_ptr[r + 1];
         for (int32_t b = b0 + l; b < b1; b += G) {
             const int64_t c = col[b];
-            double xv[DIM];
+            T xv[DIM];
 #pragma unroll
             for (int j = 0; j < DIM; ++j) xv[j] = x[c * DIM + j];
-            const double *v = val + (int64_t)b * DIM * DIM;
+            const T *v = val + (int64_t)b * DIM * DIM;
 #pragma unroll
             for (int i = 0; i < DIM; ++i)
 #pragma unroll
@@ -71,22 +76,41 @@ __global__ __launch_bounds__(256) void k_spmv_bsr(int64_t n_rows, const int32_t 
     if (r < n_rows && l == 0) {
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
-            const double o = alpha * acc[i];
-            y[r * DIM + i] = beta == 0.0 ? o : o + beta * y[r * DIM + i];
+            const T o = T(alpha) * acc[i];
+            y[r * DIM + i] = beta == 0.0 ? o : o + T(beta) * z[r * DIM + i];
         }
     }
 }
 
+// Scalar CSR, G lanes per row (rows of ~10-300 entries).
+template <int G, class T>
+__global__ __launch_bounds__(256) void k_spmv_csr(int64_t n_rows, const int64_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ ci, const T *__restrict__ v,
+                                                  const T *__restrict__ x, T *y, double alpha, double beta,
+                                                  const T *z)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = gid / G;
+    const int l = (int)(gid % G);
+    T s = T(0);
+    if (r < n_rows)
+        for (int64_t k = rp[r] + l; k < rp[r + 1]; k += G) s += v[k] * x[ci[k]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+    if (r < n_rows && l == 0) y[r] = T(alpha) * s + (beta == 0.0 ? T(0) : T(beta) * z[r]);
+}
+
 // partial[b] = sum over the block's share of a[i]*b[i] (times mask if given)
-__global__ __launch_bounds__(kRedThreads) void k_dot_partial(int64_t n, const double *__restrict__ a,
-                                                             const double *__restrict__ b,
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial(int64_t n, const T *__restrict__ a,
+                                                             const T *__restrict__ b,
                                                              const double *__restrict__ m,
                                                              double *__restrict__ part)
 {
     __shared__ double sh[kRedThreads / 64];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double t = a[i] * b[i];
+        const double t = (double)a[i] * (double)b[i];
         s += m ? m[i] * t : t;
     }
     s = block_sum(s, sh);
@@ -107,8 +131,9 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_partials(int nb, int k, 
 }
 
 // Multi-dot h = V(:,0:k)' u, columns stride ld.  Columns in chunks of 8.
-__global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int k, const double *__restrict__ V,
-                                                                  int64_t ld, const double *__restrict__ u,
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int k, const T *__restrict__ V,
+                                                                  int64_t ld, const T *__restrict__ u,
                                                                   double *__restrict__ part)
 {
     __shared__ double sh[kRedThreads / 64];
@@ -116,10 +141,10 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int
         double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const int nc = k - c0 < 8 ? k - c0 : 8;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-            const double ui = u[i];
+            const double ui = (double)u[i];
 #pragma unroll
             for (int c = 0; c < 8; ++c)
-                if (c < nc) s[c] += V[(int64_t)(c0 + c) * ld + i] * ui;
+                if (c < nc) s[c] += (double)V[(int64_t)(c0 + c) * ld + i] * ui;
         }
         for (int c = 0; c < nc; ++c) {
             const double t = block_sum(s[c], sh);
@@ -128,58 +153,84 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int
     }
 }
 
-// u = alpha*u + beta * V(:,0:k) h   (h on device)
-__global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const double *__restrict__ V, int64_t ld,
+// u = alpha*u + beta * V(:,0:k) h   (h on device, k <= 128)
+template <class TV, class TU>
+__global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV *__restrict__ V, int64_t ld,
                                                      const double *__restrict__ h, double alpha, double beta,
-                                                     double *__restrict__ u)
+                                                     TU *__restrict__ u)
 {
     __shared__ double sh[128];
-    for (int c = threadIdx.x; c < k && c < 128; c += blockDim.x) sh[c] = h[c];
+    for (int c = threadIdx.x; c < k; c += blockDim.x) sh[c] = h[c];
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int c = 0; c < k; ++c) s += V[(int64_t)c * ld + i] * (c < 128 ? sh[c] : h[c]);
-    u[i] = alpha * u[i] + beta * s;
+    for (int c = 0; c < k; ++c) s += (double)V[(int64_t)c * ld + i] * sh[c];
+    u[i] = (TU)((alpha == 0.0 ? 0.0 : alpha * (double)u[i]) + beta * s);
 }
 
-// y = a*x + b*y
-__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y)
+template <class T>
+__global__ void k_axpby(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+    if (i < n) y[i] = T(a) * x[i] + (b == 0.0 ? T(0) : T(b) * y[i]);
 }
 
-// z = a*x + b*y (z may alias neither or either)
-__global__ void k_lincomb(int64_t n, double a, const double *__restrict__ x, double b, const double *__restrict__ y,
-                          double *__restrict__ z)
+template <class T>
+__global__ void k_lincomb(int64_t n, double a, const T *__restrict__ x, double b, const T *__restrict__ y,
+                          T *__restrict__ z)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) z[i] = a * x[i] + b * y[i];
+    if (i < n) z[i] = T(a) * x[i] + T(b) * y[i];
+}
+
+template <class T>
+__global__ void k_mul(int64_t n, const double *__restrict__ m, T *__restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] *= T(m[i]);
 }
 
 // x = v (write-only; also the graph-safe way to clear a buffer)
-__global__ void k_fill(int64_t n, double v, double *__restrict__ x)
+template <class T>
+__global__ void k_fill(int64_t n, double v, T *__restrict__ x)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
+    if (i < n) x[i] = T(v);
 }
 
-__global__ void k_mul(int64_t n, const double *__restrict__ m, double *__restrict__ x)
+template <class TI, class TO>
+__global__ void k_convert(int64_t n, double a, const TI *__restrict__ x, TO *__restrict__ y)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] *= m[i];
+    if (i < n) y[i] = (TO)(a * (double)x[i]);
+}
+
+template <class TI, class TO>
+__global__ void k_axpy(int64_t n, double a, const TI *__restrict__ x, TO *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (TO)((double)y[i] + a * (double)x[i]);
+}
+
+template <class T>
+__global__ void k_scale_inv_sqrt(int64_t n, const double *__restrict__ nrm2, double tiny, T *__restrict__ u)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double nr = sqrt(nrm2[0] > 0.0 ? nrm2[0] : 0.0);
+    u[i] = nr > tiny ? (T)((double)u[i] / nr) : T(0);
 }
 
 // Node-block Jacobi: invert each diagonal block (masked rows/cols give identity).
-template <int DIM>
+template <int DIM, class T>
 __global__ void k_block_inverse(int64_t n_n, const int32_t *__restrict__ diag, const double *__restrict__ val,
-                                double *__restrict__ inv)
+                                T *__restrict__ inv)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_n) return;
     const double *a = val + (int64_t)diag[r] * DIM * DIM;
-    double *o = inv + r * DIM * DIM;
+    T *o = inv + r * DIM * DIM;
     if (DIM == 2) {
         const double det = a[0] * a[3] - a[1] * a[2];
         o[0] = a[3] / det; o[1] = -a[1] / det; o[2] = -a[2] / det; o[3] = a[0] / det;
@@ -193,57 +244,110 @@ __global__ void k_block_inverse(int64_t n_n, const int32_t *__restrict__ diag, c
     }
 }
 
-template <int DIM>
-__global__ void k_block_apply(int64_t n_n, const double *__restrict__ inv, const double *__restrict__ x,
-                              const double *__restrict__ mask, double *__restrict__ y)
+template <int DIM, class T>
+__global__ void k_block_apply(int64_t n_n, const T *__restrict__ inv, const T *__restrict__ x,
+                              const double *__restrict__ mask, T *__restrict__ y)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_n) return;
-    const double *o = inv + r * DIM * DIM;
-    double xv[DIM];
+    const T *o = inv + r * DIM * DIM;
+    T xv[DIM];
 #pragma unroll
     for (int j = 0; j < DIM; ++j) xv[j] = x[r * DIM + j];
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
-        double s = 0.0;
+        T s = T(0);
 #pragma unroll
         for (int j = 0; j < DIM; ++j) s += o[i * DIM + j] * xv[j];
-        y[r * DIM + i] = mask ? mask[r * DIM + i] * s : s;
+        y[r * DIM + i] = mask ? T(mask[r * DIM + i]) * s : s;
     }
 }
 
-}  // namespace
+// Chebyshev: d = c3*dinv.*r ; x = d
+template <class T>
+__global__ void k_cheb_first(int64_t n, double c3, const T *__restrict__ dinv, const T *__restrict__ r,
+                             T *__restrict__ d, T *__restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T dn = T(c3) * dinv[i] * r[i];
+    d[i] = dn;
+    x[i] = dn;
+}
 
-// ---------------------------------------------------------------------------
-// C ABI (device pointers)
-// ---------------------------------------------------------------------------
-static double *g_part = nullptr;   // kRedBlocks * 128 partial sums
-static double *g_red = nullptr;    // 128 reduced values
+// Chebyshev: d = c2*d + c3*dinv.*r ; x += d
+template <class T>
+__global__ void k_cheb_step(int64_t n, double c2, double c3, const T *__restrict__ dinv, const T *__restrict__ r,
+                            T *__restrict__ d, T *__restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T dn = T(c2) * d[i] + T(c3) * dinv[i] * r[i];
+    d[i] = dn;
+    x[i] += dn;
+}
 
-static int ensure_scratch()
+// y = M x for a dense n x n row-major matrix (one row per wavefront)
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_gemv(int n, const T *__restrict__ M, const T *__restrict__ x,
+                                                    T *__restrict__ y)
+{
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
+    double s = 0.0;
+    if (w < n)
+        for (int j = l; j < n; j += 64) s += (double)M[(int64_t)w * n + j] * (double)x[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (w < n && l == 0) y[w] = (T)s;
+}
+
+// element-wise z = a .* b
+__global__ void k_ewmul(int64_t n, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ z)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = a[i] * b[i];
+}
+
+double *g_part = nullptr;  // kRedBlocks * 128 partial sums
+double *g_red = nullptr;   // 128 reduced values
+
+int ensure_scratch()
 {
     if (!g_part) SSR_HIP_CHECK(hipMalloc((void **)&g_part, sizeof(double) * kRedBlocks * 128));
     if (!g_red) SSR_HIP_CHECK(hipMalloc((void **)&g_red, sizeof(double) * 128));
     return SSR_OK;
 }
 
-static unsigned red_blocks(int64_t n)
+unsigned red_blocks(int64_t n)
 {
     int64_t b = (n + kRedThreads - 1) / kRedThreads;
     return (unsigned)(b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b));
 }
 
-extern "C" {
+#define SSR_LAUNCH1(K_, n_, ...)                                                                      \
+    do {                                                                                              \
+        if ((n_) == 0) return SSR_OK;                                                                 \
+        hipLaunchKernelGGL(K_, dim3(ssr_blocks((n_), 256)), dim3(256), 0, ssr_stream(), __VA_ARGS__); \
+        SSR_HIP_CHECK(hipGetLastError());                                                             \
+        return SSR_OK;                                                                                \
+    } while (0)
 
-int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const double *val,
-                     const double *x, double *y, double alpha, double beta)
+}  // namespace
+
+namespace ssr {
+
+template <class T>
+int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const T *val, const T *x, T *y,
+             double alpha, double beta, const T *z)
 {
     if (n_rows == 0) return SSR_OK;
+    if (!z) z = y;
     const int G = ssr_tuning(SSR_TUNE_SPMV_GROUP);
     const int64_t threads = n_rows * G;
     const dim3 grid(ssr_blocks(threads, 256)), blk(256);
-#define SPMV(D_, G_) hipLaunchKernelGGL((k_spmv_bsr<D_, G_>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, \
-                                        x, y, alpha, beta)
+#define SPMV(D_, G_) hipLaunchKernelGGL((k_spmv_bsr<D_, G_, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, \
+                                        val, x, y, alpha, beta, z)
     if (dim == 3) {
         if (G == 4) SPMV(3, 4);
         else if (G == 16) SPMV(3, 16);
@@ -258,16 +362,178 @@ int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int3
     return SSR_OK;
 }
 
-// result[0] = sum_i a_i b_i (m_i) written to DEVICE memory
-int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result)
+template <class T>
+int spmv_csr(int64_t n_rows, const int64_t *rp, const int32_t *ci, const T *v, const T *x, T *y, double alpha,
+             double beta, const T *z)
+{
+    if (n_rows == 0) return SSR_OK;
+    if (!z) z = y;
+    constexpr int G = 8;
+    hipLaunchKernelGGL((k_spmv_csr<G, T>), dim3(ssr_blocks(n_rows * G, 256)), dim3(256), 0, ssr_stream(), n_rows, rp,
+                       ci, v, x, y, alpha, beta, z);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+template <class T>
+int dot(int64_t n, const T *a, const T *b, const double *mask, double *result)
 {
     int rc = ensure_scratch();
     if (rc) return rc;
     const unsigned nb = red_blocks(n);
-    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, a, b, mask, g_part);
+    hipLaunchKernelGGL((k_dot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, a, b, mask, g_part);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, ssr_stream(), (int)nb, 1, g_part, result);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
+}
+
+template <class T>
+int multidot(int64_t n, int k, const T *V, int64_t ld, const T *u, double *h)
+{
+    SSR_REQUIRE(k >= 0 && k <= 128, "multidot: k must be in [0,128]");
+    if (k == 0) return SSR_OK;
+    int rc = ensure_scratch();
+    if (rc) return rc;
+    const unsigned nb = red_blocks(n);
+    hipLaunchKernelGGL((k_multidot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u, g_part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, g_part, h);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+template <class TV, class TU>
+int gemv_update(int64_t n, int k, const TV *V, int64_t ld, const double *h, double alpha, double beta, TU *u)
+{
+    SSR_REQUIRE(k >= 0 && k <= 128, "gemv_update: k must be in [0,128]");
+    SSR_LAUNCH1((k_gemv_update<TV, TU>), n, n, k, V, ld, h, alpha, beta, u);
+}
+
+template <class T>
+int axpby(int64_t n, double a, const T *x, double b, T *y)
+{
+    SSR_LAUNCH1(k_axpby<T>, n, n, a, x, b, y);
+}
+
+template <class T>
+int lincomb(int64_t n, double a, const T *x, double b, const T *y, T *z)
+{
+    SSR_LAUNCH1(k_lincomb<T>, n, n, a, x, b, y, z);
+}
+
+template <class T>
+int mul(int64_t n, const double *m, T *x)
+{
+    SSR_LAUNCH1(k_mul<T>, n, n, m, x);
+}
+
+template <class T>
+int fill(int64_t n, double v, T *x)
+{
+    SSR_LAUNCH1(k_fill<T>, n, n, v, x);
+}
+
+template <class TI, class TO>
+int convert(int64_t n, double a, const TI *x, TO *y)
+{
+    SSR_LAUNCH1((k_convert<TI, TO>), n, n, a, x, y);
+}
+
+template <class TI, class TO>
+int axpy(int64_t n, double a, const TI *x, TO *y)
+{
+    SSR_LAUNCH1((k_axpy<TI, TO>), n, n, a, x, y);
+}
+
+template <class T>
+int scale_inv_sqrt(int64_t n, const double *nrm2, double tiny, T *u)
+{
+    SSR_LAUNCH1(k_scale_inv_sqrt<T>, n, n, nrm2, tiny, u);
+}
+
+template <class T>
+int block_jacobi_setup(int dim, int64_t n_n, const int32_t *diag, const double *val, T *inv)
+{
+    if (dim == 3) SSR_LAUNCH1((k_block_inverse<3, T>), n_n, n_n, diag, val, inv);
+    SSR_LAUNCH1((k_block_inverse<2, T>), n_n, n_n, diag, val, inv);
+}
+
+template <class T>
+int block_jacobi_apply(int dim, int64_t n_n, const T *inv, const T *x, const double *mask, T *y)
+{
+    if (dim == 3) SSR_LAUNCH1((k_block_apply<3, T>), n_n, n_n, inv, x, mask, y);
+    SSR_LAUNCH1((k_block_apply<2, T>), n_n, n_n, inv, x, mask, y);
+}
+
+template <class T>
+int cheb_first(int64_t n, double c3, const T *dinv, const T *r, T *d, T *x)
+{
+    SSR_LAUNCH1(k_cheb_first<T>, n, n, c3, dinv, r, d, x);
+}
+
+template <class T>
+int cheb_step(int64_t n, double c2, double c3, const T *dinv, const T *r, T *d, T *x)
+{
+    SSR_LAUNCH1(k_cheb_step<T>, n, n, c2, c3, dinv, r, d, x);
+}
+
+template <class T>
+int dense_gemv(int n, const T *M, const T *x, T *y)
+{
+    if (n == 0) return SSR_OK;
+    hipLaunchKernelGGL(k_dense_gemv<T>, dim3(ssr_blocks((int64_t)n * 64, 256)), dim3(256), 0, ssr_stream(), n, M, x,
+                       y);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+#define SSR_INST(T)                                                                                                  \
+    template int spmv_bsr<T>(int, int64_t, const int32_t *, const int32_t *, const T *, const T *, T *, double,       \
+                             double, const T *);                                                                     \
+    template int spmv_csr<T>(int64_t, const int64_t *, const int32_t *, const T *, const T *, T *, double, double,   \
+                             const T *);                                                                             \
+    template int dot<T>(int64_t, const T *, const T *, const double *, double *);                                    \
+    template int multidot<T>(int64_t, int, const T *, int64_t, const T *, double *);                                 \
+    template int axpby<T>(int64_t, double, const T *, double, T *);                                                   \
+    template int lincomb<T>(int64_t, double, const T *, double, const T *, T *);                                      \
+    template int mul<T>(int64_t, const double *, T *);                                                               \
+    template int fill<T>(int64_t, double, T *);                                                                      \
+    template int scale_inv_sqrt<T>(int64_t, const double *, double, T *);                                            \
+    template int block_jacobi_setup<T>(int, int64_t, const int32_t *, const double *, T *);                          \
+    template int block_jacobi_apply<T>(int, int64_t, const T *, const T *, const double *, T *);                     \
+    template int cheb_first<T>(int64_t, double, const T *, const T *, T *, T *);                                      \
+    template int cheb_step<T>(int64_t, double, double, const T *, const T *, T *, T *);                               \
+    template int dense_gemv<T>(int, const T *, const T *, T *);
+SSR_INST(double)
+SSR_INST(float)
+#undef SSR_INST
+template int gemv_update<double, double>(int64_t, int, const double *, int64_t, const double *, double, double,
+                                         double *);
+template int gemv_update<float, float>(int64_t, int, const float *, int64_t, const double *, double, double, float *);
+template int gemv_update<float, double>(int64_t, int, const float *, int64_t, const double *, double, double,
+                                        double *);
+template int convert<double, float>(int64_t, double, const double *, float *);
+template int convert<float, double>(int64_t, double, const float *, double *);
+template int convert<double, double>(int64_t, double, const double *, double *);
+template int axpy<float, double>(int64_t, double, const float *, double *);
+template int axpy<double, double>(int64_t, double, const double *, double *);
+
+}  // namespace ssr
+
+// ---------------------------------------------------------------------------
+// C ABI (double, device pointers)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const double *val,
+                     const double *x, double *y, double alpha, double beta)
+{
+    return ssr::spmv_bsr<double>(dim, n_rows, row_ptr, col, val, x, y, alpha, beta);
+}
+
+// result[0] = sum_i a_i b_i (m_i) written to DEVICE memory
+int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result)
+{
+    return ssr::dot<double>(n, a, b, mask, result);
 }
 
 // host-returning dot (synchronises)
@@ -275,7 +541,7 @@ int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, dou
 {
     int rc = ensure_scratch();
     if (rc) return rc;
-    if ((rc = ssr_dev_dot(n, a, b, mask, g_red))) return rc;
+    if ((rc = ssr::dot<double>(n, a, b, mask, g_red))) return rc;
     SSR_HIP_CHECK(hipMemcpyAsync(host_result, g_red, sizeof(double), hipMemcpyDeviceToHost, ssr_stream()));
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     return SSR_OK;
@@ -284,80 +550,53 @@ int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, dou
 // h (device, k values) = V(:,0:k)' u
 int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h)
 {
-    SSR_REQUIRE(k >= 0 && k <= 128, "multidot: k must be in [0,128]");
-    if (k == 0) return SSR_OK;
-    int rc = ensure_scratch();
-    if (rc) return rc;
-    const unsigned nb = red_blocks(n);
-    hipLaunchKernelGGL(k_multidot_partial, dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u, g_part);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, g_part, h);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
+    return ssr::multidot<double>(n, k, V, ld, u, h);
 }
 
 int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const double *h, double alpha, double beta,
                         double *u)
 {
-    if (n == 0) return SSR_OK;
-    hipLaunchKernelGGL(k_gemv_update, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, k, V, ld, h, alpha,
-                       beta, u);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
+    return ssr::gemv_update<double, double>(n, k, V, ld, h, alpha, beta, u);
 }
 
-int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y)
-{
-    if (n == 0) return SSR_OK;
-    hipLaunchKernelGGL(k_axpby, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, a, x, b, y);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
-}
+int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y) { return ssr::axpby<double>(n, a, x, b, y); }
 
 int ssr_dev_lincomb(int64_t n, double a, const double *x, double b, const double *y, double *z)
 {
-    if (n == 0) return SSR_OK;
-    hipLaunchKernelGGL(k_lincomb, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, a, x, b, y, z);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
+    return ssr::lincomb<double>(n, a, x, b, y, z);
 }
 
-int ssr_dev_fill(int64_t n, double v, double *x)
-{
-    if (n == 0) return SSR_OK;
-    hipLaunchKernelGGL(k_fill, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, v, x);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
-}
+int ssr_dev_fill(int64_t n, double v, double *x) { return ssr::fill<double>(n, v, x); }
 
-int ssr_dev_mul(int64_t n, const double *m, double *x)
-{
-    if (n == 0) return SSR_OK;
-    hipLaunchKernelGGL(k_mul, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, m, x);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
-}
+int ssr_dev_mul(int64_t n, const double *m, double *x) { return ssr::mul<double>(n, m, x); }
 
 int ssr_dev_block_jacobi_setup(int dim, int64_t n_n, const int32_t *diag, const double *val, double *inv)
 {
-    if (n_n == 0) return SSR_OK;
-    if (dim == 3)
-        hipLaunchKernelGGL(k_block_inverse<3>, dim3(ssr_blocks(n_n, 256)), dim3(256), 0, ssr_stream(), n_n, diag, val, inv);
-    else
-        hipLaunchKernelGGL(k_block_inverse<2>, dim3(ssr_blocks(n_n, 256)), dim3(256), 0, ssr_stream(), n_n, diag, val, inv);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
+    return ssr::block_jacobi_setup<double>(dim, n_n, diag, val, inv);
 }
 
 int ssr_dev_block_jacobi_apply(int dim, int64_t n_n, const double *inv, const double *x, const double *mask,
                                double *y)
 {
-    if (n_n == 0) return SSR_OK;
-    if (dim == 3)
-        hipLaunchKernelGGL(k_block_apply<3>, dim3(ssr_blocks(n_n, 256)), dim3(256), 0, ssr_stream(), n_n, inv, x, mask, y);
-    else
-        hipLaunchKernelGGL(k_block_apply<2>, dim3(ssr_blocks(n_n, 256)), dim3(256), 0, ssr_stream(), n_n, inv, x, mask, y);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
+    return ssr::block_jacobi_apply<double>(dim, n_n, inv, x, mask, y);
 }
+
+int ssr_dev_csr_spmv(int64_t n_rows, const int64_t *rp, const int32_t *ci, const double *v, const double *x,
+                     double *y, double alpha, double beta)
+{
+    return ssr::spmv_csr<double>(n_rows, rp, ci, v, x, y, alpha, beta);
+}
+
+int ssr_dev_ewmul(int64_t n, const double *a, const double *b, double *z)
+{
+    SSR_LAUNCH1(k_ewmul, n, n, a, b, z);
+}
+
+int ssr_dev_cheb_step(int64_t n, double c2, double c3, const double *dinv, const double *r, double *d, double *x)
+{
+    return ssr::cheb_step<double>(n, c2, c3, dinv, r, d, x);
+}
+
+int ssr_dev_dense_gemv(int n, const double *M, const double *x, double *y) { return ssr::dense_gemv<double>(n, M, x, y); }
 
 }  // extern "C"

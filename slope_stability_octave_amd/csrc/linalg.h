@@ -1,0 +1,63 @@
+// Typed device linear algebra shared by the Krylov solvers and the AMG
+// V-cycle (instantiated for double and float; every reduction accumulates in
+// double).  All launches go to ssr_stream(); nothing here synchronises.
+#pragma once
+#include <stdint.h>
+
+namespace ssr {
+
+// y = alpha*K*x + beta*z on the node-block CSR (dim x dim row-major blocks);
+// z == nullptr means z = y.
+template <class T>
+int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const T *val, const T *x,
+             T *y, double alpha, double beta, const T *z = nullptr);
+// y = alpha*A*x + beta*z on scalar CSR (64-bit row pointers)
+template <class T>
+int spmv_csr(int64_t n_rows, const int64_t *rp, const int32_t *ci, const T *v, const T *x, T *y, double alpha,
+             double beta, const T *z = nullptr);
+// result_dev[0] = sum a_i b_i (m_i)
+template <class T>
+int dot(int64_t n, const T *a, const T *b, const double *mask, double *result_dev);
+// h_dev[0:k] = V(:,0:k)' u, k <= 128
+template <class T>
+int multidot(int64_t n, int k, const T *V, int64_t ld, const T *u, double *h_dev);
+// u = alpha*u + beta*V(:,0:k) h_dev, k <= 128
+template <class TV, class TU>
+int gemv_update(int64_t n, int k, const TV *V, int64_t ld, const double *h_dev, double alpha, double beta, TU *u);
+// y = a*x + b*y
+template <class T>
+int axpby(int64_t n, double a, const T *x, double b, T *y);
+// z = a*x + b*y
+template <class T>
+int lincomb(int64_t n, double a, const T *x, double b, const T *y, T *z);
+// x *= m
+template <class T>
+int mul(int64_t n, const double *m, T *x);
+// x = v
+template <class T>
+int fill(int64_t n, double v, T *x);
+// y = a*x (precision conversion)
+template <class TI, class TO>
+int convert(int64_t n, double a, const TI *x, TO *y);
+// y += a*x (mixed precision accumulate)
+template <class TI, class TO>
+int axpy(int64_t n, double a, const TI *x, TO *y);
+// u *= 1/sqrt(nrm2_dev[0]) when sqrt(nrm2) > tiny, else u = 0
+template <class T>
+int scale_inv_sqrt(int64_t n, const double *nrm2_dev, double tiny, T *u);
+// node-block Jacobi
+template <class T>
+int block_jacobi_setup(int dim, int64_t n_n, const int32_t *diag, const double *val, T *inv);
+template <class T>
+int block_jacobi_apply(int dim, int64_t n_n, const T *inv, const T *x, const double *mask, T *y);
+// Chebyshev steps on D^-1 A: first (x := d = c3 dinv r) or general
+// (d = c2 d + c3 dinv r; x += d)
+template <class T>
+int cheb_first(int64_t n, double c3, const T *dinv, const T *r, T *d, T *x);
+template <class T>
+int cheb_step(int64_t n, double c2, double c3, const T *dinv, const T *r, T *d, T *x);
+// y = M x, dense row-major n x n
+template <class T>
+int dense_gemv(int n, const T *M, const T *x, T *y);
+
+}  // namespace ssr

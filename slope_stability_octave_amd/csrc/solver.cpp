@@ -14,8 +14,20 @@
 //     (one reduction per basis vector); the Hessenberg least-squares problem
 //     is updated with Givens rotations instead of a fresh QR solve, giving the
 //     same iterate and residual estimate.
+//   * The Arnoldi coefficients and the new basis vector's norm stay on the
+//     device (the vector is normalised there); the host receives each
+//     Hessenberg column asynchronously and processes it one iteration behind
+//     the device, so no iteration waits for a host round trip.  When column j
+//     meets the tolerance the already-enqueued iteration j+1 is discarded and
+//     the solution is formed from j+1 columns: the iterate is the one the
+//     sequential algorithm returns.
 //   * A happy breakdown (H(j+1,j) < 1e-14) returns the exact Krylov solution
 //     with j columns (dfgmres_solver.m:136-138 would reuse y of step j-1).
+//   * Precision 32: the Arnoldi process, the preconditioner and the deflation
+//     projector run in float on float copies of the operator and the basis;
+//     x0, the true residual r = b - A x and the accumulation x += Z y stay in
+//     double, and the float solve is repeated on the new double residual
+//     (iterative refinement) until the relative residual meets the tolerance.
 // Vectors span all dim*n_n DOFs; Dirichlet entries are kept at zero by the
 // mask after each preconditioner application, so every inner product equals
 // the reference's inner product over the free DOFs Q.
@@ -26,6 +38,7 @@
 
 #include <vector>
 
+#include "linalg.h"
 #include "ssr_common.h"
 
 struct ssr_pattern;
@@ -36,28 +49,47 @@ const double *ssr_pattern_d_mask(const ssr_pattern *p);
 void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e,
                        int64_t *nnzb, int64_t *nnz_free);
 
-extern "C" {
-int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const double *val,
-                     const double *x, double *y, double alpha, double beta);
-int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result);
-int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h);
-int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const double *h, double alpha,
-                        double beta, double *u);
-int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y);
-int ssr_dev_lincomb(int64_t n, double a, const double *x, double b, const double *y, double *z);
-int ssr_dev_mul(int64_t n, const double *m, double *x);
-int ssr_dev_block_jacobi_setup(int dim, int64_t n_n, const int32_t *diag, const double *val, double *inv);
-int ssr_dev_block_jacobi_apply(int dim, int64_t n_n, const double *inv, const double *x, const double *mask,
-                               double *y);
-}
+extern "C" int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result);
 
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    int64_t n = 0;
+    ~DevBuf() { (void)hipFree(p); }
+    int alloc(int64_t len)
+    {
+        if (p && n >= len) return SSR_OK;
+        (void)hipFree(p);
+        p = nullptr;
+        n = len;
+        SSR_HIP_CHECK(hipMalloc((void **)&p, sizeof(T) * (len > 0 ? len : 1)));
+        return SSR_OK;
+    }
+};
+
+template <class T>
+struct HostBuf {  // page-locked host memory (asynchronous device-to-host copies)
+    T *p = nullptr;
+    int64_t n = 0;
+    ~HostBuf() { (void)hipHostFree(p); }
+    int alloc(int64_t len)
+    {
+        if (p && n >= len) return SSR_OK;
+        (void)hipHostFree(p);
+        p = nullptr;
+        n = len;
+        SSR_HIP_CHECK(hipHostMalloc((void **)&p, sizeof(T) * (len > 0 ? len : 1), hipHostMallocDefault));
+        return SSR_OK;
+    }
+};
 
 struct BlockJacobi : ssr_precond {
     int dim;
     int64_t n_n;
     const int32_t *diag;
     const double *mask;
-    double *inv = nullptr;
+    DevBuf<double> inv;
+    DevBuf<float> inv32;
     BlockJacobi(const ssr_pattern *P)
     {
         int64_t n_p, n_e, nnzb, nnzf;
@@ -65,59 +97,78 @@ struct BlockJacobi : ssr_precond {
         diag = ssr_pattern_d_diag(P);
         mask = ssr_pattern_d_mask(P);
     }
-    ~BlockJacobi() { (void)hipFree(inv); }
-    int setup(const double *K) override
+    int setup(const double *K, const float *K32) override
     {
-        if (!inv) SSR_HIP_CHECK(hipMalloc((void **)&inv, sizeof(double) * n_n * dim * dim));
-        return ssr_dev_block_jacobi_setup(dim, n_n, diag, K, inv);
-    }
-    int apply(const double *x, double *y) override { return ssr_dev_block_jacobi_apply(dim, n_n, inv, x, mask, y); }
-};
-
-
-struct DevVec {
-    double *p = nullptr;
-    int64_t n = 0;
-    ~DevVec() { (void)hipFree(p); }
-    int alloc(int64_t len)
-    {
-        if (p && n >= len) return SSR_OK;
-        (void)hipFree(p);
-        p = nullptr;
-        n = len;
-        SSR_HIP_CHECK(hipMalloc((void **)&p, sizeof(double) * (len > 0 ? len : 1)));
+        int rc;
+        if ((rc = inv.alloc(n_n * dim * dim))) return rc;
+        if ((rc = ssr::block_jacobi_setup<double>(dim, n_n, diag, K, inv.p))) return rc;
+        if (K32) {
+            if ((rc = inv32.alloc(n_n * dim * dim))) return rc;
+            if ((rc = ssr::block_jacobi_setup<float>(dim, n_n, diag, K, inv32.p))) return rc;
+        }
         return SSR_OK;
+    }
+    int apply(const double *x, double *y) override { return ssr::block_jacobi_apply<double>(dim, n_n, inv.p, x, mask, y); }
+    int apply(const float *x, float *y) override
+    {
+        if (!inv32.p) {
+            ssr_set_error("block Jacobi: float apply before a mixed-precision setup");
+            return SSR_ERR_STATE;
+        }
+        return ssr::block_jacobi_apply<float>(dim, n_n, inv32.p, x, mask, y);
     }
 };
 
 struct ssr_solver {
     const ssr_pattern *P = nullptr;
     int dim = 0;
-    int64_t n_n = 0, n = 0;
+    int64_t n_n = 0, n = 0, nnzb = 0;
     const int32_t *row_ptr = nullptr, *col = nullptr;
     const double *mask = nullptr;
     const double *K = nullptr;  // operator values (borrowed)
     int maxit = 100;
     double tol = 1e-1, defl_tol = 1e-3;
+    int precision = 64;    // 64: all double; 32: float Krylov/preconditioner + double refinement
+    int max_refine = 8;    // float solves per call in precision 32
     ssr_precond *M = nullptr;
     bool own_M = true;
-    // deflation basis (columns of length n)
-    DevVec W;
+    // deflation basis (columns of length n) and its float mirror
+    DevBuf<double> W;
     int kW = 0, capW = 0;
+    DevBuf<float> W32;
+    int kW32 = -1;  // columns of W mirrored in W32 (-1: stale)
+    DevBuf<float> K32;
+    bool K32_valid = false;
     // Krylov storage
-    DevVec V, Z, t1, t2, hdev;
+    DevBuf<double> V, Z, t1, t2, r;
+    DevBuf<float> V32, Z32, t2f;
+    DevBuf<double> hcol, hW, ydev, scal;  // Arnoldi columns, deflation coefficients, LSQ solution, scalars
+    HostBuf<double> hhost, yhost;
+    std::vector<hipEvent_t> ev;
     // stats
-    int last_iters = 0;
+    int last_iters = 0, last_refine = 0;
     double last_relres = 0.0;
     int64_t total_iters = 0, n_solves = 0, n_prec = 0, n_spmv = 0;
+    ~ssr_solver()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
 
-int spmv(ssr_solver *s, const double *x, double *y)
+template <class T> const T *op_vals(const ssr_solver *s);
+template <> const double *op_vals<double>(const ssr_solver *s) { return s->K; }
+template <> const float *op_vals<float>(const ssr_solver *s) { return s->K32.p; }
+template <class T> const T *defl_basis(const ssr_solver *s);
+template <> const double *defl_basis<double>(const ssr_solver *s) { return s->W.p; }
+template <> const float *defl_basis<float>(const ssr_solver *s) { return s->W32.p; }
+
+template <class T>
+int spmv(ssr_solver *s, const T *x, T *y, double alpha = 1.0, double beta = 0.0, const T *z = nullptr)
 {
     s->n_spmv++;
-    return ssr_dev_spmv_bsr(s->dim, s->n_n, s->row_ptr, s->col, s->K, x, y, 1.0, 0.0);
+    return ssr::spmv_bsr<T>(s->dim, s->n_n, s->row_ptr, s->col, op_vals<T>(s), x, y, alpha, beta, z);
 }
 
 double hnorm(ssr_solver *s, const double *x, int *rc)
@@ -127,46 +178,173 @@ double hnorm(ssr_solver *s, const double *x, int *rc)
     return sqrt(v);
 }
 
-// h (device) = W' x, chunked by 128 columns; result on device at s->hdev
-int defl_coef(ssr_solver *s, const double *x)
+// u = alpha*u + beta*W (W' x) with the basis of precision T; coefficients in s->hW
+template <class T>
+int defl_apply(ssr_solver *s, const T *x, double alpha, double beta, T *u)
 {
     int rc;
+    const T *W = defl_basis<T>(s);
     for (int c0 = 0; c0 < s->kW; c0 += 128) {
         const int k = s->kW - c0 < 128 ? s->kW - c0 : 128;
-        if ((rc = ssr_dev_multidot(s->n, k, s->W.p + (int64_t)c0 * s->n, s->n, x, s->hdev.p + c0))) return rc;
+        if ((rc = ssr::multidot<T>(s->n, k, W + (int64_t)c0 * s->n, s->n, x, s->hW.p + c0))) return rc;
     }
-    return SSR_OK;
-}
-
-// u = alpha*u + beta*W h(dev)
-int defl_update(ssr_solver *s, double alpha, double beta, double *u)
-{
-    int rc;
     for (int c0 = 0; c0 < s->kW; c0 += 128) {
         const int k = s->kW - c0 < 128 ? s->kW - c0 : 128;
-        if ((rc = ssr_dev_gemv_update(s->n, k, s->W.p + (int64_t)c0 * s->n, s->n, s->hdev.p + c0,
-                                      c0 == 0 ? alpha : 1.0, beta, u)))
+        if ((rc = ssr::gemv_update<T, T>(s->n, k, W + (int64_t)c0 * s->n, s->n, s->hW.p + c0, c0 == 0 ? alpha : 1.0,
+                                         beta, u)))
             return rc;
     }
     return SSR_OK;
 }
 
-// x <- x - W W' A x   (t2 used as scratch)
-int project(ssr_solver *s, double *x)
+// x <- x - W W' A x   (scratch t)
+template <class T>
+int project(ssr_solver *s, T *x, T *t)
 {
     if (s->kW == 0) return SSR_OK;
     int rc;
-    if ((rc = spmv(s, x, s->t2.p))) return rc;
-    if ((rc = defl_coef(s, s->t2.p))) return rc;
-    return defl_update(s, 1.0, -1.0, x);
+    if ((rc = spmv<T>(s, x, t))) return rc;
+    return defl_apply<T>(s, t, 1.0, -1.0, x);
 }
 
-int precond(ssr_solver *s, const double *x, double *y)
+template <class T>
+int precond(ssr_solver *s, const T *x, T *y)
 {
     s->n_prec++;
     int rc = s->M->apply(x, y);
     if (rc) return rc;
-    return ssr_dev_mul(s->n, s->mask, y);
+    return ssr::mul<T>(s->n, s->mask, y);
+}
+
+int ensure_w32(ssr_solver *s)
+{
+    if (s->kW32 == s->kW) return SSR_OK;
+    int rc;
+    if ((rc = s->W32.alloc(s->n * (int64_t)(s->capW > 0 ? s->capW : 1)))) return rc;
+    if (s->kW > 0 && (rc = ssr::convert<double, float>(s->n * (int64_t)s->kW, 1.0, s->W.p, s->W32.p))) return rc;
+    s->kW32 = s->kW;
+    return SSR_OK;
+}
+
+inline int64_t col_off(int j) { return (int64_t)j * j + 2 * (int64_t)j; }  // slot j holds 2(j+1)+1 values
+
+// Flexible GMRES on A d = v0 * rnorm0 where V(:,0) = v0 (unit, precision T)
+// is already in place.  Runs at most m iterations and stops when the residual
+// estimate |g| drops below `target`; then x += Z y (x in double).
+template <class T>
+int fgmres_core(ssr_solver *s, T *V, T *Z, T *t, double rnorm0, double target, int m, double *x, int *iters,
+                double *res_out)
+{
+    const int64_t n = s->n;
+    int rc;
+    std::vector<double> H((size_t)(m + 1) * (m > 0 ? m : 1), 0.0), cs(m + 1), sn(m + 1), g(m + 2, 0.0);
+    g[0] = rnorm0;
+    double res = rnorm0;
+    int done_cols = -1;  // number of columns of the final solution once known
+    int enq = 0;         // iterations enqueued
+    int proc = 0;        // columns processed on the host
+    const int lag = 1;
+    auto process = [&](int j) -> int {
+        SSR_HIP_CHECK(hipEventSynchronize(s->ev[j]));
+        const double *h1 = s->hhost.p + col_off(j), *h2 = h1 + (j + 1), *nr2 = h2 + (j + 1);
+        double *Hc = H.data() + (size_t)j * (m + 1);
+        for (int i = 0; i <= j; ++i) Hc[i] = h1[i] + h2[i];
+        const double hn = sqrt(nr2[0] > 0.0 ? nr2[0] : 0.0);
+        Hc[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+            const double a = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
+            Hc[i + 1] = -sn[i] * Hc[i] + cs[i] * Hc[i + 1];
+            Hc[i] = a;
+        }
+        if (hn < 1e-14) {
+            // exact solution within this Krylov space: close the column with
+            // the previous rotations only
+            cs[j] = 1.0;
+            sn[j] = 0.0;
+            res = fabs(g[j + 1]);
+            done_cols = j + 1;
+            return SSR_OK;
+        }
+        const double den = hypot(Hc[j], Hc[j + 1]);
+        cs[j] = Hc[j] / den;
+        sn[j] = Hc[j + 1] / den;
+        Hc[j] = den;
+        Hc[j + 1] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        res = fabs(g[j + 1]);
+        if (res < target || j + 1 == m) done_cols = j + 1;
+        return SSR_OK;
+    };
+    while (done_cols < 0) {
+        if (enq < m) {
+            const int j = enq;
+            T *zj = Z + (int64_t)j * n;
+            T *u = V + (int64_t)(j + 1) * n;
+            double *slot = s->hcol.p + col_off(j);
+            if ((rc = precond<T>(s, V + (int64_t)j * n, zj))) return rc;
+            if ((rc = project<T>(s, zj, t))) return rc;
+            if ((rc = spmv<T>(s, zj, u))) return rc;
+            const int k = j + 1;
+            for (int pass = 0; pass < 2; ++pass) {
+                double *hp = slot + pass * k;
+                for (int c0 = 0; c0 < k; c0 += 128) {
+                    const int kk = k - c0 < 128 ? k - c0 : 128;
+                    if ((rc = ssr::multidot<T>(n, kk, V + (int64_t)c0 * n, n, u, hp + c0))) return rc;
+                }
+                for (int c0 = 0; c0 < k; c0 += 128) {
+                    const int kk = k - c0 < 128 ? k - c0 : 128;
+                    if ((rc = ssr::gemv_update<T, T>(n, kk, V + (int64_t)c0 * n, n, hp + c0, 1.0, -1.0, u))) return rc;
+                }
+            }
+            if ((rc = ssr::dot<T>(n, u, u, nullptr, slot + 2 * k))) return rc;
+            if ((rc = ssr::scale_inv_sqrt<T>(n, slot + 2 * k, 1e-14, u))) return rc;
+            SSR_HIP_CHECK(hipMemcpyAsync(s->hhost.p + col_off(j), slot, sizeof(double) * (2 * k + 1),
+                                         hipMemcpyDeviceToHost, ssr_stream()));
+            SSR_HIP_CHECK(hipEventRecord(s->ev[j], ssr_stream()));
+            enq++;
+        }
+        if (proc < enq && (enq - proc > lag || enq == m)) {
+            if ((rc = process(proc))) return rc;
+            proc++;
+        }
+    }
+    const int it = done_cols;
+    // back substitution R y = g
+    std::vector<double> y(it, 0.0);
+    for (int i = it - 1; i >= 0; --i) {
+        double tt = g[i];
+        for (int c = i + 1; c < it; ++c) tt -= H[(size_t)c * (m + 1) + i] * y[c];
+        const double d = H[(size_t)i * (m + 1) + i];
+        y[i] = d != 0.0 ? tt / d : 0.0;
+    }
+    // iterations enqueued beyond `it` only wrote V/Z columns past it; drain
+    // them before reusing the host buffers
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+    memcpy(s->yhost.p, y.data(), sizeof(double) * it);
+    SSR_HIP_CHECK(hipMemcpyAsync(s->ydev.p, s->yhost.p, sizeof(double) * it, hipMemcpyHostToDevice, ssr_stream()));
+    for (int c0 = 0; c0 < it; c0 += 128) {
+        const int k = it - c0 < 128 ? it - c0 : 128;
+        if ((rc = ssr::gemv_update<T, double>(n, k, Z + (int64_t)c0 * n, n, s->ydev.p + c0, 1.0, 1.0, x))) return rc;
+    }
+    *iters = it;
+    *res_out = res;
+    return SSR_OK;
+}
+
+int ensure_arnoldi(ssr_solver *s, int m)
+{
+    int rc;
+    const int64_t nh = col_off(m) + 1;
+    if ((rc = s->hcol.alloc(nh)) || (rc = s->hhost.alloc(nh)) || (rc = s->ydev.alloc(m + 1)) ||
+        (rc = s->yhost.alloc(m + 1)))
+        return rc;
+    while ((int)s->ev.size() < m) {
+        hipEvent_t e;
+        SSR_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        s->ev.push_back(e);
+    }
+    return SSR_OK;
 }
 
 }  // namespace
@@ -179,8 +357,8 @@ int ssr_solver_create(const ssr_pattern *P, int maxit, double tol, double defl_t
     SSR_REQUIRE(P && out, "solver_create: null");
     SSR_REQUIRE(maxit >= 0 && maxit <= 4096, "solver_create: maxit out of range");
     ssr_solver *s = new ssr_solver();
-    int64_t n_p, n_e, nnzb, nnzf;
-    ssr_pattern_sizes(P, &s->dim, &s->n_n, &n_p, &n_e, &nnzb, &nnzf);
+    int64_t n_p, n_e, nnzf;
+    ssr_pattern_sizes(P, &s->dim, &s->n_n, &n_p, &n_e, &s->nnzb, &nnzf);
     s->P = P;
     s->n = s->dim * s->n_n;
     s->row_ptr = ssr_pattern_d_row_ptr(P);
@@ -197,7 +375,8 @@ int ssr_solver_create(const ssr_pattern *P, int maxit, double tol, double defl_t
         return SSR_ERR_ARG;
     }
     int rc;
-    if ((rc = s->t1.alloc(s->n)) || (rc = s->t2.alloc(s->n)) || (rc = s->hdev.alloc(8192))) {
+    if ((rc = s->t1.alloc(s->n)) || (rc = s->t2.alloc(s->n)) || (rc = s->r.alloc(s->n)) ||
+        (rc = s->hW.alloc(8192)) || (rc = s->scal.alloc(8))) {
         delete s;
         return rc;
     }
@@ -223,13 +402,38 @@ int ssr_solver_set_precond(ssr_solver *s, ssr_precond *M)
     return SSR_OK;
 }
 
+// 64: plain double; 32: float Krylov + preconditioner with double refinement.
+int ssr_solver_set_precision(ssr_solver *s, int bits)
+{
+    SSR_REQUIRE(s && (bits == 64 || bits == 32), "solver_set_precision: bits must be 32 or 64");
+    s->precision = bits;
+    s->K32_valid = false;
+    s->kW32 = -1;
+    return SSR_OK;
+}
+
+int ssr_solver_get_precision(const ssr_solver *s, int *bits)
+{
+    SSR_REQUIRE(s && bits, "solver_get_precision: null");
+    *bits = s->precision;
+    return SSR_OK;
+}
+
 // Operator values (device, node-block CSR of the pattern; borrowed) and
 // preconditioner setup on them (setup_preconditioner / update_values).
 int ssr_solver_set_matrix(ssr_solver *s, const double *Kvals, int setup_precond)
 {
     SSR_REQUIRE(s && Kvals, "solver_set_matrix: null");
     s->K = Kvals;
-    if (setup_precond) return s->M->setup(Kvals);
+    int rc;
+    s->K32_valid = false;
+    if (s->precision == 32) {
+        const int64_t nv = s->nnzb * s->dim * s->dim;
+        if ((rc = s->K32.alloc(nv))) return rc;
+        if ((rc = ssr::convert<double, float>(nv, 1.0, Kvals, s->K32.p))) return rc;
+        s->K32_valid = true;
+    }
+    if (setup_precond) return s->M->setup(Kvals, s->precision == 32 ? s->K32.p : nullptr);
     return SSR_OK;
 }
 
@@ -242,6 +446,7 @@ int ssr_solver_deflation_size(const ssr_solver *s, int *k)
 int ssr_solver_clear_deflation(ssr_solver *s)
 {
     s->kW = 0;
+    s->kW32 = -1;
     return SSR_OK;
 }
 
@@ -264,9 +469,10 @@ int ssr_solver_expand_deflation(ssr_solver *s, const double *v)
     }
     double *dst = s->W.p + (int64_t)s->kW * s->n;
     SSR_HIP_CHECK(hipMemcpyAsync(dst, v, sizeof(double) * s->n, hipMemcpyDeviceToDevice, ssr_stream()));
-    int rc = ssr_dev_mul(s->n, s->mask, dst);
+    int rc = ssr::mul<double>(s->n, s->mask, dst);
     if (rc) return rc;
     s->kW++;
+    s->kW32 = -1;
     return SSR_OK;
 }
 
@@ -275,6 +481,7 @@ int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src)
 {
     SSR_REQUIRE(dst && src && dst->n == src->n, "copy_deflation: mismatched solvers");
     dst->kW = 0;
+    dst->kW32 = -1;
     for (int c = 0; c < src->kW; ++c) {
         int rc = ssr_solver_expand_deflation(dst, src->W.p + (int64_t)c * src->n);
         if (rc) return rc;
@@ -291,14 +498,15 @@ int ssr_solver_get_deflation(const ssr_solver *s, int c, double *out)
     return SSR_OK;
 }
 
-// A-orthonormalise the basis against the current operator (A_orthogonalize.m).
+// A-orthonormalise the basis against the current operator (A_orthogonalize.m),
+// in double.
 int ssr_solver_A_orthogonalize(ssr_solver *s)
 {
     SSR_REQUIRE(s && s->K, "A_orthogonalize: operator not set");
     const int k = s->kW;
     if (k == 0) return SSR_OK;
     int rc;
-    DevVec Wn;
+    DevBuf<double> Wn;
     if ((rc = Wn.alloc(s->n * (int64_t)k))) return rc;
     std::vector<double> sgn;  // signs of accepted columns, in acceptance order
     int acc = 0;
@@ -307,21 +515,23 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
         const double *wsrc = s->W.p + (int64_t)(k - 1 - i) * s->n;
         double *temp = Wn.p + (int64_t)acc * s->n;  // candidate slot
         SSR_HIP_CHECK(hipMemcpyAsync(temp, wsrc, sizeof(double) * s->n, hipMemcpyDeviceToDevice, ssr_stream()));
-        if ((rc = spmv(s, temp, s->t1.p))) return rc;  // A_temp
+        if ((rc = spmv<double>(s, temp, s->t1.p))) return rc;  // A_temp
         if (acc > 0) {
             // temp -= (W_orth .* norms') * (A_temp' W_orth)'
             for (int c0 = 0; c0 < acc; c0 += 128) {
                 const int kk = acc - c0 < 128 ? acc - c0 : 128;
-                if ((rc = ssr_dev_multidot(s->n, kk, Wn.p + (int64_t)c0 * s->n, s->n, s->t1.p, s->hdev.p + c0))) return rc;
+                if ((rc = ssr::multidot<double>(s->n, kk, Wn.p + (int64_t)c0 * s->n, s->n, s->t1.p, s->hW.p + c0)))
+                    return rc;
             }
             hc.resize(acc);
-            SSR_HIP_CHECK(hipMemcpyAsync(hc.data(), s->hdev.p, sizeof(double) * acc, hipMemcpyDeviceToHost, ssr_stream()));
+            SSR_HIP_CHECK(hipMemcpyAsync(hc.data(), s->hW.p, sizeof(double) * acc, hipMemcpyDeviceToHost, ssr_stream()));
             SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
             for (int c = 0; c < acc; ++c) hc[c] *= sgn[c];
-            SSR_HIP_CHECK(hipMemcpyAsync(s->hdev.p, hc.data(), sizeof(double) * acc, hipMemcpyHostToDevice, ssr_stream()));
+            SSR_HIP_CHECK(hipMemcpyAsync(s->hW.p, hc.data(), sizeof(double) * acc, hipMemcpyHostToDevice, ssr_stream()));
             for (int c0 = 0; c0 < acc; c0 += 128) {
                 const int kk = acc - c0 < 128 ? acc - c0 : 128;
-                if ((rc = ssr_dev_gemv_update(s->n, kk, Wn.p + (int64_t)c0 * s->n, s->n, s->hdev.p + c0, 1.0, -1.0, temp)))
+                if ((rc = ssr::gemv_update<double, double>(s->n, kk, Wn.p + (int64_t)c0 * s->n, s->n, s->hW.p + c0, 1.0,
+                                                           -1.0, temp)))
                     return rc;
             }
         }
@@ -331,7 +541,7 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
             double sg = 1.0;
             if (nt > 0) nt = sqrt(nt);
             else { nt = sqrt(fabs(nt)); sg = -1.0; }
-            if ((rc = ssr_dev_axpby(s->n, 1.0 / nt, temp, 0.0, temp))) return rc;
+            if ((rc = ssr::axpby<double>(s->n, 1.0 / nt, temp, 0.0, temp))) return rc;
             sgn.push_back(sg);
             acc++;
         }
@@ -342,6 +552,7 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
                                      sizeof(double) * s->n, hipMemcpyDeviceToDevice, ssr_stream()));
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     s->kW = acc;
+    s->kW32 = -1;
     return SSR_OK;
 }
 
@@ -350,121 +561,67 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
 int ssr_solver_dfgmres(ssr_solver *s, const double *b, double *x, int *iters, double *relres)
 {
     SSR_REQUIRE(s && s->K && b && x, "dfgmres: operator or vectors missing");
+    SSR_REQUIRE(s->precision == 64 || s->K32_valid, "dfgmres: float operator missing (set_matrix after set_precision)");
     const int64_t n = s->n;
     const int m = s->maxit;
     int rc;
-    if ((rc = s->V.alloc(n * (int64_t)(m + 1))) || (rc = s->Z.alloc(n * (int64_t)(m > 0 ? m : 1)))) return rc;
-    // x0
+    if ((rc = ensure_arnoldi(s, m))) return rc;
+    // x0 = W W' b (double)
     if (s->kW > 0) {
-        if ((rc = defl_coef(s, b))) return rc;
-        if ((rc = defl_update(s, 0.0, 1.0, x))) return rc;
+        if ((rc = defl_apply<double>(s, b, 0.0, 1.0, x))) return rc;
     } else {
-        SSR_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, ssr_stream()));
+        if ((rc = ssr::fill<double>(n, 0.0, x))) return rc;
     }
-    // r0 = b - A x  (into V(:,0))
-    double *V = s->V.p, *Zp = s->Z.p;
-    if ((rc = spmv(s, x, V))) return rc;
-    if ((rc = ssr_dev_axpby(n, 1.0, b, -1.0, V))) return rc;
+    // r0 = b - A x
+    if ((rc = spmv<double>(s, x, s->r.p, -1.0, 1.0, b))) return rc;
     double bnorm = hnorm(s, b, &rc);
     if (rc) return rc;
     if (bnorm == 0.0) bnorm = 1.0;
-    double res = hnorm(s, V, &rc);
+    double res = hnorm(s, s->r.p, &rc);
     if (rc) return rc;
     s->n_solves++;
+    int it_total = 0, refine = 0;
     if (res / bnorm < s->tol || m == 0) {
-        *iters = 0;
-        *relres = res / bnorm;
-        s->last_iters = 0;
-        s->last_relres = *relres;
-        return SSR_OK;
-    }
-    if ((rc = ssr_dev_axpby(n, 1.0 / res, V, 0.0, V))) return rc;
-    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1), h2(m + 1);
-    g[0] = res;
-    int it = 0;
-    bool done = false;
-    for (int j = 0; j < m && !done; ++j) {
-        double *zj = Zp + (int64_t)j * n;
-        double *u = V + (int64_t)(j + 1) * n;
-        if ((rc = precond(s, V + (int64_t)j * n, zj))) return rc;
-        if ((rc = project(s, zj))) return rc;
-        if ((rc = spmv(s, zj, u))) return rc;
-        // CGS2
-        const int k = j + 1;
-        for (int pass = 0; pass < 2; ++pass) {
-            // all coefficients first (classical Gram-Schmidt), in chunks of 128 columns
-            for (int c0 = 0; c0 < k; c0 += 128) {
-                const int kk = k - c0 < 128 ? k - c0 : 128;
-                if ((rc = ssr_dev_multidot(n, kk, V + (int64_t)c0 * n, n, u, s->hdev.p + c0))) return rc;
-            }
-            std::vector<double> &hv = pass == 0 ? h : h2;
-            SSR_HIP_CHECK(hipMemcpyAsync(hv.data(), s->hdev.p, sizeof(double) * k, hipMemcpyDeviceToHost, ssr_stream()));
-            for (int c0 = 0; c0 < k; c0 += 128) {
-                const int kk = k - c0 < 128 ? k - c0 : 128;
-                if ((rc = ssr_dev_gemv_update(n, kk, V + (int64_t)c0 * n, n, s->hdev.p + c0, 1.0, -1.0, u))) return rc;
-            }
+        // converged at x0
+    } else if (s->precision == 64) {
+        if ((rc = s->V.alloc(n * (int64_t)(m + 1))) || (rc = s->Z.alloc(n * (int64_t)m))) return rc;
+        if ((rc = ssr::axpby<double>(n, 1.0 / res, s->r.p, 0.0, s->V.p))) return rc;
+        double est = res;
+        if ((rc = fgmres_core<double>(s, s->V.p, s->Z.p, s->t2.p, res, s->tol * bnorm, m, x, &it_total, &est))) return rc;
+        res = est;
+        refine = 1;
+    } else {
+        if ((rc = s->V32.alloc(n * (int64_t)(m + 1))) || (rc = s->Z32.alloc(n * (int64_t)m)) || (rc = s->t2f.alloc(n)))
+            return rc;
+        if ((rc = ensure_w32(s))) return rc;
+        while (it_total < m && refine < s->max_refine) {
+            if ((rc = ssr::convert<double, float>(n, 1.0 / res, s->r.p, s->V32.p))) return rc;
+            // a float Krylov space cannot resolve much below ~1e-6 of its start
+            const double target = fmax(s->tol * bnorm, 1e-6 * res);
+            int it = 0;
+            double est = 0.0;
+            if ((rc = fgmres_core<float>(s, s->V32.p, s->Z32.p, s->t2f.p, res, target, m - it_total, x, &it, &est)))
+                return rc;
+            it_total += it;
+            refine++;
+            // true residual in double
+            if ((rc = spmv<double>(s, x, s->r.p, -1.0, 1.0, b))) return rc;
+            res = hnorm(s, s->r.p, &rc);
+            if (rc) return rc;
+            if (res / bnorm < s->tol || it == 0) break;
         }
-        SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
-        for (int i = 0; i < k; ++i) h[i] += h2[i];
-        const double hn = hnorm(s, u, &rc);
-        if (rc) return rc;
-        double *Hc = H.data() + (size_t)j * (m + 1);  // column j
-        for (int i = 0; i < k; ++i) Hc[i] = h[i];
-        Hc[k] = hn;
-        it = j + 1;
-        if (hn < 1e-14) {
-            // exact solution within this Krylov space: close the column with
-            // the previous rotations only
-            for (int i = 0; i < j; ++i) {
-                const double a = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
-                Hc[i + 1] = -sn[i] * Hc[i] + cs[i] * Hc[i + 1];
-                Hc[i] = a;
-            }
-            cs[j] = 1.0;
-            sn[j] = 0.0;
-            res = fabs(g[j + 1]);
-            done = true;
-            break;
-        }
-        if ((rc = ssr_dev_axpby(n, 1.0 / hn, u, 0.0, u))) return rc;
-        for (int i = 0; i < j; ++i) {
-            const double a = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
-            Hc[i + 1] = -sn[i] * Hc[i] + cs[i] * Hc[i + 1];
-            Hc[i] = a;
-        }
-        const double den = hypot(Hc[j], Hc[j + 1]);
-        cs[j] = Hc[j] / den;
-        sn[j] = Hc[j + 1] / den;
-        Hc[j] = den;
-        Hc[j + 1] = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
-        res = fabs(g[j + 1]);
-        if (res / bnorm < s->tol) done = true;
     }
-    // back substitution R y = g
-    std::vector<double> y(it, 0.0);
-    for (int i = it - 1; i >= 0; --i) {
-        double t = g[i];
-        for (int c = i + 1; c < it; ++c) t -= H[(size_t)c * (m + 1) + i] * y[c];
-        const double d = H[(size_t)i * (m + 1) + i];
-        y[i] = d != 0.0 ? t / d : 0.0;
-    }
-    for (int c0 = 0; c0 < it; c0 += 128) {
-        const int k = it - c0 < 128 ? it - c0 : 128;
-        SSR_HIP_CHECK(hipMemcpyAsync(s->hdev.p, y.data() + c0, sizeof(double) * k, hipMemcpyHostToDevice, ssr_stream()));
-        if ((rc = ssr_dev_gemv_update(n, k, Zp + (int64_t)c0 * n, n, s->hdev.p, 1.0, 1.0, x))) return rc;
-        SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
-    }
-    *iters = it;
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+    *iters = it_total;
     *relres = res / bnorm;
-    s->last_iters = it;
+    s->last_iters = it_total;
+    s->last_refine = refine;
     s->last_relres = *relres;
-    s->total_iters += it;
+    s->total_iters += it_total;
     return SSR_OK;
 }
 
-// Deflated preconditioned CG (dcg_solver.m).
+// Deflated preconditioned CG (dcg_solver.m), double.
 int ssr_solver_dcg(ssr_solver *s, const double *b, double *x, int *iters, double *relres)
 {
     SSR_REQUIRE(s && s->K && b && x, "dcg: operator or vectors missing");
@@ -473,13 +630,11 @@ int ssr_solver_dcg(ssr_solver *s, const double *b, double *x, int *iters, double
     if ((rc = s->V.alloc(4 * n))) return rc;
     double *r = s->V.p, *z = r + n, *p = z + n, *q = p + n;
     if (s->kW > 0) {
-        if ((rc = defl_coef(s, b))) return rc;
-        if ((rc = defl_update(s, 0.0, 1.0, x))) return rc;
+        if ((rc = defl_apply<double>(s, b, 0.0, 1.0, x))) return rc;
     } else {
-        SSR_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, ssr_stream()));
+        if ((rc = ssr::fill<double>(n, 0.0, x))) return rc;
     }
-    if ((rc = spmv(s, x, r))) return rc;
-    if ((rc = ssr_dev_axpby(n, 1.0, b, -1.0, r))) return rc;
+    if ((rc = spmv<double>(s, x, r, -1.0, 1.0, b))) return rc;
     const double bnorm = hnorm(s, b, &rc);
     if (rc) return rc;
     double res = hnorm(s, r, &rc) / bnorm;
@@ -490,30 +645,30 @@ int ssr_solver_dcg(ssr_solver *s, const double *b, double *x, int *iters, double
         *relres = res;
         return SSR_OK;
     }
-    if ((rc = precond(s, r, z))) return rc;
+    if ((rc = precond<double>(s, r, z))) return rc;
     SSR_HIP_CHECK(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, ssr_stream()));
-    if ((rc = project(s, p))) return rc;
+    if ((rc = project<double>(s, p, s->t2.p))) return rc;
     double gamma_old = 0.0;
     if ((rc = ssr_dot(n, r, z, nullptr, &gamma_old))) return rc;
     int j = 0;
     for (j = 1; j <= s->maxit; ++j) {
-        if ((rc = spmv(s, p, q))) return rc;
+        if ((rc = spmv<double>(s, p, q))) return rc;
         double sp = 0.0;
         if ((rc = ssr_dot(n, q, p, nullptr, &sp))) return rc;
         const double alpha = gamma_old / sp;
-        if ((rc = ssr_dev_axpby(n, alpha, p, 1.0, x))) return rc;
-        if ((rc = ssr_dev_axpby(n, -alpha, q, 1.0, r))) return rc;
+        if ((rc = ssr::axpby<double>(n, alpha, p, 1.0, x))) return rc;
+        if ((rc = ssr::axpby<double>(n, -alpha, q, 1.0, r))) return rc;
         res = hnorm(s, r, &rc) / bnorm;
         if (rc) return rc;
         if (res < s->tol) break;
-        if ((rc = precond(s, r, z))) return rc;
+        if ((rc = precond<double>(s, r, z))) return rc;
         double gamma_new = 0.0;
         if ((rc = ssr_dot(n, r, z, nullptr, &gamma_new))) return rc;
         const double beta = gamma_new / gamma_old;
         // p = P(z) + beta p   (q as scratch for P(z))
         SSR_HIP_CHECK(hipMemcpyAsync(q, z, sizeof(double) * n, hipMemcpyDeviceToDevice, ssr_stream()));
-        if ((rc = project(s, q))) return rc;
-        if ((rc = ssr_dev_axpby(n, 1.0, q, beta, p))) return rc;
+        if ((rc = project<double>(s, q, s->t2.p))) return rc;
+        if ((rc = ssr::axpby<double>(n, 1.0, q, beta, p))) return rc;
         gamma_old = gamma_new;
     }
     if (j > s->maxit) j = s->maxit;
@@ -525,7 +680,7 @@ int ssr_solver_dcg(ssr_solver *s, const double *b, double *x, int *iters, double
     return SSR_OK;
 }
 
-int ssr_solver_precond_apply(ssr_solver *s, const double *x, double *y) { return precond(s, x, y); }
+int ssr_solver_precond_apply(ssr_solver *s, const double *x, double *y) { return precond<double>(s, x, y); }
 
 const ssr_pattern *ssr_solver_pattern(const ssr_solver *s) { return s->P; }
 

@@ -46,8 +46,11 @@ int ssr_cu_count(void);  // compute units of the current device (cached)
 // Jacobi and smoothed-aggregation AMG implement it.
 struct ssr_precond {
     virtual ~ssr_precond() {}
-    virtual int setup(const double *Kvals) = 0;       // operator values (device)
+    // operator values (device); K32 is the float copy when the solver runs
+    // its Krylov iterations in float (else nullptr)
+    virtual int setup(const double *Kvals, const float *K32) = 0;
     virtual int apply(const double *x, double *y) = 0;  // y = M^{-1} x (device)
+    virtual int apply(const float *x, float *y) = 0;    // same in float
 };
 
 // Kernel-variant / launch-shape knobs (set by the autotuner or the caller).

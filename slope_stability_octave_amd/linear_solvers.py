@@ -37,6 +37,8 @@ _lib.extend_signatures({
     "ssr_solver_precond_apply": [_vp, _vp, _vp],
     "ssr_solver_stats": [_vp, _vp],
     "ssr_solver_set_amg": [_vp, _vp, _int, _vp],
+    "ssr_solver_set_precision": [_vp, _int],
+    "ssr_solver_get_precision": [_vp, _vp],
     "ssr_dev_spmv_bsr": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
     "ssr_dev_dot": [_i64, _vp, _vp, _vp, _vp],
     "ssr_dot": [_i64, _vp, _vp, _vp, _vp],
@@ -89,8 +91,9 @@ class DFGMRES:
     method = "dfgmres"
 
     def __init__(self, pattern, tolerance=1e-1, max_iterations=100, tolerance_deflation_basis=1e-3,
-                 preconditioner="block_jacobi", verbose=0, amg_options=None):
+                 preconditioner="block_jacobi", verbose=0, amg_options=None, precision=64):
         self.pattern = pattern
+        self.precision = int(precision)
         self.tolerance = float(tolerance)
         self.max_iterations = int(max_iterations)
         self.tolerance_deflation_basis = float(tolerance_deflation_basis)
@@ -104,6 +107,8 @@ class DFGMRES:
         call("ssr_solver_create", pattern.handle, self.max_iterations, self.tolerance,
              self.tolerance_deflation_basis, PRECONDITIONERS.get(preconditioner, 0), ctypes.byref(h))
         self.handle = h
+        if self.precision != 64:
+            call("ssr_solver_set_precision", self.handle, self.precision)
         if preconditioner == "amg":
             from . import amg
             amg.attach(self, amg_options)
@@ -167,7 +172,7 @@ class DFGMRES:
         """New solver with the same settings and a copy of the basis; the
         preconditioner is rebuilt on the next setup (DFGMRES.m:204-221)."""
         new = type(self)(self.pattern, self.tolerance, self.max_iterations, self.tolerance_deflation_basis,
-                         self.preconditioner_name, self.verbose, self.amg_options)
+                         self.preconditioner_name, self.verbose, self.amg_options, self.precision)
         call("ssr_solver_copy_deflation", new.handle, self.handle)
         new.profiler = self.profiler
         new.timings = self.timings  # shared, like the reference's profiler handle

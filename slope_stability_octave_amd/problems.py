@@ -74,7 +74,10 @@ def make_constitutive(prob, Davis_type="B"):
 
 
 def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", log=None, predictor=None,
-            amg_options=None):
+            amg_options=None, precision=64, recorder=None):
+    """``predictor`` may also be a factory called as predictor(cmb, f) once the
+    device problem exists (see learned.LearnedPredictor); ``recorder`` likewise
+    (factory(cmb) or a hook)."""
     from .continuation import SSR_indirect_continuation
     from .linear_solvers import DCG, DFGMRES
     p = dict(PARAMS_2D if prob["dim"] == 2 else PARAMS_3D)
@@ -86,14 +89,18 @@ def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", 
         amg_options = dict(amg_options or {})
         amg_options.setdefault("coord", prob["coord"])
     lin = cls(cmb.pattern, p["linear_tol"], p["linear_maxit"], p["deflation_tol"], preconditioner,
-              amg_options=amg_options)
+              amg_options=amg_options, precision=precision)
     f = DeviceArray.from_host(prob["f"].T)
+    if predictor is not None and getattr(predictor, "is_factory", False):
+        predictor = predictor(cmb, f)
+    if recorder is not None and getattr(recorder, "is_factory", False):
+        recorder = recorder(cmb)
     t_setup = time.perf_counter() - t0
     t1 = time.perf_counter()
     U, lam_h, om_h, umax_h, stats = SSR_indirect_continuation(
         p["lambda_init"], p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
         p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], None, prob["Q"], f, cmb,
-        lin, log=log, predictor=predictor)
+        lin, log=log, predictor=predictor, recorder=recorder)
     t_run = time.perf_counter() - t1
     timings = dict(cmb.timings)
     timings.update(lin.timings)
@@ -101,7 +108,8 @@ def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", 
     counts.update(lin.counters)
     return dict(FoS=float(lam_h[-1]), lambda_hist=lam_h, omega_hist=om_h, Umax_hist=umax_h, stats=stats,
                 U=U, setup_s=t_setup, time_to_fos_s=t_run, timings=timings, counts=counts,
-                newton_iterations=int(sum(stats["attempt_newton_iterations"])))
+                newton_iterations=int(sum(stats["attempt_newton_iterations"]) + sum(stats["init_newton_iterations"])),
+                predictor=predictor, recorder=recorder)
 
 
 # slope_stability_3D_homo_SSR.m:118-140 (recorded run: ss/slope_stability_3D_homo_SSR.ipynb:366-410)
@@ -115,3 +123,18 @@ def slope_3d_mesh(data, materials=None, mat_id=None):
     mats = materials or HOMO_2D
     mid = np.zeros(elem.shape[1], dtype=np.int64) if mat_id is None else np.asarray(mat_id)
     return _finish(coord, elem, Q, 3, mats, mid)
+
+
+def slope_2d_family(seed, h=1.0):
+    """2D homogeneous slope of a seeded family: slope angle ~ U[26, 63] deg,
+    c0 ~ U[5, 30] kPa, phi ~ U[20, 40] deg, psi = 0, E = 40 MPa, nu = 0.3,
+    gamma = 20 kN/m^3; 10 m foundation, 10 m high slope, 15 m berms.
+    Returns (problem, (slope angle, c0, phi))."""
+    rng = np.random.default_rng(seed)
+    beta = float(rng.uniform(26.0, 63.0))
+    c0 = float(rng.uniform(5.0, 30.0))
+    phi = float(rng.uniform(20.0, 40.0))
+    y2 = 10.0
+    x2 = y2 / np.tan(np.deg2rad(beta))
+    mats = [dict(c0=c0, phi=phi, psi=0.0, young=40000.0, poisson=0.3, gamma_sat=20.0, gamma_unsat=20.0)]
+    return slope_2d(h=h, x1=15.0, x2=x2, x3=15.0, y1=10.0, y2=y2, materials=mats), (beta, c0, phi)

@@ -146,3 +146,41 @@ def test_amg_preconditioned_fgmres(dim):
         assert res <= 2e-8, (name, res)
     print("iterations", its)
     assert its["amg"] * 3 <= its["block_jacobi"]
+
+
+@pytest.mark.parametrize("precond", ["block_jacobi", "amg"])
+@pytest.mark.parametrize("tol", [1e-2, 1e-8])
+def test_mixed_precision_refinement_meets_fp64_tolerance(precond, tol):
+    """Float Krylov + float preconditioner with double refinement: the TRUE
+    double residual meets the tolerance (also below float resolution, 1e-8,
+    through repeated float solves), and the solution agrees with the FP64
+    solver's to the accuracy both guarantee."""
+    prob = problems.slope_3d((2, 2, 2, 2, 2, 2), 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, problems.HOMO_2D)
+    cmb = problems.make_constitutive(prob)
+    cmb.reduction(1.0)
+    U = DeviceArray((cmb.n_n, cmb.dim), zero=True)
+    cmb.build_F_and_DS_all(1.0, U)
+    K = cmb.build_K_r_QQ_vals(1e-4)
+    rng = np.random.default_rng(11)
+    b = DeviceArray.from_host(rng.normal(size=(cmb.n_n, 3)) * cmb.Q.T)
+    xs = {}
+    for prec in (64, 32):
+        s = DFGMRES(cmb.pattern, tol, 1500, 1e-3, precond, amg_options=dict(coord=prob["coord"]), precision=prec)
+        s.setup_preconditioner(K)
+        # a deflation basis exercises the float projector as well
+        for c in range(3):
+            s.expand_deflation_basis(DeviceArray.from_host(rng.normal(size=(cmb.n_n, 3)) * cmb.Q.T))
+        s.A_orthogonalize(K)
+        x, it = s.solve(K, b)
+        r = DeviceArray((cmb.n_n, 3))
+        spmv(cmb.pattern, K, x, r)
+        res = np.linalg.norm(r.get() - b.get()) / np.linalg.norm(b.get())
+        print(prec, precond, tol, "iterations", it, "relres", res, s.last_relres, flush=True)
+        assert res < tol * 1.0001, (prec, res)
+        if prec == 32:
+            assert abs(s.last_relres - res) <= 1e-3 * res + 1e-14  # reported residual is the true one
+        xs[prec] = x.get()
+    # both solutions solve K x = b to relative residual tol; the difference is bounded by
+    # cond(K)*2*tol relative -- check a loose, tolerance-scaled agreement
+    d = np.linalg.norm(xs[32] - xs[64]) / np.linalg.norm(xs[64])
+    assert d < max(1e3 * tol, 1e-4), d
