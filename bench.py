@@ -87,6 +87,12 @@ def run_fos(args):
         if args.omega_max != 7e7:
             params["omega_max_stop"] = args.omega_max
         name = "3D homogeneous slope SSR, P2 tetrahedra, mesh SSR_homo_ada_L1 (25591 nodes, 16615 elements)"
+    elif args.workload == "fos2d_family":
+        prob, mat = problems.slope_2d_family(args.seed, h=args.h)
+        params = dict(problems.PARAMS_FAMILY)
+        if args.omega_max != 7e7:
+            params["omega_max_stop"] = args.omega_max
+        name = f"2D slope family seed {args.seed} (angle, c, phi = {tuple(round(x, 2) for x in mat)}), P2 h={args.h}"
     else:
         prob = problems.slope_2d(h=args.h)
         params = dict(problems.PARAMS_2D)
@@ -97,8 +103,10 @@ def run_fos(args):
         params["linear_tol"] = args.linear_tol
     log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
     amg_opts = {"rebuild_every": args.amg_rebuild_every} if args.amg_rebuild_every is not None else None
-    if args.amg_theta is not None:
-        amg_opts = dict(amg_opts or {}, theta=args.amg_theta)
+    for key, val in (("theta", args.amg_theta), ("cheb_degree", args.amg_cheb_degree),
+                     ("max_coarse", args.amg_max_coarse)):
+        if val is not None:
+            amg_opts = dict(amg_opts or {}, **{key: val})
     res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts,
                            precision=args.precision)
     out = {
@@ -127,6 +135,61 @@ def run_fos(args):
     print(json.dumps(out))
 
 
+def run_learned(args):
+    """Chosen form (learned predictor + FP32 Krylov/AMG with FP64 refinement)
+    against the plain GPU form (secant predictor, FP64) on held-out members of
+    the seeded 2D slope family.  The network is trained first on trajectories
+    the plain solver generates for other seeds (training time reported, not
+    part of time-to-FoS)."""
+    from slope_stability_octave_amd import _lib, learned, problems
+
+    _lib.require_gpu()
+    log = (lambda s: print(s, file=sys.stderr, flush=True))
+    t0 = time.perf_counter()
+    train_raw = learned.generate(range(args.train_seeds), h=args.h_train, log=log, linear_tol=args.linear_tol or 1e-2)
+    val_raw = learned.generate(range(1000, 1004), h=args.h_train, log=log, linear_tol=args.linear_tol or 1e-2)
+    gen_s = time.perf_counter() - t0
+    net, norm, rep = learned.fit(train_raw, val_raw, epochs=args.epochs, log=log)
+    log(f"[mlp] fit {rep}")
+    rows = []
+    tot = {"plain": 0.0, "chosen": 0.0}
+    for seed in range(2000, 2000 + args.eval_seeds):
+        prob, mat = problems.slope_2d_family(seed, h=args.h)
+        p = dict(problems.PARAMS_FAMILY, linear_tol=args.linear_tol or 1e-2)
+        r_plain = problems.run_ssr(prob, p, preconditioner="amg", precision=64)
+        pred = learned.predictor_factory(net, norm, mat)
+        r_ch = problems.run_ssr(prob, p, preconditioner="amg", precision=32, predictor=pred)
+        pr = r_ch["predictor"]
+        row = dict(seed=seed, material=[round(x, 3) for x in mat], free_dof=prob["n_dof_free"],
+                   FoS_plain=r_plain["FoS"], FoS_chosen=r_ch["FoS"], dFoS=abs(r_plain["FoS"] - r_ch["FoS"]),
+                   t_plain=r_plain["time_to_fos_s"], t_chosen=r_ch["time_to_fos_s"],
+                   newton_plain=r_plain["newton_iterations"], newton_chosen=r_ch["newton_iterations"],
+                   steps_plain=len(r_plain["lambda_hist"]), steps_chosen=len(r_ch["lambda_hist"]),
+                   predictor_used=pr.used, predictor_rejected=pr.rejected)
+        log(f"[eval] {row}")
+        rows.append(row)
+        tot["plain"] += row["t_plain"]
+        tot["chosen"] += row["t_chosen"]
+    out = {
+        "metric": "SSR time-to-FoS reduction, learned predictor + FP32 Krylov/AMG vs plain FP64 GPU form",
+        "value": 1.0 - tot["chosen"] / tot["plain"],
+        "unit": "fraction of plain time-to-FoS saved",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "f64 (Newton, constitutive) / f32 (Krylov, AMG)",
+        "data": "synthetic",
+        "config": {"workload": f"2D slope family (angle U[26,63] deg, c U[5,30] kPa, phi U[20,40] deg), "
+                               f"P2 h={args.h}, held-out seeds 2000..{1999 + args.eval_seeds}",
+                   "train": f"{args.train_seeds} seeds at h={args.h_train}", "linear_tol": args.linear_tol or 1e-2},
+        "t_plain_s": tot["plain"], "t_chosen_s": tot["chosen"],
+        "max_dFoS": max(r["dFoS"] for r in rows),
+        "newton_plain": sum(r["newton_plain"] for r in rows), "newton_chosen": sum(r["newton_chosen"] for r in rows),
+        "training": dict(rep, data_generation_s=gen_s),
+        "instances": rows,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,7 +197,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d"])
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos2d_family", "learned"])
     ap.add_argument("--h", type=float, default=0.5)
     ap.add_argument("--lambda-init", type=float, default=1.0)
     ap.add_argument("--omega-max", type=float, default=7e7)
@@ -149,6 +212,15 @@ def main():
     ap.add_argument("--precision", type=int, default=64, choices=[32, 64],
                     help="Krylov/preconditioner arithmetic of the FoS workloads (32: float with double refinement)")
     ap.add_argument("--amg-theta", type=float, default=None, help="AMG strength threshold (FoS workloads)")
+    ap.add_argument("--amg-cheb-degree", type=int, default=None, help="AMG Chebyshev smoothing degree")
+    ap.add_argument("--amg-max-coarse", type=int, default=None, help="AMG coarsest-level size")
+    ap.add_argument("--seed", type=int, default=2000, help="member of the 2D slope family (fos2d_family)")
+    ap.add_argument("--train-seeds", type=int, default=16)
+    ap.add_argument("--eval-seeds", type=int, default=4)
+    ap.add_argument("--h-train", type=float, default=1.0)
+    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the 3D kernel forms on this workload first and use the fastest")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="also time write-only, copy and read-only streaming kernels of the DS size")
     args = ap.parse_args()
@@ -160,8 +232,10 @@ def main():
     if args.variant is not None:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_CONSTITUTIVE_VARIANT, args.variant)
-    if args.workload in ("fos2d", "fos3d"):
+    if args.workload in ("fos2d", "fos3d", "fos2d_family"):
         return run_fos(args)
+    if args.workload == "learned":
+        return run_learned(args)
 
     from slope_stability_octave_amd import _lib
     from slope_stability_octave_amd import constitutive_mex as cm
@@ -196,6 +270,11 @@ def main():
     cm.reduction_dev(n, "B", dc0, dphi, dpsi, dlam, True, dcb, dsp)
     tim.stop()
     red_ms = tim.elapsed_ms()
+    tuned = None
+    if args.autotune:
+        from slope_stability_octave_amd.autotune import Autotuner
+        tuned = Autotuner().constitutive(n, dE, dcb, dsp, dG, dK, dL, dS, dD)
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         cm.constitutive_dev(3, n, dE, dcb, dsp, dG, dK, dL, dS, dD)
     _lib.synchronize()
@@ -306,6 +385,8 @@ def main():
             "stress_only_hbm_frac": 136 * n / (s_only_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "parity": parity,
             "hbm_probe": probe,
+            "kernel_variant": _lib.get_tuning(_lib.TUNE_CONSTITUTIVE_VARIANT),
+            "autotune": tuned,
         },
     }
     print(json.dumps(out))

@@ -207,6 +207,32 @@ int ssr_solver_precond_apply(ssr_solver *s, const double *x, double *y);
 /* out[4] = total iterations, solves, preconditioner applications, SpMVs */
 int ssr_solver_stats(const ssr_solver *s, int64_t *out);
 
+/* ---- learned continuation predictor (small MLP) --------------------------- */
+/* No reference counterpart: the reference continuation has the secant
+ * predictor only (SSR_indirect_continuation.m:119), which remains the
+ * fallback.  Widths {n_in, n_hidden x (n_layers-1), n_out}; tanh hidden
+ * layers; bf16 weights (round-to-nearest-even of the float master) with float
+ * accumulation in the forward pass; parameters per layer W (out x in,
+ * row-major) then b.  X, Y, dY, grad and the loss data are device pointers. */
+typedef struct ssr_mlp ssr_mlp;
+int ssr_mlp_create(int n_in, int n_hidden, int n_out, int n_layers, uint64_t seed, double out_scale,
+                   ssr_mlp **out);
+int ssr_mlp_destroy(ssr_mlp *m);
+/* info[0] = n_params, info[1] = n_layers, info[2..] = widths */
+int ssr_mlp_info(const ssr_mlp *m, int64_t *info, int cap);
+int ssr_mlp_get_params(const ssr_mlp *m, float *host);
+int ssr_mlp_set_params(ssr_mlp *m, const float *host);
+int ssr_mlp_forward(ssr_mlp *m, int B, const float *X, float *Y);
+/* grad of sum_s dY[s]'Y[s] w.r.t. the parameters at X */
+int ssr_mlp_backward(ssr_mlp *m, int B, const float *X, const float *dY, float *grad);
+/* per-sample loss delta'G delta - 2 delta'g + h + w_lam (y_nc - lam_t)^2 with
+ * delta = Y[:, 0:nc], and d(mean loss)/dY (dY may be NULL) */
+int ssr_mlp_predictor_loss(int B, int nc, const float *Y, const float *G, const float *g, const float *h,
+                           const float *lam_t, float w_lam, float *dY, float *loss);
+/* one AdamW step on the mean predictor loss; mean loss before the step to *loss_host */
+int ssr_mlp_train_step(ssr_mlp *m, int B, const float *X, const float *G, const float *g, const float *h,
+                       const float *lam_t, float w_lam, float lr, float wd, float *loss_host);
+
 #ifdef __cplusplus
 }
 #endif

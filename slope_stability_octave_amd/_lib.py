@@ -190,6 +190,7 @@ class DeviceArray:
 TUNE_CONSTITUTIVE_VARIANT = 0
 TUNE_SPMV_GROUP = 1
 TUNE_PERSIST_WAVES = 2
+TUNE_AMG_GRAPH = 3
 
 
 def set_tuning(key: int, value: int):

@@ -310,6 +310,7 @@ struct Amg : ssr_precond {
     bool want32 = false;  // keep a float copy of the device hierarchy
     std::vector<double> Kh;
     int64_t n_setup = 0;
+    int64_t generation = 0;  // bumped whenever device level data is (re)uploaded
     bool built = false;
 
     Amg(const ssr_pattern *P, const double *near_null, int n_null, const Options &o) : Pat(P), opt(o)
@@ -333,6 +334,9 @@ struct Amg : ssr_precond {
     {
         (void)hipFree(d_coarse_inv);
         (void)hipFree(d_coarse_inv32);
+        vg64.reset();
+        vg32.reset();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
     }
 
     void fine_from_bsr(HCsr &A)
@@ -530,6 +534,7 @@ struct Amg : ssr_precond {
     template <class T>
     int upload_level(size_t l)
     {
+        generation++;
         Level &lv = *L[l];
         DevLevel<T> &dl = lv.dev<T>();
         int rc;
@@ -709,6 +714,7 @@ struct Amg : ssr_precond {
             }
         }
         coarse_inv.swap(I);
+        generation++;
         int rc = dupload(&d_coarse_inv, coarse_inv);
         if (rc || !want32) return rc;
         std::vector<float> ci32(coarse_inv.begin(), coarse_inv.end());
@@ -806,12 +812,78 @@ struct Amg : ssr_precond {
         return smooth<T>(l, b, x, false);
     }
 
+    // The V-cycle as one captured HIP graph over fixed input/output buffers
+    // (about twenty kernels become one launch plus two copies).  The graph
+    // bakes in every device pointer of the hierarchy and the finest-level
+    // operator, so it is re-captured when the hierarchy is rebuilt or
+    // refreshed (generation) or the operator buffer changes.  The V-cycle
+    // contains kernels only (no memset or host synchronisation).
+    template <class T>
+    struct VGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        T *bin = nullptr, *xout = nullptr;
+        const void *key = nullptr;
+        int64_t gen = -1;
+        void reset()
+        {
+            if (ge) (void)hipGraphExecDestroy(ge);
+            if (g) (void)hipGraphDestroy(g);
+            ge = nullptr;
+            g = nullptr;
+        }
+        ~VGraph()
+        {
+            reset();
+            (void)hipFree(bin);
+            (void)hipFree(xout);
+        }
+    };
+    VGraph<double> vg64;
+    VGraph<float> vg32;
+    hipStream_t cap_stream = nullptr;  // capture only (default priority, non-blocking)
+    template <class T> VGraph<T> &vgraph();
+
+    template <class T>
+    int apply_graph(const T *x, T *y)
+    {
+        VGraph<T> &G = vgraph<T>();
+        if (!G.ge || G.key != (const void *)fine_vals<T>() || G.gen != generation) {
+            G.reset();
+            if (!G.bin) {
+                SSR_HIP_CHECK(hipMalloc((void **)&G.bin, sizeof(T) * n));
+                SSR_HIP_CHECK(hipMalloc((void **)&G.xout, sizeof(T) * n));
+            }
+            if (!cap_stream) SSR_HIP_CHECK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+            SSR_HIP_CHECK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeRelaxed));
+            ssr_capture_stream_push(cap_stream);
+            const int rc = vcycle<T>(0, G.bin, G.xout);
+            ssr_capture_stream_pop();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(cap_stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            SSR_HIP_CHECK(e);
+            G.g = g;
+            SSR_HIP_CHECK(hipGraphInstantiate(&G.ge, G.g, nullptr, nullptr, 0));
+            G.key = (const void *)fine_vals<T>();
+            G.gen = generation;
+        }
+        SSR_HIP_CHECK(hipMemcpyAsync(G.bin, x, sizeof(T) * n, hipMemcpyDeviceToDevice, ssr_stream()));
+        SSR_HIP_CHECK(hipGraphLaunch(G.ge, ssr_stream()));
+        SSR_HIP_CHECK(hipMemcpyAsync(y, G.xout, sizeof(T) * n, hipMemcpyDeviceToDevice, ssr_stream()));
+        return SSR_OK;
+    }
+
     int apply(const double *x, double *y) override
     {
         if (!built) {
             ssr_set_error("AMG: apply before setup");
             return SSR_ERR_STATE;
         }
+        if (ssr_tuning(SSR_TUNE_AMG_GRAPH)) return apply_graph<double>(x, y);
         return vcycle<double>(0, x, y);
     }
 
@@ -821,9 +893,13 @@ struct Amg : ssr_precond {
             ssr_set_error("AMG: float apply before a mixed-precision setup");
             return SSR_ERR_STATE;
         }
+        if (ssr_tuning(SSR_TUNE_AMG_GRAPH)) return apply_graph<float>(x, y);
         return vcycle<float>(0, x, y);
     }
 };
+
+template <> Amg::VGraph<double> &Amg::vgraph<double>() { return vg64; }
+template <> Amg::VGraph<float> &Amg::vgraph<float>() { return vg32; }
 
 template <> const double *Amg::fine_vals<double>() const { return Kdev; }
 template <> const float *Amg::fine_vals<float>() const { return Kdev32; }

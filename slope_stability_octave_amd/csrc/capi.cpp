@@ -22,7 +22,20 @@ void ssr_set_error(const char *fmt, ...)
     va_end(ap);
 }
 
-hipStream_t ssr_stream(void) { return g_stream; }
+// A thread may divert the library's launches to another stream for the
+// duration of a graph capture (the legacy default stream cannot be captured).
+static thread_local hipStream_t t_capture = nullptr;
+static thread_local bool t_capturing = false;
+
+hipStream_t ssr_stream(void) { return t_capturing ? t_capture : g_stream; }
+
+void ssr_capture_stream_push(hipStream_t s)
+{
+    t_capture = s;
+    t_capturing = true;
+}
+
+void ssr_capture_stream_pop(void) { t_capturing = false; }
 
 int ssr_cu_count(void)
 {
@@ -264,7 +277,7 @@ extern "C" int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, 
     return ssr_dtoh(V, dV.p, sizeof(double) * nnz_out);
 }
 
-static int g_tuning[SSR_TUNE_COUNT] = {11, 8, 0, 0, 0, 0, 0, 0};
+static int g_tuning[SSR_TUNE_COUNT] = {11, 8, 0, 1, 0, 0, 0, 0};
 
 int ssr_tuning(int key) { return (key >= 0 && key < SSR_TUNE_COUNT) ? g_tuning[key] : 0; }
 

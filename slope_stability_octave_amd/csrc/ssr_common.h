@@ -40,6 +40,9 @@ static inline unsigned ssr_blocks(int64_t n, int threads)
 }
 
 hipStream_t ssr_stream(void);
+// divert ssr_stream() of the calling thread to `s` while capturing a graph
+void ssr_capture_stream_push(hipStream_t s);
+void ssr_capture_stream_pop(void);
 int ssr_cu_count(void);  // compute units of the current device (cached)
 
 // Preconditioner interface used by the Krylov solvers (solver.cpp): block
@@ -57,5 +60,6 @@ struct ssr_precond {
 #define SSR_TUNE_CONSTITUTIVE_VARIANT 0 /* 3D return-mapping kernel form, see constitutive.hip */
 #define SSR_TUNE_SPMV_GROUP 1           /* lanes per block row in the node-block SpMV: 4, 8, 16 */
 #define SSR_TUNE_PERSIST_WAVES 2        /* wavefronts per compute unit of persistent grids (0: auto) */
+#define SSR_TUNE_AMG_GRAPH 3            /* 1: replay the V-cycle as a captured HIP graph, 0: eager launches */
 #define SSR_TUNE_COUNT 8
 int ssr_tuning(int key);

@@ -136,6 +136,7 @@ class TrajectoryRecorder:
         self.mask = mask
         self.material = material
         self.X, self.G, self.g, self.t = [], [], [], []
+        self.raw = []  # last four (lambda, omega), target omega, new lambda
 
     def __call__(self, state, U_new, lam_new):
         Uh = state["U_hist"]
@@ -163,6 +164,7 @@ class TrajectoryRecorder:
         dl1 = lh[-1] - lh[-2]
         lam_sec = lh[-1] + a * dl1
         self.t.append((lam_new - lam_sec) / dl1 if dl1 != 0 else 0.0)
+        self.raw.append(np.r_[np.asarray(lh[-4:], float), np.asarray(oh[-4:], float), om_t, lam_new])
 
     def arrays(self):
         return (np.array(self.X), np.array(self.G), np.array(self.g), np.array(self.t))
@@ -182,14 +184,16 @@ class Normaliser:
 
 
 def train(net, data, epochs=2000, lr=2e-3, wd=1e-4, w_lam=0.5, log=None, val=None):
-    """Full-batch AdamW on the GPU.  data = (X, G, g, t) with X already
-    normalised.  Returns the loss history (and validation history)."""
-    X, G, g, t = data
+    """Full-batch AdamW on the GPU.  data = (X, G, g, t[, h]) with X already
+    normalised (h defaults to 1, the normalised secant error).  Returns the
+    loss history (and validation history)."""
+    X, G, g, t = data[:4]
+    h = data[4] if len(data) > 4 else np.ones(X.shape[0])
     B = X.shape[0]
     dX = DeviceArray.from_host(np.ascontiguousarray(X, dtype=np.float32))
     dG = DeviceArray.from_host(np.ascontiguousarray(G.reshape(B, -1), dtype=np.float32))
     dg = DeviceArray.from_host(np.ascontiguousarray(g, dtype=np.float32))
-    dh = DeviceArray.from_host(np.ones(B, dtype=np.float32))
+    dh = DeviceArray.from_host(np.ascontiguousarray(h, dtype=np.float32))
     dt = DeviceArray.from_host(np.ascontiguousarray(t, dtype=np.float32))
     hist, vhist = [], []
     loss = ctypes.c_float(0.0)
@@ -299,3 +303,49 @@ def recorder_factory(material):
         return TrajectoryRecorder(cmb.mask, material)
     make.is_factory = True
     return make
+
+
+# ---------------------------------------------------------------------------
+# end-to-end: trajectories -> training -> use
+# ---------------------------------------------------------------------------
+def generate(seeds, h=1.0, params=None, preconditioner="amg", log=None, linear_tol=1e-2):
+    """Run the plain solver on the seeded 2D slope family and collect the
+    training samples of every trajectory.  Returns (X, G, g, t) raw."""
+    from . import problems
+    from .continuation import ContinuationError
+    Xs, Gs, gs, ts = [], [], [], []
+    for seed in seeds:
+        prob, mat = problems.slope_2d_family(seed, h=h)
+        p = dict(problems.PARAMS_FAMILY)
+        p.update(params or {})
+        p.setdefault("linear_tol", linear_tol)
+        try:
+            res = problems.run_ssr(prob, p, preconditioner=preconditioner, recorder=recorder_factory(mat))
+        except ContinuationError as exc:
+            if log:
+                log(f"[mlp] seed {seed} skipped: {exc}")
+            continue
+        X, G, g, t = res["recorder"].arrays()
+        if log:
+            log(f"[mlp] seed {seed} material {tuple(round(x, 2) for x in mat)} FoS {res['FoS']:.5f} "
+                f"samples {len(X)} time {res['time_to_fos_s']:.2f}s")
+        if len(X):
+            Xs.append(X), Gs.append(G), gs.append(g), ts.append(t)
+    return (np.concatenate(Xs), np.concatenate(Gs), np.concatenate(gs), np.concatenate(ts))
+
+
+def fit(train_raw, val_raw=None, epochs=3000, lr=2e-3, wd=1e-4, w_lam=0.5, seed=0, log=None):
+    """Normalise the features, train a 3-layer, 128-unit network on the GPU;
+    returns (net, normaliser, report)."""
+    norm = Normaliser.fit(train_raw[0])
+    tr = (norm(train_raw[0]),) + tuple(train_raw[1:])
+    va = None if val_raw is None else (norm(val_raw[0]),) + tuple(val_raw[1:])
+    net = MLP(N_FEATURES, 128, N_OUT, 3, seed=seed)
+    t0 = time.perf_counter()
+    hist, vhist = train(net, tr, epochs=epochs, lr=lr, wd=wd, w_lam=w_lam, log=log, val=va)
+    rep = dict(train_time_s=time.perf_counter() - t0, train_loss_first=hist[0], train_loss_last=hist[-1],
+               train=evaluate(net, tr, w_lam), samples=int(tr[0].shape[0]))
+    if va is not None:
+        rep["val"] = evaluate(net, va, w_lam)
+        rep["val_samples"] = int(va[0].shape[0])
+    return net, norm, rep

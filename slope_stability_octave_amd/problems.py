@@ -125,6 +125,11 @@ def slope_3d_mesh(data, materials=None, mat_id=None):
     return _finish(coord, elem, Q, 3, mats, mid)
 
 
+# the 2D parameters with a lower starting lambda: weak, steep members of the
+# family have factors of safety below the reference drivers' 0.9
+PARAMS_FAMILY = dict(PARAMS_2D, lambda_init=0.5)
+
+
 def slope_2d_family(seed, h=1.0):
     """2D homogeneous slope of a seeded family: slope angle ~ U[26, 63] deg,
     c0 ~ U[5, 30] kPa, phi ~ U[20, 40] deg, psi = 0, E = 40 MPa, nu = 0.3,

@@ -184,3 +184,56 @@ def test_mixed_precision_refinement_meets_fp64_tolerance(precond, tol):
     # cond(K)*2*tol relative -- check a loose, tolerance-scaled agreement
     d = np.linalg.norm(xs[32] - xs[64]) / np.linalg.norm(xs[64])
     assert d < max(1e3 * tol, 1e-4), d
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_amg_graph_replays_match_eager(prec):
+    """The V-cycle replayed from its captured HIP graph equals the eager
+    launch sequence bit for bit, on every replay, with other device work and
+    a re-capture (new operator values through a new setup) in between."""
+    from slope_stability_octave_amd import _lib
+    prob = problems.slope_3d((2, 2, 2, 2, 2, 2), 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, problems.HOMO_2D)
+    cmb = problems.make_constitutive(prob)
+    cmb.reduction(1.0)
+    U = DeviceArray((cmb.n_n, cmb.dim), zero=True)
+    cmb.build_F_and_DS_all(1.0, U)
+    K = cmb.build_K_r_QQ_vals(1e-4)
+    rng = np.random.default_rng(3)
+    s = DFGMRES(cmb.pattern, 1e-8, 200, 1e-3, "amg", amg_options=dict(coord=prob["coord"]), precision=prec)
+    s.setup_preconditioner(K)
+    name = "ssr_solver_precond_apply"
+    assert prec in (32, 64)
+    xs = [rng.normal(size=(cmb.n_n, 3)) * cmb.Q.T for _ in range(4)]
+    old = _lib.get_tuning(_lib.TUNE_AMG_GRAPH)
+    try:
+        for rep in range(2):
+            outs = {}
+            for mode in (0, 1, 1, 0, 1):
+                _lib.set_tuning(_lib.TUNE_AMG_GRAPH, mode)
+                res = []
+                for x in xs:
+                    dx = DeviceArray.from_host(x)
+                    dy = DeviceArray((cmb.n_n, 3))
+                    _lib.call(name, s.handle, _lib.ptr(dx), _lib.ptr(dy))
+                    res.append(dy.get())
+                    spmv(cmb.pattern, K, dx, dy)  # unrelated eager work between replays
+                outs.setdefault(mode, []).append(res)
+            ref = outs[0][0]
+            for mode in (0, 1):
+                for res in outs[mode]:
+                    for a, b in zip(res, ref):
+                        assert np.array_equal(a, b)
+            # whole solves (float V-cycle when prec == 32): identical iterates
+            b = DeviceArray.from_host(xs[0])
+            sols = []
+            for mode in (0, 1, 1):
+                _lib.set_tuning(_lib.TUNE_AMG_GRAPH, mode)
+                x, it = s.solve(K, b)
+                sols.append((x.get(), it))
+            assert sols[0][1] == sols[1][1] == sols[2][1]
+            assert np.array_equal(sols[0][0], sols[1][0]) and np.array_equal(sols[0][0], sols[2][0])
+            # new operator values: a new setup must invalidate the graph
+            K = cmb.build_K_r_QQ_vals(0.3)
+            s.setup_preconditioner(K)
+    finally:
+        _lib.set_tuning(_lib.TUNE_AMG_GRAPH, old)
