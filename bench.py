@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--eval-seeds", type=int, default=4)
     ap.add_argument("--h-train", type=float, default=1.0)
     ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--no-amg-graph", action="store_true",
+                    help="launch the V-cycle kernel by kernel instead of replaying its captured graph")
     ap.add_argument("--autotune", action="store_true",
                     help="time the 3D kernel forms on this workload first and use the fastest")
     ap.add_argument("--hbm-probe", action="store_true",
@@ -232,6 +234,9 @@ def main():
     if args.variant is not None:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_CONSTITUTIVE_VARIANT, args.variant)
+    if args.no_amg_graph:
+        from slope_stability_octave_amd import _lib as L
+        L.set_tuning(L.TUNE_AMG_GRAPH, 0)
     if args.workload in ("fos2d", "fos3d", "fos2d_family"):
         return run_fos(args)
     if args.workload == "learned":

@@ -90,6 +90,11 @@ int ssr_dev_constitutive_2D(int64_t n_int, const double *E, const double *c_bar,
 int ssr_dev_reduction(int64_t n_int, int davis, const double *c0, const double *phi,
                       const double *psi, const double *lambda, int lambda_per_point,
                       int davis_c_all, double *c_bar, double *sin_phi);
+/* Psi per point of the Mohr-Coulomb model (potential_3D.m / potential_2D.m),
+ * E point-major with 6 (3D) or 3 (2D) rows; the integral WEIGHT*Psi' is
+ * CONSTITUTIVE.potential_energy (CONSTITUTIVE.m:363-387). */
+int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_bar, const double *sin_phi,
+                      const double *shear, const double *bulk, const double *lame, double *Psi);
 
 /* ---- tuning ------------------------------------------------------------- */
 /* key 0: 3D constitutive kernel variant (0 direct point-major stores,

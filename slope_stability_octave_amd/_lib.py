@@ -54,6 +54,7 @@ SIGNATURES = {
     "ssr_dev_constitutive_3D": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_constitutive_2D": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_reduction": [_i64, _int, _vp, _vp, _vp, _vp, _int, _int, _vp, _vp],
+    "ssr_dev_potential": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_set_tuning": [_int, _int],
     "ssr_get_tuning": [_int, _vp],
 }

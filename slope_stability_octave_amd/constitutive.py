@@ -112,6 +112,21 @@ class CONSTITUTIVE:
         self.constitutive_problem_stress_tangent(U)
         return self.build_F(F)
 
+    # -- potential (CONSTITUTIVE.m:363-387, potential_{2D,3D}.m) ---------------
+    def potential_energy(self, U):
+        """Integrated potential sum_p WEIGHT_p Psi(E_p(U)) at the current
+        reduction (the energy functional of the direct continuation)."""
+        t0 = time.perf_counter()
+        self._strain(U)
+        if not hasattr(self, "_psi_buf"):
+            self._psi_buf = DeviceArray(self.n_int)
+        CM.potential_dev(self.dim, self.n_int, self.E, self.c_bar, self.sin_phi, self.shear, self.bulk,
+                         self.lame, self._psi_buf)
+        from .linear_solvers import dot
+        val = dot(self.WEIGHT, self._psi_buf)
+        self._t("CONSTITUTIVE.potential", t0)
+        return val
+
     # -- K_r(Q,Q) (CONSTITUTIVE.m:393-532) ------------------------------------
     def pattern_or_raise(self):
         if self.pattern is None:

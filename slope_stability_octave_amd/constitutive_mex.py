@@ -97,3 +97,9 @@ def reduction_dev(n, davis, c0, phi, psi, lam, lam_per_point, c_bar, sin_phi, da
     """Davis reduction (reduction.m:25-58) on device arrays; writes c_bar, sin_phi."""
     call("ssr_dev_reduction", int(n), ord(davis), ptr(c0), ptr(phi), ptr(psi), ptr(lam),
          1 if lam_per_point else 0, 1 if davis_c_all else 0, ptr(c_bar), ptr(sin_phi))
+
+
+def potential_dev(dim, n, E, c_bar, sin_phi, shear, bulk, lame, Psi):
+    """Psi per point (potential_{2D,3D}.m), device arrays."""
+    call("ssr_dev_potential", int(dim), int(n), ptr(E), ptr(c_bar), ptr(sin_phi), ptr(shear), ptr(bulk), ptr(lame),
+         ptr(Psi))

@@ -189,3 +189,126 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
             break
     stats["total_wall_time"] = time.perf_counter() - t_total
     return U, np.array(lambda_hist), np.array(omega_hist), np.array(Umax_hist), stats
+
+
+# ---------------------------------------------------------------------------
+# Direct SSR continuation (lambda-stepping with omega = -dJ/dlambda)
+# ---------------------------------------------------------------------------
+def omega_SSR_direct_continuation(lambda_ini, U_ini, eps, d_lambda, it_newt_max, it_damp_max, tol, r_min, K_elast,
+                                  Q, f, cmb, solver, ws, log=None):
+    """omega_SSR_direct_continuation.m:49-81: U at lambda, and omega as the
+    backward difference of the energy J = Psi - f'U over eps.  Returns
+    (U, omega, flag, newton iterations)."""
+    mask = cmb.mask
+    cmb.reduction(lambda_ini)
+    U, flag, it1, _ = newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver, ws, log)
+    if flag == 1:
+        return U, 0.0, 1, it1
+    J = cmb.potential_energy(U) - dot(f, U, mask)
+    beta = min(1.0, eps / d_lambda)
+    U_beta = DeviceArray(U.shape)
+    lincomb(beta, U_ini, 1.0 - beta, U, U_beta)
+    cmb.reduction(lambda_ini - eps)
+    U_eps, flag, it2, _ = newton(U_beta, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver, ws, log)
+    if flag == 1:
+        return U, 0.0, 1, it1 + it2
+    J_eps = cmb.potential_energy(U_eps) - dot(f, U_eps, mask)
+    return U, (J_eps - J) / eps, 0, it1 + it2
+
+
+def init_phase_SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol,
+                                       eps, r_min, K_elast, Q, f, cmb, solver, ws, log=None):
+    """init_phase_SSR_direct_continuation.m:1-143."""
+    U0 = DeviceArray(f.shape, zero=True)
+    if log:
+        log(f"Init step 1: lambda={lambda_init:.6g}")
+    U1, omega1, flag, _ = omega_SSR_direct_continuation(lambda_init, U0, eps, 1000.0, it_newt_max, it_damp_max, tol,
+                                                        r_min, K_elast, Q, f, cmb, solver.copy(), ws, log)
+    if flag == 1:
+        raise ContinuationError("Initial choice of lambda seems to be too large.")
+    d_lambda = d_lambda_init
+    lambda1 = lambda_init
+    while True:
+        lambda_it = lambda1 + d_lambda
+        solver.expand_deflation_basis(U1)
+        U_it, omega_it, flag, _ = omega_SSR_direct_continuation(lambda_it, U1, eps, d_lambda, it_newt_max,
+                                                                it_damp_max, tol, r_min, K_elast, Q, f, cmb,
+                                                                solver.copy(), ws, log)
+        if flag == 1:
+            d_lambda = d_lambda / 2
+        else:
+            U2, omega2, lambda2 = U_it, omega_it, lambda_it
+            if (omega2 - omega1) / max(1.0, omega1) < 1e-5:
+                U1, lambda1, omega1 = U2, lambda2, omega2
+            else:
+                break
+        if d_lambda < d_lambda_min:
+            raise ContinuationError("It seems that FoS is equal to lambda_init.")
+        if lambda1 > 10:
+            raise ContinuationError("It seems that the FoS is greater than 10.")
+    if log:
+        log(f"   lambda1 = {lambda1:.6g}, lambda2 = {lambda2:.6g}, omega1 = {omega1:.6g}, omega2 = {omega2:.6g}")
+    return U1, U2, omega1, omega2, lambda1, lambda2
+
+
+def SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
+                            it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb, linear_system_solver, log=None):
+    """SSR_direct_continuation.m:47-146: lambda steps, halved on failure or a
+    decreasing omega, halved when omega grows by more than 1.5x; stops on the
+    scaled lambda increase, the minimal increment or the step limit.
+    Returns (U, lambda_hist, omega_hist, Umax_hist, work_hist)."""
+    if not cmb.pattern_initialized:
+        cmb.init_K_r_pattern(Q)
+    mask = cmb.mask
+    ws = Workspace(cmb.n_n, cmb.dim)
+    eps = tol * 100
+    U_old, U, omega_old, omega, lambda_old, lam = init_phase_SSR_direct_continuation(
+        lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol, eps, r_min, K_elast, Q, f, cmb,
+        linear_system_solver, ws, log)
+    linear_system_solver.expand_deflation_basis(U)
+    lambda_hist, omega_hist = [lambda_old, lam], [omega_old, omega]
+    Umax_hist = [_umax(U), _umax(U)]
+    work_hist = [dot(U_old, f), dot(U, f)]
+    d_omega = omega - omega_old
+    d_lambda = lam - lambda_init
+    step = 2
+    while True:
+        lambda_it = lam + d_lambda
+        if log:
+            log(f"Step {step + 1}: lambda_target={lambda_it:.6g}")
+        U_it, omega_it, flag, _ = omega_SSR_direct_continuation(lambda_it, U, eps, d_lambda, it_newt_max,
+                                                                it_damp_max, tol, r_min, K_elast, Q, f, cmb,
+                                                                linear_system_solver.copy(), ws, log)
+        d_omega_test = omega_it - omega_hist[-1]
+        if flag == 1 or d_omega_test < 0:
+            if log:
+                log(f"  retry: reducing d_lambda from {d_lambda:.6g} to {d_lambda / 2:.6g}")
+            d_lambda = d_lambda / 2
+        else:
+            U, omega, lam = U_it, omega_it, lambda_it
+            step += 1
+            lambda_hist.append(lam)
+            omega_hist.append(omega)
+            Umax_hist.append(_umax(U))
+            work_hist.append(dot(U, f))
+            linear_system_solver.expand_deflation_basis(U)
+            scaled = d_lambda / d_omega_test * (omega_hist[-1] - omega_hist[0])
+            if log:
+                log(f"  accepted: lambda={lam:.6g}, d_lambda={d_lambda:.6g}, omega={omega:.6g}, "
+                    f"d_omega={d_omega:.6g}, d_lambda_diff_scaled={scaled:.6g}, U_max={Umax_hist[-1]:.6g}")
+            if scaled < d_lambda_diff_scaled_min:
+                if log:
+                    log("Stop: minimal increase of lambda achieved.")
+                break
+            if d_omega_test > 1.5 * d_omega:
+                d_lambda = d_lambda / 2
+            d_omega = d_omega_test
+        if d_lambda < d_lambda_min:
+            if log:
+                log("Stop: minimal increment of lambda achieved.")
+            break
+        if step >= step_max:
+            if log:
+                log("Stop: maximal number of steps achieved.")
+            break
+    return U, np.array(lambda_hist), np.array(omega_hist), np.array(Umax_hist), np.array(work_hist)

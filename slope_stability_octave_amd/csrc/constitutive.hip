@@ -974,6 +974,118 @@ extern "C" int ssr_dev_constitutive_2D(int64_t n, const double *E, const double 
     return SSR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Potential Psi(E) of the Mohr-Coulomb model per point (potential_3D.m,
+// potential_2D.m): the elastic energy of the trial strain minus
+// denom * lambda^2 / 2 of the active return; its integral is the energy
+// functional of the direct SSR continuation.  Same branch tests as the
+// return mapping, evaluated without FP contraction.
+// ---------------------------------------------------------------------------
+// tI: trace entering the yield test and the multipliers, tr: trace entering
+// the energies (the 3D potential uses the invariant I1 for the former and the
+// sum of the eigenvalues for the latter; in 2D both are e1+e2+e3).
+__device__ __forceinline__ double psi_branches(double e1, double e2, double e3, double tI, double tr, double cb,
+                                               double sp, double G, double K, double lam, double den_s)
+{
+#pragma clang fp contract(off)
+    const double f_tr = 2.0 * G * ((1.0 + sp) * e1 - (1.0 - sp) * e3) + 2.0 * (lam * sp) * tI - cb;
+    const double el = 0.5 * lam * tr * tr + G * (e1 * e1 + e2 * e2 + e3 * e3);
+    if (f_tr <= 0.0) return el;
+    const double g_sl = (e1 - e2) / (1.0 + sp);
+    const double g_sr = (e2 - e3) / (1.0 - sp);
+    const double g_la = (e1 + e2 - 2.0 * e3) / (3.0 - sp);
+    const double g_ra = (2.0 * e1 - e2 - e3) / (3.0 + sp);
+    const double lam_s = f_tr / den_s;
+    if (lam_s <= (g_sl < g_sr ? g_sl : g_sr)) return el - 0.5 * den_s * (lam_s * lam_s);
+    const double den_l = 4.0 * lam * sp * sp + G * (1.0 + sp) * (1.0 + sp) + 2.0 * G * (1.0 - sp) * (1.0 - sp);
+    const double lam_l = (G * ((1.0 + sp) * (e1 + e2) - 2.0 * (1.0 - sp) * e3) + 2.0 * lam * sp * tI - cb) / den_l;
+    if (g_sl < g_sr && lam_l >= g_sl && lam_l <= g_la)
+        return 0.5 * lam * tr * tr + G * (e3 * e3 + 0.5 * (e1 + e2) * (e1 + e2)) - 0.5 * den_l * (lam_l * lam_l);
+    const double den_r = 4.0 * lam * sp * sp + 2.0 * G * (1.0 + sp) * (1.0 + sp) + G * (1.0 - sp) * (1.0 - sp);
+    const double lam_r = (G * (2.0 * (1.0 + sp) * e1 - (1.0 - sp) * (e2 + e3)) + 2.0 * lam * sp * tI - cb) / den_r;
+    if (g_sl > g_sr && lam_r >= g_sr && lam_r <= g_ra)
+        return 0.5 * lam * tr * tr + G * (e1 * e1 + 0.5 * (e2 + e3) * (e2 + e3)) - 0.5 * den_r * (lam_r * lam_r);
+    const double den_a = 4.0 * K * sp * sp;
+    const double lam_a = (2.0 * K * sp * tI - cb) / den_a;
+    return 0.5 * K * tr * tr - 0.5 * den_a * (lam_a * lam_a);
+}
+
+__global__ __launch_bounds__(256) void k_potential3d(int64_t n, const double *__restrict__ E,
+                                                     const double *__restrict__ cbar, const double *__restrict__ sphi,
+                                                     const double *__restrict__ shear, const double *__restrict__ bulk,
+                                                     const double *__restrict__ lame, double *__restrict__ psi)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double *e = E + 6 * p;
+    double ps;
+    {
+#pragma clang fp contract(off)
+        const double t0 = e[0], t1 = e[1], t2 = e[2], t3 = 0.5 * e[3], t4 = 0.5 * e[4], t5 = 0.5 * e[5];
+        const double I1 = t0 + t1 + t2;
+        const double I2 = t0 * t1 + t0 * t2 + t1 * t2 - t3 * t3 - t4 * t4 - t5 * t5;
+        const double I3 = t0 * t1 * t2 - t2 * t3 * t3 - t1 * t5 * t5 - t0 * t4 * t4 + 2.0 * t3 * t4 * t5;
+        double Q = (1.0 / 9.0) * (I1 * I1 - 3.0 * I2);
+        Q = Q > 0.0 ? Q : 0.0;
+        const double R = (1.0 / 54.0) * (-2.0 * (I1 * I1 * I1) + 9.0 * (I1 * I2) - 27.0 * I3);
+        double th0 = 0.0;
+        if (Q > 0.0) th0 = R / sqrt(Q * Q * Q);
+        th0 = th0 < -1.0 ? -1.0 : (th0 > 1.0 ? 1.0 : th0);
+        const double th = acos(th0) / 3.0;
+        const double rq = sqrt(Q);
+        const double e1 = -2.0 * rq * cos(th + kTwoPiOver3) + I1 / 3.0;
+        const double e2 = -2.0 * rq * cos(th - kTwoPiOver3) + I1 / 3.0;
+        const double e3 = -2.0 * rq * cos(th) + I1 / 3.0;
+        const double sp = sphi[p], G = shear[p], lam = lame[p];
+        const double den_s = 4.0 * lam * sp * sp + 4.0 * G * (1.0 + sp * sp);
+        ps = psi_branches(e1, e2, e3, I1, e1 + e2 + e3, cbar[p], sp, G, bulk[p], lam, den_s);
+    }
+    psi[p] = ps;
+}
+
+__global__ __launch_bounds__(256) void k_potential2d(int64_t n, const double *__restrict__ E,
+                                                     const double *__restrict__ cbar, const double *__restrict__ sphi,
+                                                     const double *__restrict__ shear, const double *__restrict__ bulk,
+                                                     const double *__restrict__ lame, double *__restrict__ psi)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double *e = E + 3 * p;
+    double ps;
+    {
+#pragma clang fp contract(off)
+        const double I1 = e[0] + e[1];
+        const double I2 = sqrt((e[0] - e[1]) * (e[0] - e[1]) + e[2] * e[2]);
+        const double a1 = (I1 + I2) / 2.0, a2 = (I1 - I2) / 2.0, a3 = 0.0;
+        double e1 = a1, e2 = a2, e3 = a3;
+        if ((a1 >= a3) && (a3 > a2)) { e2 = a3; e3 = a2; }
+        if (a3 > a1) { e1 = a3; e2 = a1; e3 = a2; }
+        const double tr = e1 + e2 + e3;
+        const double sp = sphi[p], G = shear[p], lam = lame[p];
+        const double den_s = 4.0 * lam * sp * sp + 2.0 * G * (1.0 + sp) * (1.0 + sp) + 2.0 * G * (1.0 - sp) * (1.0 - sp);
+        ps = psi_branches(e1, e2, e3, tr, tr, cbar[p], sp, G, bulk[p], lam, den_s);
+    }
+    psi[p] = ps;
+}
+
+// Psi per point (potential_{2D,3D}.m); E point-major with 3 (2D) or 6 (3D) rows.
+extern "C" int ssr_dev_potential(int dim, int64_t n, const double *E, const double *c_bar, const double *sin_phi,
+                                 const double *shear, const double *bulk, const double *lame, double *Psi)
+{
+    SSR_REQUIRE(dim == 2 || dim == 3, "potential: dim must be 2 or 3");
+    SSR_REQUIRE(n >= 0, "potential: n_int must be >= 0");
+    if (n == 0) return SSR_OK;
+    SSR_REQUIRE(E && c_bar && sin_phi && shear && bulk && lame && Psi, "potential: null argument");
+    if (dim == 3)
+        hipLaunchKernelGGL(k_potential3d, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, E, c_bar, sin_phi,
+                           shear, bulk, lame, Psi);
+    else
+        hipLaunchKernelGGL(k_potential2d, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), n, E, c_bar, sin_phi,
+                           shear, bulk, lame, Psi);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
 extern "C" int ssr_dev_reduction(int64_t n, int davis, const double *c0, const double *phi,
                                  const double *psi, const double *lambda, int lambda_per_point,
                                  int davis_c_all, double *c_bar, double *sin_phi)

@@ -13,13 +13,24 @@ the prediction is used only when ||F_lambda(U) - f||_Q at (U_pred,
 lambda_pred) is below the residual of the secant start (U_sec, lambda_k);
 otherwise the secant step runs unchanged.
 
-Training sample of step k (four converged fields before it): features of the
-(lambda, omega) history and the material, and the quadratic form of the
-normalised prediction error,
-    L(delta) = ||sum_i (c_sec + delta)_i U_{k-i} - U_{k+1}||^2 / s,
-    s = ||U_sec - U_{k+1}||^2,
-i.e. L = delta' G delta - 2 delta' g + 1 with G = F'F/s, g = F'(U_{k+1} -
-U_sec)/s (F = [U_k .. U_{k-3}]): L = 1 is the secant's error, L < 1 beats it.
+Output coordinates.  The four fields F = [U_k .. U_{k-3}] are nearly
+collinear (the Gram matrix M = F'F has condition numbers around 1e8), so raw
+coefficients are a hopeless regression target.  The network instead outputs
+w in the orthonormal (Gram-Schmidt) basis of the span, scaled by the last
+step length: with M = R'R (Cholesky) and |D1| = |U_k - U_{k-1}|,
+    delta = R^-1 w |D1|,   F delta = Q w |D1|  (Q orthonormal),
+so an error in w maps isometrically to the displacement error in units of
+the last step.  M is known at run time (ten inner products of the fields).
+
+Training sample of step k: features of the (lambda, omega) history, of the
+field geometry (step-length ratios and angles between successive steps, from
+M) and of the material; target w* = R delta* / |D1| with delta* the least
+squares correction of the secant over the span; and the weight
+rho = |D1|^2 / s, s = |U_sec - U_{k+1}|^2.  The normalised error of a
+prediction is then
+    L(w) = rho |w - w*|^2 + L_opt,   L(0) = 1 (the secant),
+L_opt the best the span allows; the kernels minimise the mean of
+rho |w - w*|^2 + w_lam (t - t*)^2.
 """
 from __future__ import annotations
 
@@ -47,8 +58,10 @@ extend_signatures({
 })
 
 N_FIELDS = 4
-N_FEATURES = 12
+N_FEATURES = 17
 N_OUT = N_FIELDS + 1
+_D1 = np.array([1.0, -1.0, 0.0, 0.0])
+_DIFF = np.array([[1.0, -1.0, 0.0, 0.0], [0.0, 1.0, -1.0, 0.0], [0.0, 0.0, 1.0, -1.0]])
 
 
 class MLP:
@@ -110,9 +123,19 @@ def secant_coefficients(omega_hist, omega_target):
     return np.array([1.0 + a, -a, 0.0, 0.0]), a
 
 
-def features(lambda_hist, omega_hist, omega_target, material):
-    """12 scale-free features of the last four (lambda, omega) pairs, the
-    target omega and the material (slope angle deg, c0 kPa, phi deg)."""
+def gram_features(M):
+    """Step-length ratios and cosines between the last three steps of the
+    fields (from their Gram matrix)."""
+    MD = _DIFF @ M @ _DIFF.T
+    ln = np.sqrt(np.maximum(np.diag(MD), 1e-300))
+    return [_slog(ln[1] / ln[0]), _slog(ln[2] / ln[0]), MD[0, 1] / (ln[0] * ln[1]), MD[1, 2] / (ln[1] * ln[2]),
+            MD[0, 2] / (ln[0] * ln[2])]
+
+
+def features(lambda_hist, omega_hist, omega_target, material, M):
+    """17 scale-free features: the last four (lambda, omega) pairs and the
+    target omega (10), the geometry of the last three field steps (5), the
+    material (slope angle deg, c0 kPa, phi deg)."""
     lam = np.asarray(lambda_hist[-4:], dtype=np.float64)
     om = np.asarray(omega_hist[-4:], dtype=np.float64)
     d_om = omega_target - om[-1]
@@ -123,9 +146,30 @@ def features(lambda_hist, omega_hist, omega_target, material):
     s = dl / dw
     f += [_slog(s[i] / s[0]) if s[0] != 0 else 0.0 for i in (1, 2)]
     f += [lam[-1], math.log(om[-1] / om[0])]
+    f += gram_features(M)
     beta, c0, phi = material
     f += [beta / 90.0, c0 / 30.0, phi / 45.0]
     return np.array(f, dtype=np.float64)
+
+
+def field_gram(F, mask):
+    M = np.zeros((N_FIELDS, N_FIELDS))
+    for i in range(N_FIELDS):
+        for j in range(i, N_FIELDS):
+            M[i, j] = M[j, i] = dot(F[i], F[j], mask)
+    return M
+
+
+def whiten(M):
+    """(R, |D1|) with M = R'R; None if the fields are rank deficient."""
+    try:
+        R = np.linalg.cholesky(M).T
+    except np.linalg.LinAlgError:
+        return None
+    d1 = float(_D1 @ M @ _D1)
+    if not (d1 > 0) or not np.all(np.isfinite(R)):
+        return None
+    return R, math.sqrt(d1)
 
 
 class TrajectoryRecorder:
@@ -135,7 +179,8 @@ class TrajectoryRecorder:
     def __init__(self, mask, material):
         self.mask = mask
         self.material = material
-        self.X, self.G, self.g, self.t = [], [], [], []
+        self.X, self.G, self.g, self.h, self.t = [], [], [], [], []
+        self.L_opt = []  # best normalised error within the span (secant = 1)
         self.raw = []  # last four (lambda, omega), target omega, new lambda
 
     def __call__(self, state, U_new, lam_new):
@@ -147,27 +192,29 @@ class TrajectoryRecorder:
         c, a = secant_coefficients(oh, om_t)
         F = Uh[:N_FIELDS]
         m = self.mask
-        # Gram data on the device
-        FF = np.zeros((N_FIELDS, N_FIELDS))
-        for i in range(N_FIELDS):
-            for j in range(i, N_FIELDS):
-                FF[i, j] = FF[j, i] = dot(F[i], F[j], m)
+        M = field_gram(F, m)
         Fu = np.array([dot(F[i], U_new, m) for i in range(N_FIELDS)])
         uu = dot(U_new, U_new, m)
-        # s = ||U_sec - U*||^2 = c'FFc - 2 c'Fu + uu
-        s = float(c @ FF @ c - 2 * c @ Fu + uu)
-        if not (s > 0) or not math.isfinite(s):
+        s = float(c @ M @ c - 2 * c @ Fu + uu)  # |U_sec - U*|^2
+        wh = whiten(M)
+        if wh is None or not (s > 0) or not math.isfinite(s):
             return
-        self.X.append(features(lh, oh, om_t, self.material))
-        self.G.append(FF / s)
-        self.g.append((Fu - FF @ c) / s)
+        R, d1 = wh
+        delta = np.linalg.solve(M, Fu - M @ c)  # least-squares correction of the secant
+        w = R @ delta / d1
+        rho = d1 * d1 / s
+        self.X.append(features(lh, oh, om_t, self.material, M))
+        self.G.append(rho * np.eye(N_FIELDS))
+        self.g.append(rho * w)
+        self.h.append(rho * float(w @ w))
+        self.L_opt.append(1.0 - rho * float(w @ w))
         dl1 = lh[-1] - lh[-2]
         lam_sec = lh[-1] + a * dl1
         self.t.append((lam_new - lam_sec) / dl1 if dl1 != 0 else 0.0)
         self.raw.append(np.r_[np.asarray(lh[-4:], float), np.asarray(oh[-4:], float), om_t, lam_new])
 
     def arrays(self):
-        return (np.array(self.X), np.array(self.G), np.array(self.g), np.array(self.t))
+        return (np.array(self.X), np.array(self.G), np.array(self.g), np.array(self.t), np.array(self.h))
 
 
 class Normaliser:
@@ -197,20 +244,28 @@ def train(net, data, epochs=2000, lr=2e-3, wd=1e-4, w_lam=0.5, log=None, val=Non
     dt = DeviceArray.from_host(np.ascontiguousarray(t, dtype=np.float32))
     hist, vhist = [], []
     loss = ctypes.c_float(0.0)
+    best = (float("inf"), None)
     for ep in range(epochs):
         call("ssr_mlp_train_step", net.handle, B, ptr(dX), ptr(dG), ptr(dg), ptr(dh), ptr(dt), float(w_lam),
              float(lr), float(wd), ctypes.byref(loss))
         hist.append(loss.value)
-        if val is not None and (ep % 50 == 0 or ep == epochs - 1):
-            vhist.append((ep, evaluate(net, val, w_lam)))
+        if val is not None and (ep % 25 == 0 or ep == epochs - 1):
+            ev = evaluate(net, val, w_lam)
+            vhist.append((ep, ev))
+            score = ev["L_U"] + w_lam * ev["L_lambda"]
+            if score < best[0]:
+                best = (score, net.get_params())
         if log and (ep % 200 == 0 or ep == epochs - 1):
             log(f"[mlp] epoch {ep} loss {loss.value:.4f}" + (f" val {vhist[-1][1]}" if vhist else ""))
+    if best[1] is not None:  # early stopping on the validation set
+        net.set_params(best[1])
     return hist, vhist
 
 
 def evaluate(net, data, w_lam=0.5):
-    """Mean (L_U, L_lambda) of the network on data, and of the secant (1, t^2)."""
-    X, G, g, t = data
+    """Mean normalised displacement error L_U (secant = 1) and lambda error
+    of the network on data, with the secant's lambda error for reference."""
+    X, G, g, t = data[:4]
     Y = net.forward(X).astype(np.float64)
     B = X.shape[0]
     d = Y[:, :N_FIELDS]
@@ -250,9 +305,15 @@ class LearnedPredictor:
             return None
         om_t = state["omega_it"]
         c, a = secant_coefficients(oh, om_t)
-        x = self.norm(features(lh, oh, om_t, self.material))
+        M = field_gram(Uh[:N_FIELDS], self.cmb.mask)
+        wh = whiten(M)
+        if wh is None:
+            self.rejected += 1
+            return None
+        R, d1 = wh
+        x = self.norm(features(lh, oh, om_t, self.material, M))
         y = self.net.forward(x)[0].astype(np.float64)
-        coef = c + y[:N_FIELDS]
+        coef = c + np.linalg.solve(R, y[:N_FIELDS]) * d1
         dl1 = lh[-1] - lh[-2]
         lam_pred = lh[-1] + a * dl1 + y[N_FIELDS] * dl1
         U = DeviceArray(Uh[0].shape)
@@ -310,10 +371,10 @@ def recorder_factory(material):
 # ---------------------------------------------------------------------------
 def generate(seeds, h=1.0, params=None, preconditioner="amg", log=None, linear_tol=1e-2):
     """Run the plain solver on the seeded 2D slope family and collect the
-    training samples of every trajectory.  Returns (X, G, g, t) raw."""
+    training samples of every trajectory.  Returns (X, G, g, t, h) raw."""
     from . import problems
     from .continuation import ContinuationError
-    Xs, Gs, gs, ts = [], [], [], []
+    Xs, Gs, gs, ts, hs = [], [], [], [], []
     for seed in seeds:
         prob, mat = problems.slope_2d_family(seed, h=h)
         p = dict(problems.PARAMS_FAMILY)
@@ -325,16 +386,17 @@ def generate(seeds, h=1.0, params=None, preconditioner="amg", log=None, linear_t
             if log:
                 log(f"[mlp] seed {seed} skipped: {exc}")
             continue
-        X, G, g, t = res["recorder"].arrays()
+        X, G, g, t, hh = res["recorder"].arrays()
         if log:
+            lo = np.mean(res["recorder"].L_opt) if len(X) else float("nan")
             log(f"[mlp] seed {seed} material {tuple(round(x, 2) for x in mat)} FoS {res['FoS']:.5f} "
-                f"samples {len(X)} time {res['time_to_fos_s']:.2f}s")
+                f"samples {len(X)} span optimum L {lo:.3f} time {res['time_to_fos_s']:.2f}s")
         if len(X):
-            Xs.append(X), Gs.append(G), gs.append(g), ts.append(t)
-    return (np.concatenate(Xs), np.concatenate(Gs), np.concatenate(gs), np.concatenate(ts))
+            Xs.append(X), Gs.append(G), gs.append(g), ts.append(t), hs.append(hh)
+    return (np.concatenate(Xs), np.concatenate(Gs), np.concatenate(gs), np.concatenate(ts), np.concatenate(hs))
 
 
-def fit(train_raw, val_raw=None, epochs=3000, lr=2e-3, wd=1e-4, w_lam=0.5, seed=0, log=None):
+def fit(train_raw, val_raw=None, epochs=3000, lr=1e-3, wd=1e-3, w_lam=0.5, seed=0, log=None):
     """Normalise the features, train a 3-layer, 128-unit network on the GPU;
     returns (net, normaliser, report)."""
     norm = Normaliser.fit(train_raw[0])

@@ -143,3 +143,30 @@ def slope_2d_family(seed, h=1.0):
     x2 = y2 / np.tan(np.deg2rad(beta))
     mats = [dict(c0=c0, phi=phi, psi=0.0, young=40000.0, poisson=0.3, gamma_sat=20.0, gamma_unsat=20.0)]
     return slope_2d(h=h, x1=15.0, x2=x2, x3=15.0, y1=10.0, y2=y2, materials=mats), (beta, c0, phi)
+
+
+def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_options=None, precision=64):
+    """Direct SSR continuation (SSR_direct_continuation.m) on the GPU: the
+    second factor-of-safety estimate of the reference drivers."""
+    from .continuation import SSR_direct_continuation
+    from .linear_solvers import DFGMRES
+    p = dict(PARAMS_2D if prob["dim"] == 2 else PARAMS_3D)
+    p.update(params or {})
+    t0 = time.perf_counter()
+    cmb = make_constitutive(prob)
+    if preconditioner == "amg":
+        amg_options = dict(amg_options or {})
+        amg_options.setdefault("coord", prob["coord"])
+    lin = DFGMRES(cmb.pattern, p["linear_tol"], p["linear_maxit"], p["deflation_tol"], preconditioner,
+                  amg_options=amg_options, precision=precision)
+    f = DeviceArray.from_host(prob["f"].T)
+    t_setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    U, lam_h, om_h, umax_h, work_h = SSR_direct_continuation(
+        p["lambda_init"], p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
+        p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], None, prob["Q"], f, cmb, lin, log=log)
+    t_run = time.perf_counter() - t1
+    timings = dict(cmb.timings)
+    timings.update(lin.timings)
+    return dict(FoS=float(lam_h[-1]), lambda_hist=lam_h, omega_hist=om_h, Umax_hist=umax_h, work_hist=work_h, U=U,
+                setup_s=t_setup, time_to_fos_s=t_run, timings=timings, counts=dict(cmb.counts))
