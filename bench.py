@@ -190,6 +190,36 @@ def run_learned(args):
     print(json.dumps(out))
 
 
+def run_batch(args):
+    """Many independent 2D slope instances of uneven size (seeded family,
+    h drawn from {1, 0.75, 0.5}) solved concurrently on per-thread streams,
+    against the same instances one after another."""
+    from slope_stability_octave_amd import _lib, problems
+
+    _lib.require_gpu()
+    rng = np.random.default_rng(7)
+    hs = rng.choice([1.0, 0.75, 0.5], size=args.batch)
+    probs = [problems.slope_2d_family(3000 + i, h=float(hs[i]))[0] for i in range(args.batch)]
+    p = dict(problems.PARAMS_FAMILY, linear_tol=args.linear_tol or 1e-2)
+    seq, t_seq = problems.run_batch(probs, workers=1, params=p, precision=args.precision)
+    par, t_par = problems.run_batch(probs, workers=args.workers, params=p, precision=args.precision)
+    dfos = max(abs(a["FoS"] - b["FoS"]) for a, b in zip(seq, par))
+    out = {
+        "metric": "SSR instances per second, concurrent per-thread streams vs one after another",
+        "value": args.batch / t_par,
+        "unit": "instances/s",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "f64" if args.precision == 64 else "f64 (Newton) / f32 (Krylov, AMG)",
+        "data": "synthetic",
+        "config": {"workload": f"{args.batch} 2D slope-family instances, h in {{1, 0.75, 0.5}}",
+                   "free_dof": [int(pr["n_dof_free"]) for pr in probs], "workers": args.workers},
+        "wall_sequential_s": t_seq, "wall_concurrent_s": t_par, "speedup": t_seq / t_par,
+        "max_dFoS_concurrent_vs_sequential": dfos,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,7 +227,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos2d_family", "learned"])
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos2d_family", "learned", "batch"])
     ap.add_argument("--h", type=float, default=0.5)
     ap.add_argument("--lambda-init", type=float, default=1.0)
     ap.add_argument("--omega-max", type=float, default=7e7)
@@ -215,6 +245,8 @@ def main():
     ap.add_argument("--amg-cheb-degree", type=int, default=None, help="AMG Chebyshev smoothing degree")
     ap.add_argument("--amg-max-coarse", type=int, default=None, help="AMG coarsest-level size")
     ap.add_argument("--seed", type=int, default=2000, help="member of the 2D slope family (fos2d_family)")
+    ap.add_argument("--batch", type=int, default=16, help="instances of the batch workload")
+    ap.add_argument("--workers", type=int, default=8, help="host threads (streams) of the batch workload")
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--eval-seeds", type=int, default=4)
     ap.add_argument("--h-train", type=float, default=1.0)
@@ -241,6 +273,8 @@ def main():
         return run_fos(args)
     if args.workload == "learned":
         return run_learned(args)
+    if args.workload == "batch":
+        return run_batch(args)
 
     from slope_stability_octave_amd import _lib
     from slope_stability_octave_amd import constitutive_mex as cm
@@ -310,9 +344,14 @@ def main():
         src = dD.view(0, (nb,))
         dst = DeviceArray(nb)
         res = {}
+        res_dev = DeviceArray(1)
+
+        def read_probe():
+            _lib.call("ssr_dev_dot", nb, _lib.ptr(src), _lib.ptr(src), _lib.ptr(None), _lib.ptr(res_dev))
+
         for name, fn, nbytes in (("write", lambda: LS.fill(dst, 1.0), 8 * nb),
                                  ("copy", lambda: LS.axpby(1.0, src, 0.0, dst), 16 * nb),
-                                 ("read", lambda: LS.dot(src, src), 8 * nb)):
+                                 ("read", read_probe, 8 * nb)):
             fn()
             _lib.synchronize()
             tim.start()

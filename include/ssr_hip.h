@@ -32,6 +32,13 @@ int ssr_version(void);
 int ssr_device_count(int *count);
 int ssr_set_stream(void *hip_stream); /* NULL = default stream */
 void *ssr_get_stream(void);
+/* Per host thread: launch this thread's work on its own stream (NULL: back to
+ * the library stream).  Independent problems driven from several threads then
+ * run concurrently on the device (no reference counterpart: the reference is
+ * single-threaded Octave). */
+int ssr_set_thread_stream(void *hip_stream);
+int ssr_stream_create(void **out); /* default priority, non-blocking */
+int ssr_stream_destroy(void *hip_stream);
 int ssr_dmalloc(void **ptr, size_t bytes);
 int ssr_dfree(void *ptr);
 int ssr_htod(void *dst, const void *src, size_t bytes); /* synchronous */

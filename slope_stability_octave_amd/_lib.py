@@ -36,6 +36,9 @@ SIGNATURES = {
     "ssr_device_count": [_vp],
     "ssr_set_stream": [_vp],
     "ssr_get_stream": [],
+    "ssr_set_thread_stream": [_vp],
+    "ssr_stream_create": [_vp],
+    "ssr_stream_destroy": [_vp],
     "ssr_dmalloc": [_vp, _sz],
     "ssr_dfree": [_vp],
     "ssr_htod": [_vp, _vp, _sz],
@@ -236,3 +239,23 @@ class EventTimer:
             _lib.ssr_event_destroy(self._b)
         except Exception:
             pass
+
+
+class ThreadStream:
+    """Context manager: the calling thread's library work runs on a stream of
+    its own (created on entry, synchronised and destroyed on exit)."""
+
+    def __enter__(self):
+        p = ctypes.c_void_p(0)
+        call("ssr_stream_create", ctypes.byref(p))
+        self.stream = p
+        call("ssr_set_thread_stream", p)
+        return self
+
+    def __exit__(self, *exc):
+        try:
+            synchronize()
+        finally:
+            call("ssr_set_thread_stream", ctypes.c_void_p(0))
+            call("ssr_stream_destroy", self.stream)
+        return False

@@ -183,7 +183,10 @@ int dupload(T **d, const std::vector<T> &h)
     (void)hipFree(*d);
     *d = nullptr;
     SSR_HIP_CHECK(hipMalloc((void **)d, sizeof(T) * (h.empty() ? 1 : h.size())));
-    if (!h.empty()) SSR_HIP_CHECK(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    if (!h.empty()) {
+        SSR_HIP_CHECK(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, ssr_stream()));
+        SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+    }
     return SSR_OK;
 }
 
@@ -736,7 +739,9 @@ struct Amg : ssr_precond {
         if (rebuild || refresh_now) {
             clk.lap(11);
             Kh.resize((size_t)nnzb * dim * dim);
-            SSR_HIP_CHECK(hipMemcpy(Kh.data(), Kvals, sizeof(double) * Kh.size(), hipMemcpyDeviceToHost));
+            SSR_HIP_CHECK(hipMemcpyAsync(Kh.data(), Kvals, sizeof(double) * Kh.size(), hipMemcpyDeviceToHost,
+                                         ssr_stream()));
+            SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
             clk.lap(0);
             rc = rebuild ? build() : refresh();
         } else if (need32) {

@@ -27,7 +27,12 @@ void ssr_set_error(const char *fmt, ...)
 static thread_local hipStream_t t_capture = nullptr;
 static thread_local bool t_capturing = false;
 
-hipStream_t ssr_stream(void) { return t_capturing ? t_capture : g_stream; }
+// A thread may also run on its own stream (concurrent independent problems,
+// one host thread each): ssr_set_thread_stream.
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local bool t_has_stream = false;
+
+hipStream_t ssr_stream(void) { return t_capturing ? t_capture : (t_has_stream ? t_stream : g_stream); }
 
 void ssr_capture_stream_push(hipStream_t s)
 {
@@ -71,7 +76,34 @@ int ssr_set_stream(void *stream)
     return SSR_OK;
 }
 
-void *ssr_get_stream(void) { return (void *)g_stream; }
+void *ssr_get_stream(void) { return (void *)ssr_stream(); }
+
+// Launch everything this host thread issues on `stream` (NULL: back to the
+// library stream).  Each thread's problems then proceed concurrently with
+// other threads' on the device.
+int ssr_set_thread_stream(void *stream)
+{
+    t_stream = (hipStream_t)stream;
+    t_has_stream = stream != nullptr;
+    return SSR_OK;
+}
+
+// A new stream at the default priority that does not synchronise with the
+// legacy default stream.
+int ssr_stream_create(void **out)
+{
+    SSR_REQUIRE(out, "stream_create: null out pointer");
+    hipStream_t st = nullptr;
+    SSR_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *out = (void *)st;
+    return SSR_OK;
+}
+
+int ssr_stream_destroy(void *stream)
+{
+    if (stream) SSR_HIP_CHECK(hipStreamDestroy((hipStream_t)stream));
+    return SSR_OK;
+}
 
 int ssr_dmalloc(void **ptr, size_t bytes)
 {
@@ -94,33 +126,33 @@ int ssr_dfree(void *ptr)
 
 int ssr_htod(void *dst, const void *src, size_t bytes)
 {
-    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
-    SSR_HIP_CHECK(hipStreamSynchronize(g_stream));
+    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ssr_stream()));
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     return SSR_OK;
 }
 
 int ssr_dtoh(void *dst, const void *src, size_t bytes)
 {
-    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
-    SSR_HIP_CHECK(hipStreamSynchronize(g_stream));
+    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ssr_stream()));
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     return SSR_OK;
 }
 
 int ssr_dtod(void *dst, const void *src, size_t bytes)
 {
-    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    if (bytes) SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ssr_stream()));
     return SSR_OK;
 }
 
 int ssr_dzero(void *dst, size_t bytes)
 {
-    if (bytes) SSR_HIP_CHECK(hipMemsetAsync(dst, 0, bytes, g_stream));
+    if (bytes) SSR_HIP_CHECK(hipMemsetAsync(dst, 0, bytes, ssr_stream()));
     return SSR_OK;
 }
 
 int ssr_synchronize(void)
 {
-    SSR_HIP_CHECK(hipStreamSynchronize(g_stream));
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     return SSR_OK;
 }
 
@@ -140,7 +172,7 @@ int ssr_event_destroy(void *ev)
 
 int ssr_event_record(void *ev)
 {
-    SSR_HIP_CHECK(hipEventRecord((hipEvent_t)ev, g_stream));
+    SSR_HIP_CHECK(hipEventRecord((hipEvent_t)ev, ssr_stream()));
     return SSR_OK;
 }
 

@@ -9,6 +9,9 @@
 // 64-lane wavefront reduction by cross-lane shuffles then LDS across waves,
 // always accumulated in double.  Results of reductions stay on the device so
 // that a caller can chain them without host round trips.
+#include <mutex>
+#include <unordered_map>
+
 #include "linalg.h"
 #include "ssr_common.h"
 
@@ -309,13 +312,23 @@ __global__ void k_ewmul(int64_t n, const double *__restrict__ a, const double *_
     if (i < n) z[i] = a[i] * b[i];
 }
 
-double *g_part = nullptr;  // kRedBlocks * 128 partial sums
-double *g_red = nullptr;   // 128 reduced values
+// Reduction scratch per stream, so that threads running independent
+// problems on their own streams never share partial sums.
+struct RedScratch {
+    double *part = nullptr;  // kRedBlocks * 128 partial sums
+    double *red = nullptr;   // 128 reduced values
+};
+std::mutex g_scr_mu;
+std::unordered_map<hipStream_t, RedScratch> g_scr;
 
-int ensure_scratch()
+int scratch(RedScratch **out)
 {
-    if (!g_part) SSR_HIP_CHECK(hipMalloc((void **)&g_part, sizeof(double) * kRedBlocks * 128));
-    if (!g_red) SSR_HIP_CHECK(hipMalloc((void **)&g_red, sizeof(double) * 128));
+    const hipStream_t st = ssr_stream();
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    RedScratch &r = g_scr[st];  // element references survive rehashing
+    if (!r.part) SSR_HIP_CHECK(hipMalloc((void **)&r.part, sizeof(double) * kRedBlocks * 128));
+    if (!r.red) SSR_HIP_CHECK(hipMalloc((void **)&r.red, sizeof(double) * 128));
+    *out = &r;
     return SSR_OK;
 }
 
@@ -378,11 +391,12 @@ int spmv_csr(int64_t n_rows, const int64_t *rp, const int32_t *ci, const T *v, c
 template <class T>
 int dot(int64_t n, const T *a, const T *b, const double *mask, double *result)
 {
-    int rc = ensure_scratch();
+    RedScratch *sc = nullptr;
+    int rc = scratch(&sc);
     if (rc) return rc;
     const unsigned nb = red_blocks(n);
-    hipLaunchKernelGGL((k_dot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, a, b, mask, g_part);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, ssr_stream(), (int)nb, 1, g_part, result);
+    hipLaunchKernelGGL((k_dot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, a, b, mask, sc->part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, ssr_stream(), (int)nb, 1, sc->part, result);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }
@@ -392,11 +406,13 @@ int multidot(int64_t n, int k, const T *V, int64_t ld, const T *u, double *h)
 {
     SSR_REQUIRE(k >= 0 && k <= 128, "multidot: k must be in [0,128]");
     if (k == 0) return SSR_OK;
-    int rc = ensure_scratch();
+    RedScratch *sc = nullptr;
+    int rc = scratch(&sc);
     if (rc) return rc;
     const unsigned nb = red_blocks(n);
-    hipLaunchKernelGGL((k_multidot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u, g_part);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, g_part, h);
+    hipLaunchKernelGGL((k_multidot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
+                       sc->part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, sc->part, h);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }
@@ -539,10 +555,11 @@ int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask,
 // host-returning dot (synchronises)
 int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result)
 {
-    int rc = ensure_scratch();
+    RedScratch *sc = nullptr;
+    int rc = scratch(&sc);
     if (rc) return rc;
-    if ((rc = ssr::dot<double>(n, a, b, mask, g_red))) return rc;
-    SSR_HIP_CHECK(hipMemcpyAsync(host_result, g_red, sizeof(double), hipMemcpyDeviceToHost, ssr_stream()));
+    if ((rc = ssr::dot<double>(n, a, b, mask, sc->red))) return rc;
+    SSR_HIP_CHECK(hipMemcpyAsync(host_result, sc->red, sizeof(double), hipMemcpyDeviceToHost, ssr_stream()));
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     return SSR_OK;
 }

@@ -25,6 +25,14 @@
 
 namespace {
 
+// stream-ordered copy on the library stream, completed before returning
+hipError_t ssr_copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ssr_stream());
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(ssr_stream());
+}
+
 __host__ __device__ inline uint16_t f2bf(float f)
 {
     uint32_t u;
@@ -291,11 +299,11 @@ int ssr_mlp_create(int n_in, int n_hidden, int n_out, int n_layers, uint64_t see
     if ((e = hipMalloc((void **)&m->m, sizeof(float) * off)) != hipSuccess) return fail(e);
     if ((e = hipMalloc((void **)&m->v, sizeof(float) * off)) != hipSuccess) return fail(e);
     if ((e = hipMalloc((void **)&m->pb, sizeof(uint16_t) * off)) != hipSuccess) return fail(e);
-    if ((e = hipMemcpy(m->p, h.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+    if ((e = ssr_copy_sync(m->p, h.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
     std::vector<float> z((size_t)off, 0.0f);
-    if ((e = hipMemcpy(m->m, z.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
-    if ((e = hipMemcpy(m->v, z.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
-    if ((e = hipMemcpy(m->grad, z.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+    if ((e = ssr_copy_sync(m->m, z.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+    if ((e = ssr_copy_sync(m->v, z.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+    if ((e = ssr_copy_sync(m->grad, z.data(), sizeof(float) * off, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
     int rc = m->refresh_bf16();
     if (rc) {
         delete m;
@@ -325,17 +333,17 @@ int ssr_mlp_info(const ssr_mlp *m, int64_t *info, int cap)
 int ssr_mlp_get_params(const ssr_mlp *m, float *host)
 {
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
-    SSR_HIP_CHECK(hipMemcpy(host, m->p, sizeof(float) * m->n_params, hipMemcpyDeviceToHost));
+    SSR_HIP_CHECK(ssr_copy_sync(host, m->p, sizeof(float) * m->n_params, hipMemcpyDeviceToHost));
     return SSR_OK;
 }
 
 int ssr_mlp_set_params(ssr_mlp *m, const float *host)
 {
-    SSR_HIP_CHECK(hipMemcpy(m->p, host, sizeof(float) * m->n_params, hipMemcpyHostToDevice));
+    SSR_HIP_CHECK(ssr_copy_sync(m->p, host, sizeof(float) * m->n_params, hipMemcpyHostToDevice));
     m->step = 0;
     std::vector<float> z((size_t)m->n_params, 0.0f);
-    SSR_HIP_CHECK(hipMemcpy(m->m, z.data(), sizeof(float) * m->n_params, hipMemcpyHostToDevice));
-    SSR_HIP_CHECK(hipMemcpy(m->v, z.data(), sizeof(float) * m->n_params, hipMemcpyHostToDevice));
+    SSR_HIP_CHECK(ssr_copy_sync(m->m, z.data(), sizeof(float) * m->n_params, hipMemcpyHostToDevice));
+    SSR_HIP_CHECK(ssr_copy_sync(m->v, z.data(), sizeof(float) * m->n_params, hipMemcpyHostToDevice));
     return m->refresh_bf16();
 }
 

@@ -51,7 +51,10 @@ int upload(T **dst, const std::vector<T> &src)
         ssr_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
         return SSR_ERR_ALLOC;
     }
-    if (!src.empty()) SSR_HIP_CHECK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    if (!src.empty()) {
+        SSR_HIP_CHECK(hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, ssr_stream()));
+        SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+    }
     return SSR_OK;
 }
 }  // namespace

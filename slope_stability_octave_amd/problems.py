@@ -170,3 +170,34 @@ def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_option
     timings.update(lin.timings)
     return dict(FoS=float(lam_h[-1]), lambda_hist=lam_h, omega_hist=om_h, Umax_hist=umax_h, work_hist=work_h, U=U,
                 setup_s=t_setup, time_to_fos_s=t_run, timings=timings, counts=dict(cmb.counts))
+
+
+def run_batch(problems, workers=8, params=None, preconditioner="amg", precision=64, predictor_factories=None):
+    """Independent SSR instances (of any sizes) solved concurrently: each
+    worker thread drives its instances on a HIP stream of its own, so the
+    kernels of different instances overlap on the device (the library
+    releases the Python lock inside every call).  Returns the per-instance
+    results (in input order) and the wall time of the whole batch."""
+    from concurrent.futures import ThreadPoolExecutor
+    from ._lib import ThreadStream
+    import threading
+
+    local = threading.local()
+    streams = []
+    lock = threading.Lock()
+
+    def solve(i):
+        if not hasattr(local, "ts"):
+            local.ts = ThreadStream().__enter__()
+            with lock:
+                streams.append(local.ts)
+        pf = predictor_factories[i] if predictor_factories else None
+        return run_ssr(problems[i], params, preconditioner=preconditioner, precision=precision, predictor=pf)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        results = list(ex.map(solve, range(len(problems))))
+    wall = time.perf_counter() - t0
+    for ts in streams:
+        ts.__exit__(None, None, None)
+    return results, wall
