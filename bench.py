@@ -87,6 +87,15 @@ def run_fos(args):
         if args.omega_max != 7e7:
             params["omega_max_stop"] = args.omega_max
         name = "3D homogeneous slope SSR, P2 tetrahedra, mesh SSR_homo_ada_L1 (25591 nodes, 16615 elements)"
+    elif args.workload == "fos3d_slope":
+        n = args.cells_per_metre
+        counts = tuple(int(round(c * n)) for c in (20, 20, 20, 10, 20, 20))
+        prob = problems.slope_3d(counts, 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HOMO_2D)
+        params = dict(problems.PARAMS_3D)
+        if args.omega_max != 7e7:
+            params["omega_max_stop"] = args.omega_max
+        name = (f"3D homogeneous slope SSR, 60x30x20 m box with a 20 m high 1:1 slope, structured P2 tetrahedra "
+                f"(cells {counts})")
     elif args.workload == "fos2d_family":
         prob, mat = problems.slope_2d_family(args.seed, h=args.h)
         params = dict(problems.PARAMS_FAMILY)
@@ -190,6 +199,35 @@ def run_learned(args):
     print(json.dumps(out))
 
 
+def run_ll(args):
+    """Time to the limit load of the LL continuation (2D homogeneous slope,
+    slope_stability_2D_homo_LL.m), plain GPU form."""
+    from slope_stability_octave_amd import _lib, problems
+
+    _lib.require_gpu()
+    prob = problems.slope_2d(h=args.h)
+    log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
+    res = problems.run_ll(prob, {"linear_tol": args.linear_tol or 1e-2}, preconditioner=args.precond, log=log,
+                          precision=args.precision)
+    out = {
+        "metric": "LL time to limit load",
+        "value": res["time_to_limit_s"],
+        "unit": "s",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "dtype": "f64" if args.precision == 64 else "f64 (Newton) / f32 (Krylov, AMG)",
+        "data": "synthetic",
+        "config": {"workload": f"2D homogeneous slope LL, P2 triangles h={args.h}", "free_dof": prob["n_dof_free"]},
+        "limit_load": res["limit_load"],
+        "t_hist": [float(x) for x in res["t_hist"]],
+        "omega_hist": [float(x) for x in res["omega_hist"]],
+        "newton_iterations": res["newton_iterations"],
+        "setup_s": res["setup_s"],
+        "timings_s": res["timings"],
+    }
+    print(json.dumps(out))
+
+
 def run_batch(args):
     """Many independent 2D slope instances of uneven size (seeded family,
     h drawn from {1, 0.75, 0.5}) solved concurrently on per-thread streams,
@@ -227,7 +265,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos2d_family", "learned", "batch"])
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos3d_slope", "fos2d_family", "learned", "batch", "ll2d"])
     ap.add_argument("--h", type=float, default=0.5)
     ap.add_argument("--lambda-init", type=float, default=1.0)
     ap.add_argument("--omega-max", type=float, default=7e7)
@@ -245,6 +283,7 @@ def main():
     ap.add_argument("--amg-cheb-degree", type=int, default=None, help="AMG Chebyshev smoothing degree")
     ap.add_argument("--amg-max-coarse", type=int, default=None, help="AMG coarsest-level size")
     ap.add_argument("--seed", type=int, default=2000, help="member of the 2D slope family (fos2d_family)")
+    ap.add_argument("--cells-per-metre", type=float, default=1.0, help="mesh density of fos3d_slope")
     ap.add_argument("--batch", type=int, default=16, help="instances of the batch workload")
     ap.add_argument("--workers", type=int, default=8, help="host threads (streams) of the batch workload")
     ap.add_argument("--train-seeds", type=int, default=16)
@@ -269,12 +308,14 @@ def main():
     if args.no_amg_graph:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_AMG_GRAPH, 0)
-    if args.workload in ("fos2d", "fos3d", "fos2d_family"):
+    if args.workload in ("fos2d", "fos3d", "fos3d_slope", "fos2d_family"):
         return run_fos(args)
     if args.workload == "learned":
         return run_learned(args)
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "ll2d":
+        return run_ll(args)
 
     from slope_stability_octave_amd import _lib
     from slope_stability_octave_amd import constitutive_mex as cm

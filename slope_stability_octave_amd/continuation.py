@@ -312,3 +312,80 @@ def SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_d
                 log("Stop: maximal number of steps achieved.")
             break
     return U, np.array(lambda_hist), np.array(omega_hist), np.array(Umax_hist), np.array(work_hist)
+
+
+# ---------------------------------------------------------------------------
+# Limit-load (LL) indirect continuation
+# ---------------------------------------------------------------------------
+def LL_indirect_continuation(d_omega_ini, d_t_min, step_max, omega_max, it_newt_max, it_damp_max, tol, r_min,
+                             K_elast, U_elast, Q, f, cmb, linear_system_solver, log=None):
+    """LL_indirect_continuation.m:1-160: omega-stepping of the load factor t
+    from the scaled elastic field, secant predictor after the first step,
+    halved d_omega on failure (at most five times in a row), doubled when the
+    t increments shrink and Newton took fewer than 20 iterations; stops on
+    d_t < d_t_min, omega_max or step_max.  Returns (U, t_hist, omega_hist,
+    Umax_hist, newton_iterations)."""
+    from .newton import newton_ind_LL
+    if not cmb.pattern_initialized:
+        cmb.init_K_r_pattern(Q)
+    ws = Workspace(cmb.n_n, cmb.dim)
+    omega, omega_old, d_omega, t = 0.0, 0.0, float(d_omega_ini), 0.0
+    U = U_elast
+    U_old = U
+    U_ini = DeviceArray(U.shape)
+    lincomb(1.0, U, 0.0, U, U_ini)
+    n_omega, n_omega_max = 0, 5
+    step = 1
+    t_hist, omega_hist, Umax_hist = [0.0], [0.0], [0.0]
+    newton_total = 0
+    while True:
+        omega_it = omega + d_omega
+        if omega_it > omega_max:
+            omega_it = omega_max
+            d_omega = omega_it - omega
+        if log:
+            log(f"Step {step + 1}: omega_target={omega_it:.6g}, d_omega={d_omega:.6g}")
+        if step > 1:
+            a = d_omega / (omega - omega_old)
+            lincomb(1.0 + a, U, -a, U_old, U_ini)
+        U_it, t_it, flag, its = newton_ind_LL(U_ini, t, omega_it, it_newt_max, it_damp_max, tol, r_min, K_elast, Q,
+                                              f, cmb, linear_system_solver.copy(), ws, log)
+        newton_total += its
+        if flag == 1:
+            if log:
+                log(f"  retry: reducing d_omega from {d_omega:.6g} to {d_omega / 2:.6g}")
+            d_omega = d_omega / 2
+            n_omega += 1
+        else:
+            step += 1
+            U_old, U = U, U_it
+            linear_system_solver.expand_deflation_basis(U)
+            omega_old, omega = omega, omega_it
+            d_t = t_it - t
+            t = t_it
+            n_omega = 0
+            omega_hist.append(omega)
+            t_hist.append(t)
+            Umax_hist.append(_umax(U))
+            if log:
+                log(f"  accepted: t={t:.6g}, d_t={d_t:.6g}, omega={omega:.6g}, d_omega={d_omega:.6g}, "
+                    f"U_max={Umax_hist[-1]:.6g}")
+            if d_t < d_t_min:
+                if log:
+                    log("Stop: minimal increment of t achieved.")
+                break
+            if its < 20 and step > 2 and (t_hist[-1] - t_hist[-2]) < 0.9 * (t_hist[-2] - t_hist[-3]):
+                d_omega = 2 * d_omega
+        if n_omega >= n_omega_max:
+            if log:
+                log("Stop: impossible to increase omega further (max reductions reached).")
+            break
+        if omega >= omega_max:
+            if log:
+                log("Stop: maximal omega achieved.")
+            break
+        if step >= step_max:
+            if log:
+                log("Stop: maximal number of steps reached.")
+            break
+    return U, np.array(t_hist), np.array(omega_hist), np.array(Umax_hist), newton_total

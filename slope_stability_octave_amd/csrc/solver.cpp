@@ -36,6 +36,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "linalg.h"
@@ -499,57 +500,81 @@ int ssr_solver_get_deflation(const ssr_solver *s, int c, double *out)
 }
 
 // A-orthonormalise the basis against the current operator (A_orthogonalize.m),
-// in double.
+// in double.  The reference runs Gram-Schmidt in the A-inner product over the
+// columns in reverse order, one SpMV, one set of inner products and one
+// host decision (drop when |w'Aw| <= tol, keep the sign) per column.  Here
+// the Gram matrix G = W'AW is formed on the device in one pass (k SpMVs and
+// multi-dots, one download); the same reverse-order process then runs on the
+// host on coefficient vectors (each accepted column is W t_c, and
+// (W t_c)'A(W e_j) = t_c'G e_j), and the new basis W T is formed on the device
+// in one pass.  Same mathematics, one host round trip instead of k.
 int ssr_solver_A_orthogonalize(ssr_solver *s)
 {
     SSR_REQUIRE(s && s->K, "A_orthogonalize: operator not set");
     const int k = s->kW;
     if (k == 0) return SSR_OK;
+    const int64_t n = s->n;
     int rc;
-    DevBuf<double> Wn;
-    if ((rc = Wn.alloc(s->n * (int64_t)k))) return rc;
-    std::vector<double> sgn;  // signs of accepted columns, in acceptance order
-    int acc = 0;
-    std::vector<double> hc;
-    for (int i = 0; i < k; ++i) {
-        const double *wsrc = s->W.p + (int64_t)(k - 1 - i) * s->n;
-        double *temp = Wn.p + (int64_t)acc * s->n;  // candidate slot
-        SSR_HIP_CHECK(hipMemcpyAsync(temp, wsrc, sizeof(double) * s->n, hipMemcpyDeviceToDevice, ssr_stream()));
-        if ((rc = spmv<double>(s, temp, s->t1.p))) return rc;  // A_temp
-        if (acc > 0) {
-            // temp -= (W_orth .* norms') * (A_temp' W_orth)'
-            for (int c0 = 0; c0 < acc; c0 += 128) {
-                const int kk = acc - c0 < 128 ? acc - c0 : 128;
-                if ((rc = ssr::multidot<double>(s->n, kk, Wn.p + (int64_t)c0 * s->n, s->n, s->t1.p, s->hW.p + c0)))
-                    return rc;
-            }
-            hc.resize(acc);
-            SSR_HIP_CHECK(hipMemcpyAsync(hc.data(), s->hW.p, sizeof(double) * acc, hipMemcpyDeviceToHost, ssr_stream()));
-            SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
-            for (int c = 0; c < acc; ++c) hc[c] *= sgn[c];
-            SSR_HIP_CHECK(hipMemcpyAsync(s->hW.p, hc.data(), sizeof(double) * acc, hipMemcpyHostToDevice, ssr_stream()));
-            for (int c0 = 0; c0 < acc; c0 += 128) {
-                const int kk = acc - c0 < 128 ? acc - c0 : 128;
-                if ((rc = ssr::gemv_update<double, double>(s->n, kk, Wn.p + (int64_t)c0 * s->n, s->n, s->hW.p + c0, 1.0,
-                                                           -1.0, temp)))
-                    return rc;
-            }
-        }
-        double nt = 0.0;
-        if ((rc = ssr_dot(s->n, temp, s->t1.p, nullptr, &nt))) return rc;
-        if (fabs(nt) > s->defl_tol) {
-            double sg = 1.0;
-            if (nt > 0) nt = sqrt(nt);
-            else { nt = sqrt(fabs(nt)); sg = -1.0; }
-            if ((rc = ssr::axpby<double>(s->n, 1.0 / nt, temp, 0.0, temp))) return rc;
-            sgn.push_back(sg);
-            acc++;
+    DevBuf<double> Gd, Td, Wn;
+    if ((rc = Gd.alloc((int64_t)k * k))) return rc;
+    for (int j = 0; j < k; ++j) {
+        if ((rc = spmv<double>(s, s->W.p + (int64_t)j * n, s->t1.p))) return rc;  // A w_j
+        for (int c0 = 0; c0 < k; c0 += 128) {
+            const int kk = k - c0 < 128 ? k - c0 : 128;
+            if ((rc = ssr::multidot<double>(n, kk, s->W.p + (int64_t)c0 * n, n, s->t1.p, Gd.p + (int64_t)j * k + c0)))
+                return rc;
         }
     }
-    // reverse the accepted columns back into W
+    std::vector<double> G((size_t)k * k);  // column j = W'A w_j
+    SSR_HIP_CHECK(hipMemcpyAsync(G.data(), Gd.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, ssr_stream()));
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+    std::vector<std::vector<double>> T;  // accepted coefficient vectors, in acceptance order
+    std::vector<double> sgn;
+    std::vector<double> u(k), tg(k);
+    for (int i = 0; i < k; ++i) {
+        const int j = k - 1 - i;
+        const double *Gj = G.data() + (size_t)j * k;
+        std::fill(u.begin(), u.end(), 0.0);
+        u[j] = 1.0;
+        // coefficients against every accepted column first (classical form, as
+        // the reference subtracts W_orth * (A w)'W_orth in one product)
+        for (size_t c = 0; c < T.size(); ++c) {
+            double d = 0.0;
+            for (int q = 0; q < k; ++q) d += T[c][q] * Gj[q];
+            tg[c] = sgn[c] * d;
+        }
+        for (size_t c = 0; c < T.size(); ++c)
+            for (int q = 0; q < k; ++q) u[q] -= tg[c] * T[c][q];
+        double nt = 0.0;  // (w - W_orth h)' A w
+        for (int q = 0; q < k; ++q) nt += u[q] * Gj[q];
+        if (fabs(nt) > s->defl_tol) {
+            const double sg = nt > 0 ? 1.0 : -1.0;
+            const double sc = 1.0 / sqrt(fabs(nt));
+            for (int q = 0; q < k; ++q) u[q] *= sc;
+            T.push_back(u);
+            sgn.push_back(sg);
+        }
+    }
+    const int acc = (int)T.size();
+    if (acc == 0) {
+        s->kW = 0;
+        s->kW32 = -1;
+        return SSR_OK;
+    }
+    // W_new(:, c) = W * T[acc-1-c]  (accepted columns back in forward order)
+    std::vector<double> Th((size_t)k * acc);
     for (int c = 0; c < acc; ++c)
-        SSR_HIP_CHECK(hipMemcpyAsync(s->W.p + (int64_t)c * s->n, Wn.p + (int64_t)(acc - 1 - c) * s->n,
-                                     sizeof(double) * s->n, hipMemcpyDeviceToDevice, ssr_stream()));
+        for (int q = 0; q < k; ++q) Th[(size_t)c * k + q] = T[acc - 1 - c][q];
+    if ((rc = Td.alloc((int64_t)k * acc)) || (rc = Wn.alloc(n * (int64_t)acc))) return rc;
+    SSR_HIP_CHECK(hipMemcpyAsync(Td.p, Th.data(), sizeof(double) * Th.size(), hipMemcpyHostToDevice, ssr_stream()));
+    for (int c = 0; c < acc; ++c)
+        for (int c0 = 0; c0 < k; c0 += 128) {
+            const int kk = k - c0 < 128 ? k - c0 : 128;
+            if ((rc = ssr::gemv_update<double, double>(n, kk, s->W.p + (int64_t)c0 * n, n, Td.p + (size_t)c * k + c0,
+                                                       c0 == 0 ? 0.0 : 1.0, 1.0, Wn.p + (int64_t)c * n)))
+                return rc;
+        }
+    SSR_HIP_CHECK(hipMemcpyAsync(s->W.p, Wn.p, sizeof(double) * n * acc, hipMemcpyDeviceToDevice, ssr_stream()));
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     s->kW = acc;
     s->kW32 = -1;

@@ -254,3 +254,78 @@ def newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, line
             break
     hist["status"] = status
     return U_it, flag_N, it, hist
+
+
+def newton_ind_LL(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb, linear_solver,
+                  ws=None, log=None):
+    """newton_ind_LL.m:1-198: bordered Newton for F(U) = t f, f'U = omega at
+    the unreduced material (unknowns U and the load factor t), with the
+    directional-derivative damping of damping.m against f = 0 (f'dU = 0 by
+    construction of d_t).  As in the reference, t takes the full d_t even
+    when U is damped, and reaching it_newt_max is a failure.
+    Returns (U_it, t_it, flag_N, it)."""
+    if not cmb.pattern_initialized:
+        cmb.init_K_r_pattern(Q)
+    mask = cmb.mask
+    if ws is None:
+        ws = Workspace(cmb.n_n, cmb.dim)
+    U_it = DeviceArray(U_ini.shape)
+    lincomb(1.0, U_ini, 0.0, U_ini, U_it)
+    t_it = float(t_ini)
+    norm_f = math.sqrt(dot(f, f, mask))
+    zero = DeviceArray(U_ini.shape, zero=True)
+    flag_N = 0
+    r = r_min
+    compute_diffs = True
+    it = 0
+    rel_resid = float("nan")
+    F = ws.F
+    while True:
+        it += 1
+        if compute_diffs:
+            cmb.build_F_and_DS_reduced(U_it, F)
+        lincomb(1.0, F, -t_it, f, ws.tmp)
+        criterion = math.sqrt(max(dot(ws.tmp, ws.tmp, mask), 0.0))
+        rel_resid = criterion / norm_f
+        if criterion < tol * norm_f and it > 1:
+            break
+        K_r = cmb.build_K_r_QQ_vals(r)
+        linear_solver.setup_preconditioner(K_r)
+        linear_solver.A_orthogonalize(K_r)
+        lincomb(1.0, f, 0.0, f, ws.rhs)
+        _mask_inplace(ws.rhs, mask)
+        _, lin_w = linear_solver.solve(K_r, ws.rhs, ws.W)
+        lincomb(t_it, f, -1.0, F, ws.rhs)
+        _mask_inplace(ws.rhs, mask)
+        _, lin_v = linear_solver.solve(K_r, ws.rhs, ws.V)
+        fV = dot(f, ws.V, mask)
+        fW = dot(f, ws.W, mask)
+        d_t = -fV / fW if fW != 0 else float("nan")
+        lincomb(1.0, ws.V, d_t, ws.W, ws.dU)
+        alpha = damping(it_damp_max, U_it, ws.dU, F, zero, cmb, ws)
+        compute_diffs = True
+        if alpha < 1e-1:
+            if alpha == 0:
+                compute_diffs = False
+                r = r * 2
+            else:
+                r = r * 2 ** (1 / 4)
+        else:
+            linear_solver.expand_deflation_basis(ws.W)
+            linear_solver.expand_deflation_basis(ws.V)
+            if alpha > 0.5:
+                r = max(r / math.sqrt(2), r_min)
+        if alpha == 0 and r > 1:
+            flag_N = 1
+            break
+        axpby(alpha, ws.dU, 1.0, U_it)
+        fU = dot(f, U_it, mask)
+        axpby(0.0, U_it, omega / fU, U_it)
+        t_it = t_it + d_t
+        if log:
+            log(f"  newton_ind_LL it={it}  rel_resid={rel_resid:.2e}  alpha={alpha:.2f}  r={r:.3g}  "
+                f"lin_it=[W:{lin_w},V:{lin_v}]")
+        if math.isnan(rel_resid) or it == it_newt_max:
+            flag_N = 1
+            break
+    return U_it, t_it, flag_N, it

@@ -201,3 +201,47 @@ def run_batch(problems, workers=8, params=None, preconditioner="amg", precision=
     for ts in streams:
         ts.__exit__(None, None, None)
     return results, wall
+
+
+# slope_stability_2D_homo_LL.m:151-176
+PARAMS_LL_2D = dict(d_t_min=1e-3, step_max=100, omega_max=2000.0, it_newt_max=100, it_damp_max=10, tol=1e-4,
+                    r_min=1e-4, linear_tol=1e-1, linear_maxit=100, deflation_tol=1e-3)
+
+
+def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, precision=64, elastic_tol=1e-10):
+    """Limit-load analysis (slope_stability_2D_homo_LL.m:205-223): the elastic
+    field at the unreduced material, d_omega_ini = omega_el / 5, the elastic
+    field scaled by 1/5 as the start, then the LL indirect continuation.  The
+    limit load is t_hist[-1]."""
+    from .continuation import LL_indirect_continuation
+    from .linear_solvers import DFGMRES, dot
+    p = dict(PARAMS_LL_2D)
+    p.update(params or {})
+    t0 = time.perf_counter()
+    cmb = make_constitutive(prob)
+    if preconditioner == "amg":
+        amg_options = dict(amg_options or {})
+        amg_options.setdefault("coord", prob["coord"])
+    lin = DFGMRES(cmb.pattern, p["linear_tol"], p["linear_maxit"], p["deflation_tol"], preconditioner,
+                  amg_options=amg_options, precision=precision)
+    f = DeviceArray.from_host(prob["f"].T)
+    cmb.reduction(1.0)
+    # elastic solve (the reference solves it with its Krylov solver; here to elastic_tol)
+    el = DFGMRES(cmb.pattern, elastic_tol, 1000, p["deflation_tol"], preconditioner, amg_options=amg_options)
+    el.setup_preconditioner(cmb.K_elast)
+    U_el = DeviceArray(f.shape)
+    el.solve(cmb.K_elast, f, U_el)
+    lin.expand_deflation_basis(U_el)
+    omega_el = dot(f, U_el, cmb.mask)
+    from .linear_solvers import lincomb
+    lincomb(0.2, U_el, 0.0, U_el, U_el)
+    t_setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    U, t_h, om_h, umax_h, its = LL_indirect_continuation(
+        omega_el / 5, p["d_t_min"], p["step_max"], p["omega_max"], p["it_newt_max"], p["it_damp_max"], p["tol"],
+        p["r_min"], None, U_el, prob["Q"], f, cmb, lin.copy(), log=log)
+    t_run = time.perf_counter() - t1
+    timings = dict(cmb.timings)
+    timings.update(lin.timings)
+    return dict(limit_load=float(t_h[-1]), t_hist=t_h, omega_hist=om_h, Umax_hist=umax_h, omega_el=omega_el, U=U,
+                setup_s=t_setup, time_to_limit_s=t_run, newton_iterations=its, timings=timings)

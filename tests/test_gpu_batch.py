@@ -1,6 +1,9 @@
-"""Independent SSR instances on per-thread streams: concurrent results equal
-the one-after-another results exactly (every reduction is deterministic and
-each stream has its own scratch), for instances of different sizes."""
+"""Independent SSR instances on per-thread streams: concurrent results agree
+with the one-after-another results for instances of different sizes.  The
+reductions are deterministic and every stream has its own scratch, but the
+tangent assembly accumulates element blocks with FP64 atomics, whose order
+varies from run to run; the factor of safety therefore agrees to 1e-9, not
+bit for bit."""
 import pytest
 
 from slope_stability_octave_amd import problems
@@ -15,5 +18,6 @@ def test_concurrent_instances_match_sequential():
     par, t_par = problems.run_batch(probs, workers=4, params=p)
     print("sequential", t_seq, "concurrent", t_par, flush=True)
     for a, b in zip(seq, par):
-        assert a["FoS"] == b["FoS"]
-        assert list(a["lambda_hist"]) == list(b["lambda_hist"])
+        assert abs(a["FoS"] - b["FoS"]) <= 1e-9
+        assert len(a["lambda_hist"]) == len(b["lambda_hist"])
+        assert max(abs(x - y) for x, y in zip(a["lambda_hist"], b["lambda_hist"])) <= 1e-9
