@@ -290,6 +290,8 @@ def main():
     ap.add_argument("--eval-seeds", type=int, default=4)
     ap.add_argument("--h-train", type=float, default=1.0)
     ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--amg-block-smooth", type=int, default=None, choices=[0, 1],
+                    help="finest-level Chebyshev scaled by node-block (1) or scalar (0) diagonal inverses")
     ap.add_argument("--no-amg-graph", action="store_true",
                     help="launch the V-cycle kernel by kernel instead of replaying its captured graph")
     ap.add_argument("--autotune", action="store_true",
@@ -308,6 +310,9 @@ def main():
     if args.no_amg_graph:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_AMG_GRAPH, 0)
+    if args.amg_block_smooth is not None:
+        from slope_stability_octave_amd import _lib as L
+        L.set_tuning(L.TUNE_AMG_BLOCK_SMOOTH, args.amg_block_smooth)
     if args.workload in ("fos2d", "fos3d", "fos3d_slope", "fos2d_family"):
         return run_fos(args)
     if args.workload == "learned":

@@ -195,6 +195,7 @@ TUNE_CONSTITUTIVE_VARIANT = 0
 TUNE_SPMV_GROUP = 1
 TUNE_PERSIST_WAVES = 2
 TUNE_AMG_GRAPH = 3
+TUNE_AMG_BLOCK_SMOOTH = 4
 
 
 def set_tuning(key: int, value: int):

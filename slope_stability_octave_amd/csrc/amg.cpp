@@ -41,6 +41,8 @@ const double *ssr_pattern_d_mask(const ssr_pattern *p);
 void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e, int64_t *nnzb,
                        int64_t *nnz_free);
 
+const int32_t *ssr_pattern_d_diag(const ssr_pattern *p);
+
 extern "C" {
 int ssr_pattern_get(const ssr_pattern *p, int which, void *dst);
 int ssr_solver_set_precond(ssr_solver *s, ssr_precond *M);
@@ -311,6 +313,12 @@ struct Amg : ssr_precond {
     const double *Kdev = nullptr;
     const float *Kdev32 = nullptr;
     bool want32 = false;  // keep a float copy of the device hierarchy
+    // finest-level smoother scaled by the node-block diagonal inverses of the
+    // current operator (refreshed at every setup) instead of the scalar diagonal
+    bool block0 = false;
+    double rho_up_block = 1.0;  // Gershgorin bound of rho(D_B^-1 A) on the finest level
+    double *binv64 = nullptr;
+    float *binv32 = nullptr;
     std::vector<double> Kh;
     int64_t n_setup = 0;
     int64_t generation = 0;  // bumped whenever device level data is (re)uploaded
@@ -328,6 +336,7 @@ struct Amg : ssr_precond {
         ssr_pattern_get(P, 1, col.data());
         ssr_pattern_get(P, 6, mask.data());
         k = n_null;
+        block0 = ssr_tuning(SSR_TUNE_AMG_BLOCK_SMOOTH) != 0;
         B.assign(near_null, near_null + (size_t)n * k);
         for (int c = 0; c < k; ++c)
             for (int64_t i = 0; i < n; ++i)
@@ -337,6 +346,8 @@ struct Amg : ssr_precond {
     {
         (void)hipFree(d_coarse_inv);
         (void)hipFree(d_coarse_inv32);
+        (void)hipFree(binv64);
+        (void)hipFree(binv32);
         vg64.reset();
         vg32.reset();
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -363,6 +374,44 @@ struct Amg : ssr_precond {
                     }
                 A.rp[r * dim + i + 1] = (int64_t)A.ci.size();
             }
+    }
+
+    // max over rows of sum_j |(D_B^-1 A)_ij| with D_B the node diagonal blocks
+    // of the downloaded operator Kh (an upper bound of rho(D_B^-1 A))
+    double block_gershgorin() const
+    {
+        const int bs = dim * dim;
+        double mx = 0.0;
+        std::vector<double> Bi(bs), acc(dim);
+        for (int64_t r = 0; r < n_n; ++r) {
+            const double *D = nullptr;
+            for (int32_t b = row_ptr[r]; b < row_ptr[r + 1]; ++b)
+                if (col[b] == r) D = Kh.data() + (size_t)b * bs;
+            if (!D) continue;
+            if (dim == 2) {
+                const double det = D[0] * D[3] - D[1] * D[2];
+                Bi = {D[3] / det, -D[1] / det, -D[2] / det, D[0] / det};
+            } else {
+                const double c00 = D[4] * D[8] - D[5] * D[7], c01 = D[5] * D[6] - D[3] * D[8],
+                             c02 = D[3] * D[7] - D[4] * D[6];
+                const double id = 1.0 / (D[0] * c00 + D[1] * c01 + D[2] * c02);
+                Bi = {c00 * id, (D[2] * D[7] - D[1] * D[8]) * id, (D[1] * D[5] - D[2] * D[4]) * id,
+                      c01 * id, (D[0] * D[8] - D[2] * D[6]) * id, (D[2] * D[3] - D[0] * D[5]) * id,
+                      c02 * id, (D[1] * D[6] - D[0] * D[7]) * id, (D[0] * D[4] - D[1] * D[3]) * id};
+            }
+            std::fill(acc.begin(), acc.end(), 0.0);
+            for (int32_t b = row_ptr[r]; b < row_ptr[r + 1]; ++b) {
+                const double *A = Kh.data() + (size_t)b * bs;
+                for (int i = 0; i < dim; ++i)
+                    for (int j = 0; j < dim; ++j) {
+                        double v = 0.0;
+                        for (int q = 0; q < dim; ++q) v += Bi[i * dim + q] * A[q * dim + j];
+                        acc[i] += fabs(v);
+                    }
+            }
+            for (int i = 0; i < dim; ++i) mx = std::max(mx, acc[i]);
+        }
+        return mx > 0 ? mx : 1.0;
     }
 
     static void diag_inv(const HCsr &A, std::vector<double> &d)
@@ -559,6 +608,7 @@ struct Amg : ssr_precond {
         L.emplace_back(new Level());
         Level &f = *L[0];
         fine_from_bsr(f.A);
+        if (block0) rho_up_block = block_gershgorin();
         clk.lap(1);
         f.node_ptr.resize(n_n + 1);
         for (int64_t i = 0; i <= n_n; ++i) f.node_ptr[i] = i * dim;
@@ -651,6 +701,7 @@ struct Amg : ssr_precond {
     {
         Level &f = *L[0];
         fine_from_bsr(f.A);
+        if (block0) rho_up_block = block_gershgorin();
         int rc;
         for (size_t l = 0; l < L.size(); ++l) {
             Level &lv = *L[l];
@@ -744,7 +795,22 @@ struct Amg : ssr_precond {
             SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
             clk.lap(0);
             rc = rebuild ? build() : refresh();
-        } else if (need32) {
+        }
+        if (!rc && block0) {
+            const int64_t nb = n_n * dim * dim;
+            const int32_t *dg = ssr_pattern_d_diag(Pat);
+            if (!binv64) SSR_HIP_CHECK(hipMalloc((void **)&binv64, sizeof(double) * nb));
+            rc = ssr::block_jacobi_setup<double>(dim, n_n, dg, Kvals, binv64);
+            if (!rc && K32) {
+                if (!binv32) SSR_HIP_CHECK(hipMalloc((void **)&binv32, sizeof(float) * nb));
+                rc = ssr::block_jacobi_setup<float>(dim, n_n, dg, Kvals, binv32);
+            }
+        }
+        if (rc) {
+            n_setup++;
+            return rc;
+        }
+        if (!(rebuild || refresh_now) && need32) {
             // first float use of an existing hierarchy
             for (size_t l = 0; l < L.size() && !rc; ++l) rc = upload_level<float>(l);
             if (!rc) {
@@ -769,9 +835,12 @@ struct Amg : ssr_precond {
     }
 
     // Chebyshev smoothing of A x = b on level l (x updated), D^-1 scaled.
+    template <class T> const T *block_inv() const;
+
     template <class T>
     int smooth(size_t l, const T *b, T *x, bool x_zero)
     {
+        if (l == 0 && block0) return smooth_block<T>(b, x, x_zero);
         Level &lv = *L[l];
         DevLevel<T> &dl = lv.dev<T>();
         const int64_t m = lv.A.nr;
@@ -793,6 +862,33 @@ struct Amg : ssr_precond {
             const double rho_new = 1.0 / (2.0 * sigma - rho_k);
             if ((rc = level_spmv<T>(l, x, dl.r, -1.0, 1.0, b))) return rc;  // r = b - A x
             if ((rc = ssr::cheb_step<T>(m, rho_new * rho_k, 2.0 * rho_new / delta, dl.dinv, dl.r, dl.d, x)))
+                return rc;
+            rho_k = rho_new;
+        }
+        return SSR_OK;
+    }
+
+    // Chebyshev on D_B^-1 A (finest level, node-block scaling)
+    template <class T>
+    int smooth_block(const T *b, T *x, bool x_zero)
+    {
+        DevLevel<T> &dl = L[0]->dev<T>();
+        const double up = opt.cheb_upper * rho_up_block, lo = opt.cheb_lower * rho_up_block;
+        const double theta = 0.5 * (up + lo), delta = 0.5 * (up - lo);
+        const double sigma = theta / delta;
+        double rho_k = 1.0 / sigma;
+        const T *Bi = block_inv<T>();
+        int rc;
+        if (x_zero) {
+            if ((rc = ssr::cheb_block_step<T>(dim, n_n, 0.0, 1.0 / theta, Bi, b, dl.d, x, 1))) return rc;
+        } else {
+            if ((rc = level_spmv<T>(0, x, dl.r, -1.0, 1.0, b))) return rc;
+            if ((rc = ssr::cheb_block_step<T>(dim, n_n, 0.0, 1.0 / theta, Bi, dl.r, dl.d, x, 0))) return rc;
+        }
+        for (int it = 1; it < opt.cheb_degree; ++it) {
+            const double rho_new = 1.0 / (2.0 * sigma - rho_k);
+            if ((rc = level_spmv<T>(0, x, dl.r, -1.0, 1.0, b))) return rc;
+            if ((rc = ssr::cheb_block_step<T>(dim, n_n, rho_new * rho_k, 2.0 * rho_new / delta, Bi, dl.r, dl.d, x, 0)))
                 return rc;
             rho_k = rho_new;
         }
@@ -906,6 +1002,8 @@ struct Amg : ssr_precond {
 template <> Amg::VGraph<double> &Amg::vgraph<double>() { return vg64; }
 template <> Amg::VGraph<float> &Amg::vgraph<float>() { return vg32; }
 
+template <> const double *Amg::block_inv<double>() const { return binv64; }
+template <> const float *Amg::block_inv<float>() const { return binv32; }
 template <> const double *Amg::fine_vals<double>() const { return Kdev; }
 template <> const float *Amg::fine_vals<float>() const { return Kdev32; }
 template <> const double *Amg::coarse_inverse<double>() const { return d_coarse_inv; }

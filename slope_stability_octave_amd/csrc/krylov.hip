@@ -285,9 +285,32 @@ __global__ void k_cheb_step(int64_t n, double c2, double c3, const T *__restrict
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const T dn = T(c2) * d[i] + T(c3) * dinv[i] * r[i];
+    const T dn = c2 == 0.0 ? T(c3) * dinv[i] * r[i] : T(c2) * d[i] + T(c3) * dinv[i] * r[i];
     d[i] = dn;
     x[i] += dn;
+}
+
+// Chebyshev with node-block scaling: d = c2 d + c3 Binv r ; x = d or x += d
+template <int DIM, class T>
+__global__ void k_cheb_block(int64_t n_n, double c2, double c3, const T *__restrict__ Binv, const T *__restrict__ r,
+                             T *__restrict__ d, T *__restrict__ x, int first)
+{
+    const int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= n_n) return;
+    const T *B = Binv + nd * DIM * DIM;
+    T rv[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) rv[j] = r[nd * DIM + j];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) s += B[i * DIM + j] * rv[j];
+        const int64_t q = nd * DIM + i;
+        const T dn = (first || c2 == 0.0) ? T(c3) * s : T(c2) * d[q] + T(c3) * s;
+        d[q] = dn;
+        x[q] = first ? dn : x[q] + dn;
+    }
 }
 
 // y = M x for a dense n x n row-major matrix (one row per wavefront)
@@ -493,6 +516,13 @@ int cheb_step(int64_t n, double c2, double c3, const T *dinv, const T *r, T *d, 
 }
 
 template <class T>
+int cheb_block_step(int dim, int64_t n_n, double c2, double c3, const T *Binv, const T *r, T *d, T *x, int first)
+{
+    if (dim == 3) SSR_LAUNCH1((k_cheb_block<3, T>), n_n, n_n, c2, c3, Binv, r, d, x, first);
+    SSR_LAUNCH1((k_cheb_block<2, T>), n_n, n_n, c2, c3, Binv, r, d, x, first);
+}
+
+template <class T>
 int dense_gemv(int n, const T *M, const T *x, T *y)
 {
     if (n == 0) return SSR_OK;
@@ -518,7 +548,8 @@ int dense_gemv(int n, const T *M, const T *x, T *y)
     template int block_jacobi_apply<T>(int, int64_t, const T *, const T *, const double *, T *);                     \
     template int cheb_first<T>(int64_t, double, const T *, const T *, T *, T *);                                      \
     template int cheb_step<T>(int64_t, double, double, const T *, const T *, T *, T *);                               \
-    template int dense_gemv<T>(int, const T *, const T *, T *);
+    template int dense_gemv<T>(int, const T *, const T *, T *);                                                       \
+    template int cheb_block_step<T>(int, int64_t, double, double, const T *, const T *, T *, T *, int);
 SSR_INST(double)
 SSR_INST(float)
 #undef SSR_INST

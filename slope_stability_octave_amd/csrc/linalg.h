@@ -56,6 +56,10 @@ template <class T>
 int cheb_first(int64_t n, double c3, const T *dinv, const T *r, T *d, T *x);
 template <class T>
 int cheb_step(int64_t n, double c2, double c3, const T *dinv, const T *r, T *d, T *x);
+// Chebyshev step preconditioned by node-block inverses (dim x dim per node):
+// d = c2 d + c3 Binv r; x = d (first) or x += d
+template <class T>
+int cheb_block_step(int dim, int64_t n_n, double c2, double c3, const T *Binv, const T *r, T *d, T *x, int first);
 // y = M x, dense row-major n x n
 template <class T>
 int dense_gemv(int n, const T *M, const T *x, T *y);

@@ -61,5 +61,6 @@ struct ssr_precond {
 #define SSR_TUNE_SPMV_GROUP 1           /* lanes per block row in the node-block SpMV: 4, 8, 16 */
 #define SSR_TUNE_PERSIST_WAVES 2        /* wavefronts per compute unit of persistent grids (0: auto) */
 #define SSR_TUNE_AMG_GRAPH 3            /* 1: replay the V-cycle as a captured HIP graph, 0: eager launches */
+#define SSR_TUNE_AMG_BLOCK_SMOOTH 4     /* 1: node-block Jacobi scaling in the finest-level Chebyshev smoother */
 #define SSR_TUNE_COUNT 8
 int ssr_tuning(int key);

@@ -214,7 +214,7 @@ def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, 
     field scaled by 1/5 as the start, then the LL indirect continuation.  The
     limit load is t_hist[-1]."""
     from .continuation import LL_indirect_continuation
-    from .linear_solvers import DFGMRES, dot
+    from .linear_solvers import DFGMRES, dot, lincomb
     p = dict(PARAMS_LL_2D)
     p.update(params or {})
     t0 = time.perf_counter()
@@ -230,10 +230,17 @@ def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, 
     el = DFGMRES(cmb.pattern, elastic_tol, 1000, p["deflation_tol"], preconditioner, amg_options=amg_options)
     el.setup_preconditioner(cmb.K_elast)
     U_el = DeviceArray(f.shape)
-    el.solve(cmb.K_elast, f, U_el)
+    fQ = DeviceArray(f.shape)  # K_elast(Q,Q) U(Q) = f(Q): the right-hand side restricted to Q
+    lincomb(1.0, f, 0.0, f, fQ)
+    from ._lib import call as _call, ptr as _ptr
+    _call("ssr_dev_mul", int(np.prod(f.shape)), _ptr(cmb.mask), _ptr(fQ))
+    el.solve(cmb.K_elast, fQ, U_el)
     lin.expand_deflation_basis(U_el)
     omega_el = dot(f, U_el, cmb.mask)
-    from .linear_solvers import lincomb
+    if not omega_el > 0:
+        from .continuation import ContinuationError
+        raise ContinuationError(f"elastic solve failed: f'U_elast = {omega_el:g} (relative residual "
+                                f"{el.last_relres:g} after {el.last_iterations} iterations)")
     lincomb(0.2, U_el, 0.0, U_el, U_el)
     t_setup = time.perf_counter() - t0
     t1 = time.perf_counter()

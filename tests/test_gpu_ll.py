@@ -1,27 +1,41 @@
 """Limit-load (LL) continuation on the 2D homogeneous slope, h = 1/2 (10641
 nodes, 20840 unknowns), against the run recorded with the reference
-(slope_stability/slope_stability_2D_homo_LL.ipynb): d_omega_ini = omega_el/5 =
-84.2891 and the load factor t after each of the first seven accepted steps
-(0.198547 ... 1.38539, printed to 6 digits); the limit load the reference
-stopped at is 1.52993 (omega 711.19).  t at a given omega is a property of the
-converged solution; compared at 1e-4 relative, the final plateau value at
-5e-3 (the stop rule d_t < 1e-3 fires at an omega that depends on the step
-history)."""
+(slope_stability/slope_stability_2D_homo_LL.ipynb, "accepted: t=..., omega=..."
+lines: 19 points of the t(omega) curve up to the limit load 1.52993 at omega
+711.19).
+
+The reference solves the elastic start with its Krylov solver at tolerance
+1e-1, so its omega_el (hence its omega step d_omega_ini = omega_el/5 = 84.29)
+is that solve's, not the exact elastic work (this build solves it to 1e-10:
+omega_el/5 = 110.37).  The omega grids therefore differ; t at a given omega is
+a property of the converged solution, so the comparison interpolates this
+build's curve at the recorded omegas: 2e-4 relative on the elastic branch
+(t linear in omega), 1.5e-3 where the curve bends (linear interpolation
+between this build's points spaced up to 14 apart)."""
+import numpy as np
 import pytest
 
 from slope_stability_octave_amd import problems
 
 pytestmark = pytest.mark.gpu
 
-REF_T = [0.198547, 0.396955, 0.595056, 0.792906, 0.990559, 1.18806, 1.38539]
+REF = [(84.2891, 0.198547), (168.578, 0.396955), (252.867, 0.595056), (337.157, 0.792906), (421.446, 0.990559),
+       (505.735, 1.18806), (590.024, 1.38539), (632.168, 1.48329), (642.705, 1.50421), (647.973, 1.51178),
+       (653.241, 1.51688), (658.509, 1.52016), (663.777, 1.52245), (669.045, 1.52415), (674.313, 1.52539),
+       (679.581, 1.52641), (690.117, 1.52791), (700.653, 1.52903), (711.19, 1.52993)]
 
 
-def test_ll_2d_recorded_trajectory():
+def test_ll_2d_recorded_curve():
     prob = problems.slope_2d(h=0.5)
     res = problems.run_ll(prob, dict(linear_tol=1e-2), preconditioner="amg")
-    t, om = res["t_hist"], res["omega_hist"]
-    print("omega_el/5", res["omega_el"] / 5, "t", list(t), "omega", list(om), flush=True)
-    assert abs(res["omega_el"] / 5 - 84.2891) <= 1e-4 * 84.2891
-    for k, tr in enumerate(REF_T):
-        assert abs(t[k + 1] - tr) <= 1e-4 * tr, (k, t[k + 1], tr)
-    assert abs(res["limit_load"] - 1.52993) <= 5e-3
+    t, om = np.asarray(res["t_hist"]), np.asarray(res["omega_hist"])
+    print("omega_el/5", res["omega_el"] / 5, "t", list(t), "omega", list(om), "time", res["time_to_limit_s"],
+          flush=True)
+    assert np.all(np.diff(om) > 0) and om[-1] >= REF[-1][0]
+    for k, (o, tr) in enumerate(REF):
+        ti = np.interp(o, om, t)
+        tol = 2e-4 if k < 7 else 1.5e-3
+        assert abs(ti - tr) <= tol * tr, (o, ti, tr)
+    # the curve flattens at the limit load (last increments below 1e-3)
+    assert t[-1] - t[-2] < 1e-3
+    assert 1.529 < res["limit_load"] < 1.535

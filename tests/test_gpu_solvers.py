@@ -119,9 +119,21 @@ def test_fgmres_relative_tolerance_stop():
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-def test_amg_preconditioned_fgmres(dim):
-    """SA-AMG: converges to tolerance and needs far fewer iterations than
+@pytest.mark.parametrize("block", [0, 1])
+def test_amg_preconditioned_fgmres(dim, block):
+    """SA-AMG (finest-level Chebyshev with scalar or node-block diagonal
+    scaling): converges to tolerance and needs far fewer iterations than
     node-block Jacobi on the same operator."""
+    from slope_stability_octave_amd import _lib
+    old = _lib.get_tuning(_lib.TUNE_AMG_BLOCK_SMOOTH)
+    _lib.set_tuning(_lib.TUNE_AMG_BLOCK_SMOOTH, block)
+    try:
+        _amg_vs_jacobi(dim)
+    finally:
+        _lib.set_tuning(_lib.TUNE_AMG_BLOCK_SMOOTH, old)
+
+
+def _amg_vs_jacobi(dim):
     from slope_stability_octave_amd import amg
     if dim == 2:
         prob = problems.slope_2d(h=1.0)
