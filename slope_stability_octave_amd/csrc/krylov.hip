@@ -172,6 +172,179 @@ __global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV 
     u[i] = (TU)((alpha == 0.0 ? 0.0 : alpha * (double)u[i]) + beta * s);
 }
 
+// ---------------------------------------------------------------------------
+// Multi-vector SpMV Y(:,s) = K X(:,s) for s < nv <= kSpmmNV: every block of
+// K is read once for all nv vectors (the basis products of the deflation
+// re-orthogonalisation).  Same lane grouping as k_spmv_bsr.
+// ---------------------------------------------------------------------------
+constexpr int kSpmmNV = 8;
+
+template <int DIM, int G, class T>
+__global__ __launch_bounds__(256) void k_spmm_bsr(int64_t n_rows, const int32_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                  const T *__restrict__ X, int64_t ldx, T *__restrict__ Y,
+                                                  int64_t ldy, int nv)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = gid / G;
+    const int l = (int)(gid % G);
+    T acc[kSpmmNV][DIM];
+#pragma unroll
+    for (int s = 0; s < kSpmmNV; ++s)
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) acc[s][i] = T(0);
+    if (r < n_rows) {
+        const int32_t b0 = row_ptr[r], b1 = row_ptr[r + 1];
+        for (int32_t b = b0 + l; b < b1; b += G) {
+            const int64_t c = col[b];
+            T v[DIM * DIM];
+#pragma unroll
+            for (int e = 0; e < DIM * DIM; ++e) v[e] = val[(int64_t)b * DIM * DIM + e];
+#pragma unroll
+            for (int s = 0; s < kSpmmNV; ++s) {
+                if (s < nv) {
+                    T xv[DIM];
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) xv[j] = X[s * ldx + c * DIM + j];
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                        for (int j = 0; j < DIM; ++j) acc[s][i] += v[i * DIM + j] * xv[j];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kSpmmNV; ++s)
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) acc[s][i] += __shfl_xor(acc[s][i], o, G);
+    if (r < n_rows && l == 0) {
+#pragma unroll
+        for (int s = 0; s < kSpmmNV; ++s)
+            if (s < nv)
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) Y[s * ldy + r * DIM + i] = acc[s][i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tall-skinny dense products of the deflation basis (double, column-major):
+//   k_gram_partial: partial X'Y over one row chunk for one 64 x 64 tile of
+//     the k x k result (tile pair = blockIdx.x, chunk = blockIdx.y); the
+//     chunks are then summed in order by k_sum_chunks (deterministic).
+//   k_gemm_tall:    Y = X C for X n x k, C k x m (one 64-row x 64-column
+//     output tile per block).
+// 256 threads as 16 x 16, 4 x 4 outputs each; operands staged through LDS in
+// 32-deep slabs read along the columns (coalesced).
+// ---------------------------------------------------------------------------
+constexpr int kTile = 64, kSlab = 32;
+
+__global__ __launch_bounds__(256) void k_gram_partial(int64_t n, int k, int tiles, int64_t rows_per_chunk,
+                                                      const double *__restrict__ X, const double *__restrict__ Y,
+                                                      double *__restrict__ part)
+{
+    __shared__ double xs[kSlab][kTile + 1];
+    __shared__ double ys[kSlab][kTile + 1];
+    const int i0 = (blockIdx.x % tiles) * kTile, j0 = (blockIdx.x / tiles) * kTile;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+    const int64_t r1 = r0 + rows_per_chunk < n ? r0 + rows_per_chunk : n;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+    for (int64_t rb = r0; rb < r1; rb += kSlab) {
+        for (int e = threadIdx.x; e < kSlab * kTile; e += 256) {
+            const int rr = e % kSlab, cc = e / kSlab;
+            const int64_t r = rb + rr;
+            xs[rr][cc] = (r < r1 && i0 + cc < k) ? X[(int64_t)(i0 + cc) * n + r] : 0.0;
+            ys[rr][cc] = (r < r1 && j0 + cc < k) ? Y[(int64_t)(j0 + cc) * n + r] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int rr = 0; rr < kSlab; ++rr) {
+            double a[4], b[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) a[p] = xs[rr][tx + 16 * p];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[q] = ys[rr][ty + 16 * q];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+        }
+        __syncthreads();
+    }
+    double *out = part + (int64_t)blockIdx.y * k * k;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = i0 + tx + 16 * p, j = j0 + ty + 16 * q;
+            if (i < k && j < k) out[(int64_t)j * k + i] = acc[p][q];
+        }
+}
+
+// out[e] = sum over c < nc of part[c*len + e], in chunk order
+__global__ void k_sum_chunks(int nc, int64_t len, const double *__restrict__ part, double *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    double s = 0.0;
+    for (int c = 0; c < nc; ++c) s += part[(int64_t)c * len + e];
+    out[e] = s;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, const double *__restrict__ X,
+                                                   const double *__restrict__ C, double *__restrict__ Y)
+{
+    __shared__ double xs[kSlab][kTile + 1];  // [q][row]
+    __shared__ double cs[kSlab][kTile + 1];  // [q][column]
+    const int64_t r0 = (int64_t)blockIdx.x * kTile;
+    const int c0 = blockIdx.y * kTile;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
+    for (int q0 = 0; q0 < k; q0 += kSlab) {
+        for (int e = threadIdx.x; e < kSlab * kTile; e += 256) {
+            const int rr = e % kTile, qq = e / kTile;
+            xs[qq][rr] = (r0 + rr < n && q0 + qq < k) ? X[(int64_t)(q0 + qq) * n + r0 + rr] : 0.0;
+            const int q2 = e % kSlab, cc = e / kSlab;
+            cs[q2][cc] = (q0 + q2 < k && c0 + cc < m) ? C[(int64_t)(c0 + cc) * k + q0 + q2] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int qq = 0; qq < kSlab; ++qq) {
+            double a[4], b[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) a[p] = xs[qq][tx + 16 * p];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[q] = cs[qq][ty + 16 * q];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c0 + ty + 16 * q;
+        if (c >= m) continue;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int64_t r = r0 + tx + 16 * p;
+            if (r < n) Y[(int64_t)c * n + r] = acc[p][q];
+        }
+    }
+}
+
 template <class T>
 __global__ void k_axpby(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y)
 {
@@ -532,7 +705,85 @@ int dense_gemv(int n, const T *M, const T *x, T *y)
     return SSR_OK;
 }
 
+template <class T>
+int spmm_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const T *val, const T *X,
+             int64_t ldx, T *Y, int64_t ldy, int nv)
+{
+    SSR_REQUIRE(dim == 2 || dim == 3, "spmm_bsr: dim must be 2 or 3");
+    SSR_REQUIRE(nv >= 0 && nv <= kSpmmNV, "spmm_bsr: nv must be in [0,8]");
+    SSR_REQUIRE(ldx >= n_rows * dim && ldy >= n_rows * dim, "spmm_bsr: leading dimension too small");
+    if (n_rows == 0 || nv == 0) return SSR_OK;
+    constexpr int G = 8;
+    const dim3 grid(ssr_blocks(n_rows * G, 256)), blk(256);
+    if (dim == 3)
+        hipLaunchKernelGGL((k_spmm_bsr<3, G, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, X, ldx, Y,
+                           ldy, nv);
+    else
+        hipLaunchKernelGGL((k_spmm_bsr<2, G, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, X, ldx, Y,
+                           ldy, nv);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+namespace {
+// row chunks of the Gram product: about 2048 blocks in all, each chunk a
+// multiple of the slab depth and at least 8 slabs deep
+void gram_chunks(int64_t n, int k, int *tiles, int64_t *rows_per_chunk, int *chunks)
+{
+    const int t = (k + kTile - 1) / kTile;
+    const int64_t want = (2048 + (int64_t)t * t - 1) / ((int64_t)t * t);
+    int64_t rpc = (n + want - 1) / want;
+    if (rpc < 8 * kSlab) rpc = 8 * kSlab;
+    rpc = (rpc + kSlab - 1) / kSlab * kSlab;
+    *tiles = t;
+    *rows_per_chunk = rpc;
+    *chunks = (int)((n + rpc - 1) / rpc);
+}
+}  // namespace
+
+int64_t gram_work_len(int64_t n, int k)
+{
+    if (n <= 0 || k <= 0) return 0;
+    int t, nc;
+    int64_t rpc;
+    gram_chunks(n, k, &t, &rpc, &nc);
+    return (int64_t)nc * k * k;
+}
+
+int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *work, int64_t work_len)
+{
+    SSR_REQUIRE(k >= 0 && k <= 16384, "gram: k out of range");
+    if (k == 0) return SSR_OK;
+    const int64_t kk = (int64_t)k * k;
+    if (n == 0) return fill<double>(kk, 0.0, G);
+    int t, nc;
+    int64_t rpc;
+    gram_chunks(n, k, &t, &rpc, &nc);
+    SSR_REQUIRE(work && work_len >= (int64_t)nc * kk, "gram: workspace too small");
+    SSR_REQUIRE((int64_t)t * t <= 0x7fffffff && nc <= 65535, "gram: grid too large");
+    hipLaunchKernelGGL(k_gram_partial, dim3((unsigned)(t * t), (unsigned)nc), dim3(256), 0, ssr_stream(), n, k, t,
+                       rpc, X, Y, work);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(ssr_blocks(kk, 256)), dim3(256), 0, ssr_stream(), nc, kk, work, G);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y)
+{
+    SSR_REQUIRE(k >= 0 && m >= 0 && m <= 65535 * kTile, "gemm_tall: sizes out of range");
+    if (n == 0 || m == 0) return SSR_OK;
+    if (k == 0) return fill<double>(n * m, 0.0, Y);
+    const int64_t rb = (n + kTile - 1) / kTile;
+    SSR_REQUIRE(rb <= 0x7fffffff, "gemm_tall: too many rows");
+    hipLaunchKernelGGL(k_gemm_tall, dim3((unsigned)rb, (unsigned)((m + kTile - 1) / kTile)), dim3(256), 0,
+                       ssr_stream(), n, k, m, X, C, Y);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
 #define SSR_INST(T)                                                                                                  \
+    template int spmm_bsr<T>(int, int64_t, const int32_t *, const int32_t *, const T *, const T *, int64_t, T *,      \
+                             int64_t, int);                                                                          \
     template int spmv_bsr<T>(int, int64_t, const int32_t *, const int32_t *, const T *, const T *, T *, double,       \
                              double, const T *);                                                                     \
     template int spmv_csr<T>(int64_t, const int64_t *, const int32_t *, const T *, const T *, T *, double, double,   \

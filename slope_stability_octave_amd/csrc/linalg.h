@@ -11,6 +11,17 @@ namespace ssr {
 template <class T>
 int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const T *val, const T *x,
              T *y, double alpha, double beta, const T *z = nullptr);
+// Y(:, s) = K X(:, s) for s < nv <= 8 (columns at strides ldx, ldy)
+template <class T>
+int spmm_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const T *val, const T *X,
+             int64_t ldx, T *Y, int64_t ldy, int nv);
+// G (k x k, column-major) = X' Y for X, Y n x k column-major (ld n); work
+// holds gram_work_len(n, k) doubles of per-chunk partial sums, added in a
+// fixed order (deterministic)
+int64_t gram_work_len(int64_t n, int k);
+int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *work, int64_t work_len);
+// Y (n x m) = X (n x k) C (k x m), all column-major (ld n, k, n)
+int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y);
 // y = alpha*A*x + beta*z on scalar CSR (64-bit row pointers)
 template <class T>
 int spmv_csr(int64_t n_rows, const int64_t *rp, const int32_t *ci, const T *v, const T *x, T *y, double alpha,

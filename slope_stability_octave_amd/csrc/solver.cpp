@@ -144,6 +144,7 @@ struct ssr_solver {
     DevBuf<double> V, Z, t1, t2, r;
     DevBuf<float> V32, Z32, t2f;
     DevBuf<double> hcol, hW, ydev, scal;  // Arnoldi columns, deflation coefficients, LSQ solution, scalars
+    DevBuf<double> AW, W2, Gd, Td, gwork;  // re-orthogonalisation: A W, new basis, Gram matrix, coefficients
     HostBuf<double> hhost, yhost;
     std::vector<hipEvent_t> ev;
     // stats
@@ -503,11 +504,12 @@ int ssr_solver_get_deflation(const ssr_solver *s, int c, double *out)
 // in double.  The reference runs Gram-Schmidt in the A-inner product over the
 // columns in reverse order, one SpMV, one set of inner products and one
 // host decision (drop when |w'Aw| <= tol, keep the sign) per column.  Here
-// the Gram matrix G = W'AW is formed on the device in one pass (k SpMVs and
-// multi-dots, one download); the same reverse-order process then runs on the
-// host on coefficient vectors (each accepted column is W t_c, and
-// (W t_c)'A(W e_j) = t_c'G e_j), and the new basis W T is formed on the device
-// in one pass.  Same mathematics, one host round trip instead of k.
+// the same process runs on coefficient vectors: AW = A W (multi-vector SpMV,
+// K read once per 8 columns), the Gram matrix G = W'AW (one tiled pass, one
+// download), the reverse-order decisions on the host (an accepted column is
+// W t_c, and (W t_c)'A(W e_j) = t_c'G e_j; t_c is supported on indices >= the
+// column it started from), and the new basis W T (one tiled pass).  Same
+// mathematics, one host round trip instead of k.
 int ssr_solver_A_orthogonalize(ssr_solver *s)
 {
     SSR_REQUIRE(s && s->K, "A_orthogonalize: operator not set");
@@ -515,47 +517,58 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
     if (k == 0) return SSR_OK;
     const int64_t n = s->n;
     int rc;
-    DevBuf<double> Gd, Td, Wn;
-    if ((rc = Gd.alloc((int64_t)k * k))) return rc;
-    for (int j = 0; j < k; ++j) {
-        if ((rc = spmv<double>(s, s->W.p + (int64_t)j * n, s->t1.p))) return rc;  // A w_j
-        for (int c0 = 0; c0 < k; c0 += 128) {
-            const int kk = k - c0 < 128 ? k - c0 : 128;
-            if ((rc = ssr::multidot<double>(n, kk, s->W.p + (int64_t)c0 * n, n, s->t1.p, Gd.p + (int64_t)j * k + c0)))
-                return rc;
-        }
+    const int64_t wl = ssr::gram_work_len(n, k);
+    if ((rc = s->AW.alloc(n * (int64_t)s->capW)) || (rc = s->Gd.alloc((int64_t)k * k)) ||
+        (rc = s->gwork.alloc(wl)))
+        return rc;
+    for (int c0 = 0; c0 < k; c0 += 8) {
+        const int nv = k - c0 < 8 ? k - c0 : 8;
+        if ((rc = ssr::spmm_bsr<double>(s->dim, s->n_n, s->row_ptr, s->col, s->K, s->W.p + (int64_t)c0 * n, n,
+                                        s->AW.p + (int64_t)c0 * n, n, nv)))
+            return rc;
+        s->n_spmv += nv;
     }
+    if ((rc = ssr::gram(n, k, s->W.p, s->AW.p, s->Gd.p, s->gwork.p, wl))) return rc;
     std::vector<double> G((size_t)k * k);  // column j = W'A w_j
-    SSR_HIP_CHECK(hipMemcpyAsync(G.data(), Gd.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, ssr_stream()));
+    SSR_HIP_CHECK(hipMemcpyAsync(G.data(), s->Gd.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, ssr_stream()));
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
-    std::vector<std::vector<double>> T;  // accepted coefficient vectors, in acceptance order
-    std::vector<double> sgn;
-    std::vector<double> u(k), tg(k);
+    // accepted coefficient vectors, in acceptance order, as rows of Tm (k
+    // wide) with their first nonzero index lo[c]
+    std::vector<double> Tm, sgn, u(k), tg;
+    std::vector<int> lo;
+    Tm.reserve((size_t)k * 16);
     for (int i = 0; i < k; ++i) {
         const int j = k - 1 - i;
         const double *Gj = G.data() + (size_t)j * k;
-        std::fill(u.begin(), u.end(), 0.0);
-        u[j] = 1.0;
-        // coefficients against every accepted column first (classical form, as
-        // the reference subtracts W_orth * (A w)'W_orth in one product)
-        for (size_t c = 0; c < T.size(); ++c) {
+        const size_t acc = sgn.size();
+        // coefficients against every accepted column first (classical form,
+        // as the reference subtracts W_orth * (A w)'W_orth in one product)
+        tg.resize(acc);
+        for (size_t c = 0; c < acc; ++c) {
+            const double *t = Tm.data() + c * k;
             double d = 0.0;
-            for (int q = 0; q < k; ++q) d += T[c][q] * Gj[q];
+            for (int q = lo[c]; q < k; ++q) d += t[q] * Gj[q];
             tg[c] = sgn[c] * d;
         }
-        for (size_t c = 0; c < T.size(); ++c)
-            for (int q = 0; q < k; ++q) u[q] -= tg[c] * T[c][q];
+        std::fill(u.begin() + j, u.end(), 0.0);
+        u[j] = 1.0;
+        for (size_t c = 0; c < acc; ++c) {
+            const double *t = Tm.data() + c * k;
+            const double g = tg[c];
+            for (int q = lo[c]; q < k; ++q) u[q] -= g * t[q];
+        }
         double nt = 0.0;  // (w - W_orth h)' A w
-        for (int q = 0; q < k; ++q) nt += u[q] * Gj[q];
+        for (int q = j; q < k; ++q) nt += u[q] * Gj[q];
         if (fabs(nt) > s->defl_tol) {
-            const double sg = nt > 0 ? 1.0 : -1.0;
             const double sc = 1.0 / sqrt(fabs(nt));
-            for (int q = 0; q < k; ++q) u[q] *= sc;
-            T.push_back(u);
-            sgn.push_back(sg);
+            const size_t off = Tm.size();
+            Tm.resize(off + k, 0.0);
+            for (int q = j; q < k; ++q) Tm[off + q] = u[q] * sc;
+            sgn.push_back(nt > 0 ? 1.0 : -1.0);
+            lo.push_back(j);
         }
     }
-    const int acc = (int)T.size();
+    const int acc = (int)sgn.size();
     if (acc == 0) {
         s->kW = 0;
         s->kW32 = -1;
@@ -564,18 +577,14 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
     // W_new(:, c) = W * T[acc-1-c]  (accepted columns back in forward order)
     std::vector<double> Th((size_t)k * acc);
     for (int c = 0; c < acc; ++c)
-        for (int q = 0; q < k; ++q) Th[(size_t)c * k + q] = T[acc - 1 - c][q];
-    if ((rc = Td.alloc((int64_t)k * acc)) || (rc = Wn.alloc(n * (int64_t)acc))) return rc;
-    SSR_HIP_CHECK(hipMemcpyAsync(Td.p, Th.data(), sizeof(double) * Th.size(), hipMemcpyHostToDevice, ssr_stream()));
-    for (int c = 0; c < acc; ++c)
-        for (int c0 = 0; c0 < k; c0 += 128) {
-            const int kk = k - c0 < 128 ? k - c0 : 128;
-            if ((rc = ssr::gemv_update<double, double>(n, kk, s->W.p + (int64_t)c0 * n, n, Td.p + (size_t)c * k + c0,
-                                                       c0 == 0 ? 0.0 : 1.0, 1.0, Wn.p + (int64_t)c * n)))
-                return rc;
-        }
-    SSR_HIP_CHECK(hipMemcpyAsync(s->W.p, Wn.p, sizeof(double) * n * acc, hipMemcpyDeviceToDevice, ssr_stream()));
-    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+        std::copy(Tm.begin() + (size_t)(acc - 1 - c) * k, Tm.begin() + (size_t)(acc - c) * k,
+                  Th.begin() + (size_t)c * k);
+    if ((rc = s->Td.alloc((int64_t)k * acc)) || (rc = s->W2.alloc(n * (int64_t)s->capW))) return rc;
+    SSR_HIP_CHECK(hipMemcpyAsync(s->Td.p, Th.data(), sizeof(double) * Th.size(), hipMemcpyHostToDevice, ssr_stream()));
+    if ((rc = ssr::gemm_tall(n, k, acc, s->W.p, s->Td.p, s->W2.p))) return rc;
+    std::swap(s->W.p, s->W2.p);
+    std::swap(s->W.n, s->W2.n);
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));  // Th leaves scope
     s->kW = acc;
     s->kW32 = -1;
     return SSR_OK;

@@ -157,6 +157,11 @@ class DFGMRES:
         self.set_operator(K)
         call("ssr_solver_A_orthogonalize", self.handle)
         self._t("DFGMRES.A_orthogonalize", t0)
+        if self.counters is not None:
+            k = self.deflation_size
+            c = self.counters
+            c["DFGMRES.deflation_size.max"] = max(c.get("DFGMRES.deflation_size.max", 0), k)
+            c["DFGMRES.deflation_size.sum"] = c.get("DFGMRES.deflation_size.sum", 0) + k
 
     def deflation_basis(self):
         """Host copy of the basis, columns over all DOFs."""

@@ -16,11 +16,11 @@ from slope_stability_octave_amd.linear_solvers import DCG, DFGMRES, dot, spmv
 pytestmark = pytest.mark.gpu
 
 
-def operator(dim=2, r=1.0, h=2.0):
+def operator(dim=2, r=1.0, h=2.0, counts=(1, 1, 1, 1, 1, 1)):
     if dim == 2:
         prob = problems.slope_2d(h=h)
     else:
-        prob = problems.slope_3d((1, 1, 1, 1, 1, 1), 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, problems.HOMO_2D)
+        prob = problems.slope_3d(counts, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, problems.HOMO_2D)
     cmb = problems.make_constitutive(prob)
     cmb.reduction(1.0)
     U = DeviceArray((cmb.n_n, cmb.dim), zero=True)
@@ -102,6 +102,31 @@ def test_A_orthogonalize_and_deflated_solve():
     x, it = s.solve(K, b)
     assert it <= 1
     assert np.linalg.norm(x.get().ravel()[qf] - vecs[2]) <= 1e-6 * np.linalg.norm(vecs[2])
+
+
+def test_A_orthogonalize_large_basis_matches_oracle():
+    """72 columns (two dependent) on the 3D operator: two 64-column tiles of
+    the Gram product, a partial 8-column group of the multi-vector SpMV and
+    several row chunks; the basis must equal the oracle's column for column,
+    and a second pass must leave an A-orthonormal basis unchanged."""
+    from oracle import krylov_oracle as KO
+    cmb, K, Kq, qf = operator(3, counts=(2, 2, 2, 2, 2, 2))
+    rng = np.random.default_rng(5)
+    n = cmb.pattern.n_dof
+    Wq = rng.normal(size=(qf.size, 70))
+    Wq = np.concatenate([Wq, (Wq[:, 3] - 0.5 * Wq[:, 40])[:, None], (Wq[:, 64] + Wq[:, 65])[:, None]], axis=1)
+    s = DFGMRES(cmb.pattern, 1e-8, 100, 1e-3)
+    s.setup_preconditioner(K)
+    for c in range(Wq.shape[1]):
+        s.expand_deflation_basis(DeviceArray.from_host(_full(Wq[:, c], qf, n, 3)))
+    s.A_orthogonalize(K)
+    Wb = s.deflation_basis()[qf]
+    Wo, sg = KO.a_orthogonalize(Wq, Kq, 1e-3)
+    assert Wo.shape == (qf.size, 70) and Wb.shape == Wo.shape
+    assert np.abs(Wb - Wo).max() <= 1e-9 * np.abs(Wo).max()
+    s.A_orthogonalize(K)
+    Wb2 = s.deflation_basis()[qf]
+    assert np.abs(np.abs(Wb2) - np.abs(Wb)).max() <= 1e-9 * np.abs(Wb).max()
 
 
 def test_fgmres_relative_tolerance_stop():
