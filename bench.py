@@ -111,11 +111,7 @@ def run_fos(args):
     if args.linear_tol is not None:
         params["linear_tol"] = args.linear_tol
     log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
-    amg_opts = {"rebuild_every": args.amg_rebuild_every} if args.amg_rebuild_every is not None else None
-    for key, val in (("theta", args.amg_theta), ("cheb_degree", args.amg_cheb_degree),
-                     ("max_coarse", args.amg_max_coarse)):
-        if val is not None:
-            amg_opts = dict(amg_opts or {}, **{key: val})
+    amg_opts = amg_options(args)
     res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts,
                            precision=args.precision)
     out = {
@@ -131,7 +127,7 @@ def run_fos(args):
                    "free_dof": prob["n_dof_free"], "int_points": prob["n_int"],
                    "lambda_init": params["lambda_init"], "omega_max_stop": params["omega_max_stop"],
                    "preconditioner": args.precond, "krylov_precision": args.precision,
-                   "linear_tol": params["linear_tol"]},
+                   "linear_tol": params["linear_tol"], "amg_options": dict(amg_opts or {})},
         "FoS": res["FoS"],
         "steps": len(res["lambda_hist"]),
         "newton_iterations": res["newton_iterations"],
@@ -199,6 +195,17 @@ def run_learned(args):
     print(json.dumps(out))
 
 
+def amg_options(args):
+    """AMG options named on the command line (None: the defaults)."""
+    opts = None
+    for key, val in (("rebuild_every", args.amg_rebuild_every), ("theta", args.amg_theta),
+                     ("cheb_degree", args.amg_cheb_degree), ("max_coarse", args.amg_max_coarse),
+                     ("reuse_hierarchy", False if args.amg_per_step else None)):
+        if val is not None:
+            opts = dict(opts or {}, **{key: val})
+    return opts
+
+
 def run_ll(args):
     """Time to the limit load of the LL continuation (2D homogeneous slope,
     slope_stability_2D_homo_LL.m), plain GPU form."""
@@ -207,8 +214,9 @@ def run_ll(args):
     _lib.require_gpu()
     prob = problems.slope_2d(h=args.h)
     log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
+    amg_opts = amg_options(args)
     res = problems.run_ll(prob, {"linear_tol": args.linear_tol or 1e-2}, preconditioner=args.precond, log=log,
-                          precision=args.precision)
+                          precision=args.precision, amg_options=amg_opts)
     out = {
         "metric": "LL time to limit load",
         "value": res["time_to_limit_s"],
@@ -217,13 +225,15 @@ def run_ll(args):
         "n_gpus": 1,
         "dtype": "f64" if args.precision == 64 else "f64 (Newton) / f32 (Krylov, AMG)",
         "data": "synthetic",
-        "config": {"workload": f"2D homogeneous slope LL, P2 triangles h={args.h}", "free_dof": prob["n_dof_free"]},
+        "config": {"workload": f"2D homogeneous slope LL, P2 triangles h={args.h}", "free_dof": prob["n_dof_free"],
+                   "amg_options": dict(amg_opts or {})},
         "limit_load": res["limit_load"],
         "t_hist": [float(x) for x in res["t_hist"]],
         "omega_hist": [float(x) for x in res["omega_hist"]],
         "newton_iterations": res["newton_iterations"],
         "setup_s": res["setup_s"],
         "timings_s": res["timings"],
+        "counts": res["counts"],
     }
     print(json.dumps(out))
 
@@ -292,6 +302,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--amg-block-smooth", type=int, default=None, choices=[0, 1],
                     help="finest-level Chebyshev scaled by node-block (1) or scalar (0) diagonal inverses")
+    ap.add_argument("--amg-per-step", action="store_true",
+                    help="a new AMG hierarchy for every continuation step (default: one hierarchy shared by all "
+                         "steps, rebuilt every --amg-rebuild-every setups)")
     ap.add_argument("--no-amg-graph", action="store_true",
                     help="launch the V-cycle kernel by kernel instead of replaying its captured graph")
     ap.add_argument("--autotune", action="store_true",

@@ -211,6 +211,16 @@ int ssr_solver_deflation_size(const ssr_solver *s, int *k);
 int ssr_solver_clear_deflation(ssr_solver *s);
 int ssr_solver_expand_deflation(ssr_solver *s, const double *v);       /* DFGMRES.m:163-170 */
 int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src); /* DFGMRES.m:204-221 */
+/* dst uses src's preconditioner object (shared; freed with the last user).
+ * No reference counterpart: DFGMRES.m:204-221 gives a copy its own
+ * preconditioner, set up again on first use.  Sharing keeps one AMG hierarchy
+ * across the per-step copies of the continuation (the finest level always
+ * applies the current operator).  Calls on dst and src must not overlap. */
+int ssr_solver_share_preconditioner(ssr_solver *dst, const ssr_solver *src);
+/* The next setup_precond of s rebuilds its preconditioner in full (AMG: new
+ * aggregates, prolongators and Galerkin levels).  No reference counterpart:
+ * the reference rebuilds on every update (hypre_boomeramg_mex.cpp:1151-1211). */
+int ssr_solver_invalidate_preconditioner(ssr_solver *s);
 int ssr_solver_get_deflation(const ssr_solver *s, int c, double *out);
 int ssr_solver_A_orthogonalize(ssr_solver *s);                         /* A_orthogonalize.m */
 int ssr_solver_dfgmres(ssr_solver *s, const double *b, double *x, int *iters, double *relres);

@@ -7,8 +7,12 @@ import numpy as np
 
 from ._lib import call, ptr
 
+# reuse_hierarchy: the per-step solver copies of a continuation share one AMG
+# object, so the hierarchy is built once (then every rebuild_every setups);
+# the finest level always applies the current tangent; a failed Newton solve
+# of the continuation asks for a rebuild before its retry.
 DEFAULTS = dict(theta=0.0, max_levels=12, max_coarse=600, cheb_degree=2, cheb_lower=1.0 / 30.0, cheb_upper=1.1,
-                refresh_every=0, rebuild_every=0)
+                refresh_every=0, rebuild_every=0, reuse_hierarchy=True)
 
 
 def rigid_body_modes(coord):

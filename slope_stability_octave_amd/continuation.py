@@ -30,6 +30,15 @@ class ContinuationError(RuntimeError):
     pass
 
 
+
+def _rebuild_after_failure(solver):
+    """A failed Newton solve asks for a fresh preconditioner before the retry:
+    with one AMG hierarchy shared by the steps' solver copies, the retry's
+    first setup rebuilds it from the current tangent."""
+    inv = getattr(solver, "invalidate_preconditioner", None)
+    if inv is not None:
+        inv()
+
 def _umax(U):
     u = U.get()
     return float(np.sqrt((u * u).sum(axis=1)).max())
@@ -146,6 +155,7 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
         if flag == 1:
             if log:
                 log(f"  retry: reducing d_omega from {d_omega:.6g} to {d_omega / 2:.6g}")
+            _rebuild_after_failure(linear_system_solver)
             d_omega = d_omega / 2
             n_omega += 1
         else:
@@ -235,6 +245,7 @@ def init_phase_SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min,
                                                                 it_damp_max, tol, r_min, K_elast, Q, f, cmb,
                                                                 solver.copy(), ws, log)
         if flag == 1:
+            _rebuild_after_failure(solver)
             d_lambda = d_lambda / 2
         else:
             U2, omega2, lambda2 = U_it, omega_it, lambda_it
@@ -283,6 +294,8 @@ def SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_d
         if flag == 1 or d_omega_test < 0:
             if log:
                 log(f"  retry: reducing d_lambda from {d_lambda:.6g} to {d_lambda / 2:.6g}")
+            if flag == 1:
+                _rebuild_after_failure(linear_system_solver)
             d_lambda = d_lambda / 2
         else:
             U, omega, lam = U_it, omega_it, lambda_it
@@ -354,6 +367,7 @@ def LL_indirect_continuation(d_omega_ini, d_t_min, step_max, omega_max, it_newt_
         if flag == 1:
             if log:
                 log(f"  retry: reducing d_omega from {d_omega:.6g} to {d_omega / 2:.6g}")
+            _rebuild_after_failure(linear_system_solver)
             d_omega = d_omega / 2
             n_omega += 1
         else:

@@ -194,7 +194,7 @@ int dupload(T **d, const std::vector<T> &h)
 
 template <class T>
 struct DCsr {
-    int64_t nr = 0;
+    int64_t nr = 0, nnz = 0;
     int64_t *rp = nullptr;
     int32_t *ci = nullptr;
     T *v = nullptr;
@@ -207,6 +207,7 @@ struct DCsr {
     int upload(const HCsr &A)
     {
         nr = A.nr;
+        nnz = (int64_t)A.ci.size();
         int rc;
         if ((rc = dupload(&rp, A.rp)) || (rc = dupload(&ci, A.ci))) return rc;
         std::vector<T> vt(A.v.begin(), A.v.end());
@@ -214,7 +215,7 @@ struct DCsr {
     }
     int spmv(const T *x, T *y, double a, double b, const T *z = nullptr) const
     {
-        return ssr::spmv_csr<T>(nr, rp, ci, v, x, y, a, b, z);
+        return ssr::spmv_csr<T>(nr, rp, ci, v, x, y, a, b, z, nnz);
     }
 };
 
@@ -323,6 +324,7 @@ struct Amg : ssr_precond {
     int64_t n_setup = 0;
     int64_t generation = 0;  // bumped whenever device level data is (re)uploaded
     bool built = false;
+    bool force_rebuild = false;  // set by invalidate(): rebuild at the next setup
 
     Amg(const ssr_pattern *P, const double *near_null, int n_null, const Options &o) : Pat(P), opt(o)
     {
@@ -775,6 +777,8 @@ struct Amg : ssr_precond {
         return dupload(&d_coarse_inv32, ci32);
     }
 
+    void invalidate() override { force_rebuild = true; }
+
     int setup(const double *Kvals, const float *K32) override
     {
         // The finest level always applies the current operator (Kvals / K32
@@ -784,7 +788,9 @@ struct Amg : ssr_precond {
         Kdev32 = K32;
         const bool need32 = K32 != nullptr && !want32;
         if (K32) want32 = true;
-        const bool rebuild = !built || (opt.rebuild_every > 0 && n_setup % opt.rebuild_every == 0);
+        const bool rebuild =
+            !built || force_rebuild || (opt.rebuild_every > 0 && n_setup % opt.rebuild_every == 0);
+        force_rebuild = false;
         const bool refresh_now = !rebuild && opt.refresh_every > 0 && n_setup % opt.refresh_every == 0;
         int rc = SSR_OK;
         if (rebuild || refresh_now) {

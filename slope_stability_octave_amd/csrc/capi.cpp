@@ -309,7 +309,7 @@ extern "C" int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, 
     return ssr_dtoh(V, dV.p, sizeof(double) * nnz_out);
 }
 
-static int g_tuning[SSR_TUNE_COUNT] = {11, 8, 0, 1, 0, 0, 0, 0};
+static int g_tuning[SSR_TUNE_COUNT] = {11, 0, 0, 1, 0, 0, 0, 0};
 
 int ssr_tuning(int key) { return (key >= 0 && key < SSR_TUNE_COUNT) ? g_tuning[key] : 0; }
 
@@ -318,7 +318,8 @@ extern "C" int ssr_set_tuning(int key, int value)
 {
     SSR_REQUIRE(key >= 0 && key < SSR_TUNE_COUNT, "set_tuning: unknown key %d", key);
     if (key == SSR_TUNE_SPMV_GROUP)
-        SSR_REQUIRE(value == 4 || value == 8 || value == 16, "set_tuning: SpMV group must be 4, 8 or 16");
+        SSR_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64,
+                    "set_tuning: SpMV group must be 0 (by dimension), 4, 8, 16, 32 or 64");
     g_tuning[key] = value;
     return SSR_OK;
 }

@@ -85,6 +85,47 @@ __global__ __launch_bounds__(256) void k_spmv_bsr(int64_t n_rows, const int32_t 
     }
 }
 
+// The same product with one wavefront per block row and lanes over the
+// row's values: lane l owns entry q = l mod dim^2 of block l / dim^2 in each
+// pass, so every pass reads 64 / dim^2 consecutive blocks (63 of 64 lanes in
+// 3D) as one contiguous span, and each lane keeps a single partial sum of row
+// entry q / dim.  Three masked wavefront sums give the dim outputs.
+template <class T>
+__device__ __forceinline__ T wave_sum_t(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int DIM, class T>
+__global__ __launch_bounds__(256) void k_spmv_bsr_w(int64_t n_rows, const int32_t *__restrict__ row_ptr,
+                                                    const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                    const T *__restrict__ x, T *y, double alpha, double beta,
+                                                    const T *z)
+{
+    constexpr int DD = DIM * DIM, BPW = 64 / DD, ACT = BPW * DD;
+    const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
+    const int lb = l / DD, q = l - lb * DD, i = q / DIM, j = q - i * DIM;
+    T acc = T(0);
+    if (r < n_rows && l < ACT) {
+        const int32_t b0 = row_ptr[r], b1 = row_ptr[r + 1];
+        for (int32_t b = b0 + lb; b < b1; b += BPW) acc += val[(int64_t)b * DD + q] * x[(int64_t)col[b] * DIM + j];
+    }
+    T o[DIM];
+#pragma unroll
+    for (int ii = 0; ii < DIM; ++ii) o[ii] = wave_sum_t<T>(i == ii ? acc : T(0));
+    if (r < n_rows && l < DIM) {
+        T v = o[0];
+#pragma unroll
+        for (int ii = 1; ii < DIM; ++ii)
+            if (l == ii) v = o[ii];
+        const T a = T(alpha) * v;
+        y[r * DIM + l] = beta == 0.0 ? a : a + T(beta) * z[r * DIM + l];
+    }
+}
+
 // Scalar CSR, G lanes per row (rows of ~10-300 entries).
 template <int G, class T>
 __global__ __launch_bounds__(256) void k_spmv_csr(int64_t n_rows, const int64_t *__restrict__ rp,
@@ -133,26 +174,75 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_partials(int nb, int k, 
     if (threadIdx.x == 0) out[c] = s;
 }
 
-// Multi-dot h = V(:,0:k)' u, columns stride ld.  Columns in chunks of 8.
+// Sums of N per-lane values over the wavefront (N a power of two <= 64) in
+// N-1 + 6-log2(N) shuffles: log2(N) halving exchanges (lanes xor 32, 16, ...)
+// leave each lane with one of the N values summed over its exchange partners,
+// selected by its high lane bits; the remaining xor steps finish the sum.
+// Lane l then holds the total of value md_index<N>(l).
+template <int N>
+__device__ __forceinline__ int md_index(int l)
+{
+    int c = 0;
+#pragma unroll
+    for (int m = 32, b = N / 2; b > 0; m >>= 1, b >>= 1) c += ((l & m) ? b : 0);
+    return c;
+}
+
+template <int N>
+__device__ __forceinline__ double wave_sum_n(double (&s)[N], int l)
+{
+#pragma unroll
+    for (int m = 32, h = N / 2; h > 0; m >>= 1, h >>= 1) {
+        const bool up = (l & m) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            const double send = up ? s[j] : s[j + h];
+            s[j] = (up ? s[j + h] : s[j]) + __shfl_xor(send, m, 64);
+        }
+    }
+    double v = s[0];
+#pragma unroll
+    for (int m = 32 / N; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Multi-dot h = V(:,0:k)' u, columns stride ld, k <= 128.  Columns in
+// passes of kMdCols; each wavefront reduces its pass sums by shuffles into
+// LDS with no block barrier, and one barrier at the end combines the four
+// wavefronts' sums per column into this block's partial.  Passes of 8
+// columns measured faster than 16; for more than 32 columns the grid gives
+// each thread about one row per pass (many wavefronts in flight), for fewer a
+// 1024-block grid keeps the partials short.
+constexpr int kMdCols = 8, kMdBlocks = 8192;
+
 template <class T>
 __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int k, const T *__restrict__ V,
                                                                   int64_t ld, const T *__restrict__ u,
                                                                   double *__restrict__ part)
 {
-    __shared__ double sh[kRedThreads / 64];
-    for (int c0 = 0; c0 < k; c0 += 8) {
-        double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        const int nc = k - c0 < 8 ? k - c0 : 8;
+    __shared__ double sh[kRedThreads / 64][128];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int ci = md_index<kMdCols>(l);
+    for (int c0 = 0; c0 < k; c0 += kMdCols) {
+        double s[kMdCols];
+#pragma unroll
+        for (int c = 0; c < kMdCols; ++c) s[c] = 0.0;
+        const int nc = k - c0 < kMdCols ? k - c0 : kMdCols;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
             const double ui = (double)u[i];
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
+            for (int c = 0; c < kMdCols; ++c)
                 if (c < nc) s[c] += (double)V[(int64_t)(c0 + c) * ld + i] * ui;
         }
-        for (int c = 0; c < nc; ++c) {
-            const double t = block_sum(s[c], sh);
-            if (threadIdx.x == 0) part[(int64_t)(c0 + c) * gridDim.x + blockIdx.x] = t;
-        }
+        const double t = wave_sum_n<kMdCols>(s, l);
+        if ((l & (64 / kMdCols - 1)) == 0 && ci < nc) sh[w][c0 + ci] = t;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < k; c += blockDim.x) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < kRedThreads / 64; ++q) t += sh[q][c];
+        part[(int64_t)c * gridDim.x + blockIdx.x] = t;
     }
 }
 
@@ -511,7 +601,7 @@ __global__ void k_ewmul(int64_t n, const double *__restrict__ a, const double *_
 // Reduction scratch per stream, so that threads running independent
 // problems on their own streams never share partial sums.
 struct RedScratch {
-    double *part = nullptr;  // kRedBlocks * 128 partial sums
+    double *part = nullptr;  // kMdBlocks * 128 partial sums
     double *red = nullptr;   // 128 reduced values
 };
 std::mutex g_scr_mu;
@@ -522,7 +612,7 @@ int scratch(RedScratch **out)
     const hipStream_t st = ssr_stream();
     std::lock_guard<std::mutex> lk(g_scr_mu);
     RedScratch &r = g_scr[st];  // element references survive rehashing
-    if (!r.part) SSR_HIP_CHECK(hipMalloc((void **)&r.part, sizeof(double) * kRedBlocks * 128));
+    if (!r.part) SSR_HIP_CHECK(hipMalloc((void **)&r.part, sizeof(double) * kMdBlocks * 128));
     if (!r.red) SSR_HIP_CHECK(hipMalloc((void **)&r.red, sizeof(double) * 128));
     *out = &r;
     return SSR_OK;
@@ -552,18 +642,29 @@ int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
 {
     if (n_rows == 0) return SSR_OK;
     if (!z) z = y;
-    const int G = ssr_tuning(SSR_TUNE_SPMV_GROUP);
+    int G = ssr_tuning(SSR_TUNE_SPMV_GROUP);
+    if (G == 0) G = dim == 3 ? 32 : 4;  // measured best for P2 node-block rows (about 28 blocks in 3D, 11 in 2D)
     const int64_t threads = n_rows * G;
     const dim3 grid(ssr_blocks(threads, 256)), blk(256);
 #define SPMV(D_, G_) hipLaunchKernelGGL((k_spmv_bsr<D_, G_, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, \
                                         val, x, y, alpha, beta, z)
-    if (dim == 3) {
+    if (G == 64) {
+        const dim3 gw(ssr_blocks(n_rows * 64, 256));
+        if (dim == 3)
+            hipLaunchKernelGGL((k_spmv_bsr_w<3, T>), gw, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, x, y, alpha,
+                               beta, z);
+        else
+            hipLaunchKernelGGL((k_spmv_bsr_w<2, T>), gw, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, x, y, alpha,
+                               beta, z);
+    } else if (dim == 3) {
         if (G == 4) SPMV(3, 4);
         else if (G == 16) SPMV(3, 16);
+        else if (G == 32) SPMV(3, 32);
         else SPMV(3, 8);
     } else {
         if (G == 4) SPMV(2, 4);
         else if (G == 16) SPMV(2, 16);
+        else if (G == 32) SPMV(2, 32);
         else SPMV(2, 8);
     }
 #undef SPMV
@@ -573,13 +674,22 @@ int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
 
 template <class T>
 int spmv_csr(int64_t n_rows, const int64_t *rp, const int32_t *ci, const T *v, const T *x, T *y, double alpha,
-             double beta, const T *z)
+             double beta, const T *z, int64_t nnz)
 {
     if (n_rows == 0) return SSR_OK;
     if (!z) z = y;
-    constexpr int G = 8;
-    hipLaunchKernelGGL((k_spmv_csr<G, T>), dim3(ssr_blocks(n_rows * G, 256)), dim3(256), 0, ssr_stream(), n_rows, rp,
-                       ci, v, x, y, alpha, beta, z);
+    // lanes per row from the average row length (8 when unknown): the
+    // Galerkin coarse levels and the restriction have rows of hundreds
+    const int64_t avg = nnz < 0 ? 24 : nnz / n_rows;
+    const dim3 blk(256);
+#define CSR(G_) hipLaunchKernelGGL((k_spmv_csr<G_, T>), dim3(ssr_blocks(n_rows * G_, 256)), blk, 0, ssr_stream(), \
+                                   n_rows, rp, ci, v, x, y, alpha, beta, z)
+    if (avg <= 12) CSR(4);
+    else if (avg <= 48) CSR(8);
+    else if (avg <= 96) CSR(16);
+    else if (avg <= 192) CSR(32);
+    else CSR(64);
+#undef CSR
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }
@@ -605,7 +715,9 @@ int multidot(int64_t n, int k, const T *V, int64_t ld, const T *u, double *h)
     RedScratch *sc = nullptr;
     int rc = scratch(&sc);
     if (rc) return rc;
-    const unsigned nb = red_blocks(n);
+    const int64_t cap = k <= 32 ? kRedBlocks : kMdBlocks;
+    const int64_t nbl = (n + kRedThreads - 1) / kRedThreads;
+    const unsigned nb = (unsigned)(nbl < 1 ? 1 : (nbl > cap ? cap : nbl));
     hipLaunchKernelGGL((k_multidot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
                        sc->part);
     hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, sc->part, h);
@@ -787,7 +899,7 @@ int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double 
     template int spmv_bsr<T>(int, int64_t, const int32_t *, const int32_t *, const T *, const T *, T *, double,       \
                              double, const T *);                                                                     \
     template int spmv_csr<T>(int64_t, const int64_t *, const int32_t *, const T *, const T *, T *, double, double,   \
-                             const T *);                                                                             \
+                             const T *, int64_t);                                                                    \
     template int dot<T>(int64_t, const T *, const T *, const double *, double *);                                    \
     template int multidot<T>(int64_t, int, const T *, int64_t, const T *, double *);                                 \
     template int axpby<T>(int64_t, double, const T *, double, T *);                                                   \

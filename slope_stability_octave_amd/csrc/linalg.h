@@ -22,10 +22,11 @@ int64_t gram_work_len(int64_t n, int k);
 int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *work, int64_t work_len);
 // Y (n x m) = X (n x k) C (k x m), all column-major (ld n, k, n)
 int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y);
-// y = alpha*A*x + beta*z on scalar CSR (64-bit row pointers)
+// y = alpha*A*x + beta*z on scalar CSR (64-bit row pointers); nnz (when
+// known) picks the lanes per row
 template <class T>
 int spmv_csr(int64_t n_rows, const int64_t *rp, const int32_t *ci, const T *v, const T *x, T *y, double alpha,
-             double beta, const T *z = nullptr);
+             double beta, const T *z = nullptr, int64_t nnz = -1);
 // result_dev[0] = sum a_i b_i (m_i)
 template <class T>
 int dot(int64_t n, const T *a, const T *b, const double *mask, double *result_dev);

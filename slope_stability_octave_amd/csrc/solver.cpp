@@ -37,6 +37,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "linalg.h"
@@ -131,8 +132,8 @@ struct ssr_solver {
     double tol = 1e-1, defl_tol = 1e-3;
     int precision = 64;    // 64: all double; 32: float Krylov/preconditioner + double refinement
     int max_refine = 8;    // float solves per call in precision 32
-    ssr_precond *M = nullptr;
-    bool own_M = true;
+    std::shared_ptr<ssr_precond> Msp;  // may be shared with copies of this solver
+    ssr_precond *M = nullptr;          // = Msp.get()
     // deflation basis (columns of length n) and its float mirror
     DevBuf<double> W;
     int kW = 0, capW = 0;
@@ -370,7 +371,8 @@ int ssr_solver_create(const ssr_pattern *P, int maxit, double tol, double defl_t
     s->tol = tol;
     s->defl_tol = defl_tol;
     if (precond_type == 0) {
-        s->M = new BlockJacobi(P);
+        s->Msp.reset(new BlockJacobi(P));
+        s->M = s->Msp.get();
     } else {
         delete s;
         ssr_set_error("solver_create: unknown preconditioner %d", precond_type);
@@ -389,7 +391,6 @@ int ssr_solver_create(const ssr_pattern *P, int maxit, double tol, double defl_t
 int ssr_solver_destroy(ssr_solver *s)
 {
     if (!s) return SSR_OK;
-    if (s->own_M) delete s->M;
     delete s;
     return SSR_OK;
 }
@@ -398,9 +399,27 @@ int ssr_solver_destroy(ssr_solver *s)
 int ssr_solver_set_precond(ssr_solver *s, ssr_precond *M)
 {
     SSR_REQUIRE(s && M, "solver_set_precond: null");
-    if (s->own_M) delete s->M;
+    s->Msp.reset(M);
     s->M = M;
-    s->own_M = true;
+    return SSR_OK;
+}
+
+// The next setup of the solver's preconditioner rebuilds it in full.
+int ssr_solver_invalidate_preconditioner(ssr_solver *s)
+{
+    SSR_REQUIRE(s && s->M, "invalidate_preconditioner: null");
+    s->M->invalidate();
+    return SSR_OK;
+}
+
+// dst applies (and sets up) src's preconditioner object from now on; the
+// object lives until the last solver using it is destroyed.  Calls on the
+// two solvers must not overlap.
+int ssr_solver_share_preconditioner(ssr_solver *dst, const ssr_solver *src)
+{
+    SSR_REQUIRE(dst && src && dst->P == src->P, "share_preconditioner: solvers on different patterns");
+    dst->Msp = src->Msp;
+    dst->M = dst->Msp.get();
     return SSR_OK;
 }
 
@@ -629,9 +648,22 @@ int ssr_solver_dfgmres(ssr_solver *s, const double *b, double *x, int *iters, do
             return rc;
         if ((rc = ensure_w32(s))) return rc;
         while (it_total < m && refine < s->max_refine) {
+            if (refine > 0 && s->kW > 0) {
+                // The projected Krylov space has no component in span(W), so a
+                // restart first takes the coarse correction of its residual,
+                // as x0 = W W' b does for the first pass.
+                if ((rc = defl_apply<double>(s, s->r.p, 1.0, 1.0, x))) return rc;
+                if ((rc = spmv<double>(s, x, s->r.p, -1.0, 1.0, b))) return rc;
+                res = hnorm(s, s->r.p, &rc);
+                if (rc) return rc;
+                if (res / bnorm < s->tol) break;
+            }
             if ((rc = ssr::convert<double, float>(n, 1.0 / res, s->r.p, s->V32.p))) return rc;
-            // a float Krylov space cannot resolve much below ~1e-6 of its start
-            const double target = fmax(s->tol * bnorm, 1e-6 * res);
+            // a float Krylov space loses orthogonality well before it could
+            // resolve 1e-6 of its start with a weak preconditioner; each pass
+            // asks for 1e-4 and deeper reductions come from restarts on the
+            // double residual
+            const double target = fmax(s->tol * bnorm, 1e-4 * res);
             int it = 0;
             double est = 0.0;
             if ((rc = fgmres_core<float>(s, s->V32.p, s->Z32.p, s->t2f.p, res, target, m - it_total, x, &it, &est)))

@@ -54,11 +54,14 @@ struct ssr_precond {
     virtual int setup(const double *Kvals, const float *K32) = 0;
     virtual int apply(const double *x, double *y) = 0;  // y = M^{-1} x (device)
     virtual int apply(const float *x, float *y) = 0;    // same in float
+    // the next setup rebuilds whatever it would otherwise keep
+    virtual void invalidate() {}
 };
 
 // Kernel-variant / launch-shape knobs (set by the autotuner or the caller).
 #define SSR_TUNE_CONSTITUTIVE_VARIANT 0 /* 3D return-mapping kernel form, see constitutive.hip */
-#define SSR_TUNE_SPMV_GROUP 1           /* lanes per block row in the node-block SpMV: 4, 8, 16 */
+#define SSR_TUNE_SPMV_GROUP 1           /* lanes per block row in the node-block SpMV: 4, 8, 16, 32; 64 =
+                                           one wavefront per row, lanes over the values; 0 = by dimension */
 #define SSR_TUNE_PERSIST_WAVES 2        /* wavefronts per compute unit of persistent grids (0: auto) */
 #define SSR_TUNE_AMG_GRAPH 3            /* 1: replay the V-cycle as a captured HIP graph, 0: eager launches */
 #define SSR_TUNE_AMG_BLOCK_SMOOTH 4     /* 1: node-block Jacobi scaling in the finest-level Chebyshev smoother */

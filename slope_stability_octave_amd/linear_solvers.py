@@ -30,6 +30,8 @@ _lib.extend_signatures({
     "ssr_solver_clear_deflation": [_vp],
     "ssr_solver_expand_deflation": [_vp, _vp],
     "ssr_solver_copy_deflation": [_vp, _vp],
+    "ssr_solver_share_preconditioner": [_vp, _vp],
+    "ssr_solver_invalidate_preconditioner": [_vp],
     "ssr_solver_get_deflation": [_vp, _int, _vp],
     "ssr_solver_A_orthogonalize": [_vp],
     "ssr_solver_dfgmres": [_vp, _vp, _vp, _vp, _vp],
@@ -57,6 +59,11 @@ _lib.extend_signatures({
 })
 
 PRECONDITIONERS = {"block_jacobi": 0}
+
+
+def _amg_option(options, key):
+    from . import amg
+    return (options or {}).get(key, amg.DEFAULTS[key])
 
 
 def dot(a, b, mask=None) -> float:
@@ -138,6 +145,13 @@ class DFGMRES:
     def update_preconditioner_values(self, K):
         self.setup_preconditioner(K)
 
+    def invalidate_preconditioner(self):
+        """The next setup rebuilds the preconditioner in full (also for every
+        copy sharing it)."""
+        call("ssr_solver_invalidate_preconditioner", self.handle)
+        if self.counters is not None:
+            self.counters["AMG.rebuilds_requested"] = self.counters.get("AMG.rebuilds_requested", 0) + 1
+
     def set_operator(self, K):
         self._K = K
         call("ssr_solver_set_matrix", self.handle, ptr(K), 0)
@@ -175,10 +189,14 @@ class DFGMRES:
 
     def copy(self):
         """New solver with the same settings and a copy of the basis; the
-        preconditioner is rebuilt on the next setup (DFGMRES.m:204-221)."""
+        preconditioner is rebuilt on the next setup (DFGMRES.m:204-221), or,
+        with amg_options['reuse_hierarchy'] (the default), the copy shares
+        this solver's AMG object so one hierarchy serves every copy."""
         new = type(self)(self.pattern, self.tolerance, self.max_iterations, self.tolerance_deflation_basis,
                          self.preconditioner_name, self.verbose, self.amg_options, self.precision)
         call("ssr_solver_copy_deflation", new.handle, self.handle)
+        if self.preconditioner_name == "amg" and _amg_option(self.amg_options, "reuse_hierarchy"):
+            call("ssr_solver_share_preconditioner", new.handle, self.handle)
         new.profiler = self.profiler
         new.timings = self.timings  # shared, like the reference's profiler handle
         new.counters = self.counters
@@ -202,6 +220,8 @@ class DFGMRES:
         if self.counters is not None:
             self.counters["DFGMRES.solve.n_iters"] = self.counters.get("DFGMRES.solve.n_iters", 0) + it.value
             self.counters["DFGMRES.solve.calls"] = self.counters.get("DFGMRES.solve.calls", 0) + 1
+            if it.value >= self.max_iterations:
+                self.counters["DFGMRES.solve.at_maxit"] = self.counters.get("DFGMRES.solve.at_maxit", 0) + 1
         return x, it.value
 
     def stats(self):

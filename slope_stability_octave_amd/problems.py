@@ -250,5 +250,7 @@ def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, 
     t_run = time.perf_counter() - t1
     timings = dict(cmb.timings)
     timings.update(lin.timings)
+    counts = dict(cmb.counts)
+    counts.update(lin.counters)
     return dict(limit_load=float(t_h[-1]), t_hist=t_h, omega_hist=om_h, Umax_hist=umax_h, omega_el=omega_el, U=U,
-                setup_s=t_setup, time_to_limit_s=t_run, newton_iterations=its, timings=timings)
+                setup_s=t_setup, time_to_limit_s=t_run, newton_iterations=its, timings=timings, counts=counts)

@@ -47,7 +47,7 @@ def test_spmv_tuner():
     old = _lib.get_tuning(_lib.TUNE_SPMV_GROUP)
     try:
         rec = Autotuner().spmv(cmb.pattern, cmb.K_elast, reps=5)
-        assert rec["best"] in (4, 8, 16)
+        assert rec["best"] in (4, 8, 16, 32, 64)
         assert _lib.get_tuning(_lib.TUNE_SPMV_GROUP) == rec["best"]
     finally:
         _lib.set_tuning(_lib.TUNE_SPMV_GROUP, old)

@@ -9,7 +9,7 @@ import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from slope_stability_octave_amd import problems
+from slope_stability_octave_amd import _lib, problems
 from slope_stability_octave_amd._lib import DeviceArray
 from slope_stability_octave_amd.linear_solvers import DCG, DFGMRES, dot, spmv
 
@@ -49,14 +49,22 @@ def _full(vq, qf, n_dof, dim):
     return x.reshape(-1, dim)
 
 
+@pytest.mark.parametrize("group", [0, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_spmv_matches_scipy(dim):
-    cmb, K, Kq, qf = operator(dim)
+def test_spmv_matches_scipy(dim, group):
+    """Every SpMV form (lanes per block row; 64 = one wavefront per row with
+    lanes over the values) against scipy."""
+    cmb, K, Kq, qf = operator(dim, counts=(2, 2, 2, 2, 2, 2))
     rng = np.random.default_rng(1)
     xq = rng.normal(size=qf.size)
     x = DeviceArray.from_host(_full(xq, qf, cmb.pattern.n_dof, dim))
     y = DeviceArray((cmb.n_n, dim))
-    spmv(cmb.pattern, K, x, y)
+    old = _lib.get_tuning(_lib.TUNE_SPMV_GROUP)
+    try:
+        _lib.set_tuning(_lib.TUNE_SPMV_GROUP, group)
+        spmv(cmb.pattern, K, x, y)
+    finally:
+        _lib.set_tuning(_lib.TUNE_SPMV_GROUP, old)
     yq = y.get().ravel()[qf]
     ref = Kq @ xq
     assert np.abs(yq - ref).max() <= 1e-12 * np.abs(ref).max()
@@ -127,6 +135,26 @@ def test_A_orthogonalize_large_basis_matches_oracle():
     s.A_orthogonalize(K)
     Wb2 = s.deflation_basis()[qf]
     assert np.abs(np.abs(Wb2) - np.abs(Wb)).max() <= 1e-9 * np.abs(Wb).max()
+
+
+@pytest.mark.parametrize("k", [1, 5, 8, 13, 128])
+def test_multidot_and_gemv_update_match_numpy(k):
+    """Tall-skinny V'u (8-column passes, shuffle reduction) and u = a u + b V h
+    against numpy at a size with a partial last block."""
+    import ctypes
+    from slope_stability_octave_amd.linear_solvers import call, ptr  # noqa: F401  (binds signatures)
+    n = 100_003
+    rng = np.random.default_rng(k)
+    V = rng.normal(size=(k, n))  # k columns of length n, column stride n
+    u = rng.normal(size=n)
+    dV, du = DeviceArray.from_host(V), DeviceArray.from_host(u)
+    dh = DeviceArray(k)
+    call("ssr_dev_multidot", n, k, ptr(dV), n, ptr(du), ptr(dh))
+    ref = V @ u
+    assert np.abs(dh.get() - ref).max() <= 1e-12 * np.abs(V).max() * np.abs(u).max() * np.sqrt(n)
+    call("ssr_dev_gemv_update", n, k, ptr(dV), n, ptr(dh), ctypes.c_double(0.5), ctypes.c_double(-2.0), ptr(du))
+    ref_u = 0.5 * u - 2.0 * (V.T @ dh.get())
+    assert np.abs(du.get() - ref_u).max() <= 1e-12 * np.abs(ref_u).max() * k
 
 
 def test_fgmres_relative_tolerance_stop():
