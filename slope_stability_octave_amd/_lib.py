@@ -196,6 +196,7 @@ TUNE_SPMV_GROUP = 1
 TUNE_PERSIST_WAVES = 2
 TUNE_AMG_GRAPH = 3
 TUNE_AMG_BLOCK_SMOOTH = 4
+TUNE_ASSEMBLY_FORM = 5
 
 
 def set_tuning(key: int, value: int):

@@ -7,6 +7,9 @@ library stream and installs the fastest through ssr_set_tuning:
       9  36-wide LDS staging of DS, 32-point phases
       10 as 9 with the inputs read straight into registers
       11 as 10 with 16-point phases (smallest LDS footprint)
+      14 as 11 with each wavefront's 16-point phases spread over the array
+         (neighbouring wavefronts write neighbouring spans), non-temporal
+      15 as 14 with ordinary stores
   * the lanes per block row of the node-block SpMV (TUNE_SPMV_GROUP: 4, 8, 16, 32, or 64 for one
     wavefront per row with lanes over the values).
 Results are kept per (kind, shape) key in a JSON cache file when a path is
@@ -22,7 +25,7 @@ import numpy as np
 from . import _lib
 from ._lib import DeviceArray, EventTimer
 
-CONSTITUTIVE_CANDIDATES = (2, 9, 10, 11)
+CONSTITUTIVE_CANDIDATES = (2, 9, 10, 11, 14, 15)
 SPMV_CANDIDATES = (4, 8, 16, 32, 64)
 
 

@@ -31,6 +31,9 @@ const int32_t *ssr_pattern_d_elem(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_brow(const ssr_pattern *p);
 const int64_t *ssr_pattern_d_ref_pos(const ssr_pattern *p);
 const double *ssr_pattern_d_mask(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_cptr(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_ccode(const ssr_pattern *p);
+int ssr_pattern_keblk(const ssr_pattern *p, double **out);
 void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e,
                        int64_t *nnzb, int64_t *nnz_free);
 
@@ -76,7 +79,9 @@ __device__ __forceinline__ void atomic_add_f64(double *p, double v)
 // Tangent assembly
 // ---------------------------------------------------------------------------
 // DSL: 0 = MEX layout (ns*ns column-major per point), 1 = packed symmetric.
-// SCAT: 0 = node-block CSR via emap, 1 = reference int64 scatter map.
+// SCAT: 0 = node-block CSR via emap (atomic adds), 1 = reference int64
+// scatter map (atomic adds), 2 = plain stores of the element's pair blocks
+// to out[(e * NPAIR + pair) * dim^2] for the gather pass.
 template <int DIM, int NP, int NQ, int DSL, int SCAT>
 __global__ __launch_bounds__(64) void k_assemble(
     int64_t n_e, const double *__restrict__ D1, const double *__restrict__ D2,
@@ -171,7 +176,13 @@ __global__ __launch_bounds__(64) void k_assemble(
 
         const int64_t e = e0 + le;
         if (active && e < n_e) {
-            if (SCAT == 0) {
+            if (SCAT == 2) {
+                double *o = out + ((int64_t)e * NPAIR + pr) * DIM * DIM;
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) o[i * DIM + j] = acc[i][j];
+            } else if (SCAT == 0) {
                 const int32_t bab = emap[(e * NP + pa) * NP + pb];
                 double *o = out + (int64_t)bab * DIM * DIM;
 #pragma unroll
@@ -337,6 +348,28 @@ __global__ void k_ds_elastic(int64_t n, int ns, const double *__restrict__ shear
     DS[t] = 2.0 * dev * shear[p] + vol * bulk[p];
 }
 
+// Block values as the sum of their element blocks, in element order (no
+// atomics, so the result is the same on every run): dim^2 lanes per block,
+// lane q taking entry q of each contribution (entry q^T of a transposed one),
+// so a contribution is read as one contiguous 8*dim^2-byte span.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_gather_blocks(int64_t nnzb, const int32_t *__restrict__ cptr,
+                                                       const int32_t *__restrict__ code,
+                                                       const double *__restrict__ ke, double *__restrict__ out)
+{
+    constexpr int DD = DIM * DIM;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = t / DD;
+    if (b >= nnzb) return;
+    const int q = (int)(t - b * DD), i = q / DIM, j = q - i * DIM, qt = j * DIM + i;
+    double s = 0.0;
+    for (int32_t c = cptr[b]; c < cptr[b + 1]; ++c) {
+        const int32_t cd = code[c];
+        s += ke[(int64_t)(cd >> 1) * DD + ((cd & 1) ? qt : q)];
+    }
+    out[t] = s;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -354,7 +387,10 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         const unsigned grid = (unsigned)std::min<int64_t>(n_e, want * 16);
 #define L3(DSL_, SC_) hipLaunchKernelGGL((k_assemble<3, 10, 11, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
-        if (dsl == 0 && scat == 0) L3(0, 0);
+        if (scat == 2) {
+            if (dsl == 0) L3(0, 2);
+            else L3(1, 2);
+        } else if (dsl == 0 && scat == 0) L3(0, 0);
         else if (dsl == 0) L3(0, 1);
         else if (scat == 0) L3(1, 0);
         else L3(1, 1);
@@ -366,7 +402,10 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         const unsigned grid = (unsigned)std::min<int64_t>(groups, want * 16);
 #define L2(DSL_, SC_) hipLaunchKernelGGL((k_assemble<2, 6, 7, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
-        if (dsl == 0 && scat == 0) L2(0, 0);
+        if (scat == 2) {
+            if (dsl == 0) L2(0, 2);
+            else L2(1, 2);
+        } else if (dsl == 0 && scat == 0) L2(0, 0);
         else if (dsl == 0) L2(0, 1);
         else if (scat == 0) L2(1, 0);
         else L2(1, 1);
@@ -404,6 +443,23 @@ int ssr_dev_assemble_bsr(const ssr_pattern *P, int n_q, const double *DPhi1, con
     int64_t n_n, n_p, n_e, nnzb, nnzf;
     ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
     SSR_REQUIRE(ssr_pattern_d_emap(P), "assemble_bsr: pattern has no device tables");
+    if (ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) == 0 && ssr_pattern_d_cptr(P)) {
+        double *ke = nullptr;
+        int rc = ssr_pattern_keblk(P, &ke);
+        if (rc) return rc;
+        if ((rc = assemble_launch(dim, n_p, n_e, n_q, ds_layout, 2, DPhi1, DPhi2, DPhi3, DS, WEIGHT, nullptr,
+                                  nullptr, ke)))
+            return rc;
+        const int64_t threads = nnzb * dim * dim;
+        if (dim == 3)
+            hipLaunchKernelGGL(k_gather_blocks<3>, dim3(ssr_blocks(threads, 256)), dim3(256), 0, ssr_stream(), nnzb,
+                               ssr_pattern_d_cptr(P), ssr_pattern_d_ccode(P), ke, Kvals);
+        else
+            hipLaunchKernelGGL(k_gather_blocks<2>, dim3(ssr_blocks(threads, 256)), dim3(256), 0, ssr_stream(), nnzb,
+                               ssr_pattern_d_cptr(P), ssr_pattern_d_ccode(P), ke, Kvals);
+        SSR_HIP_CHECK(hipGetLastError());
+        return SSR_OK;
+    }
     SSR_HIP_CHECK(hipMemsetAsync(Kvals, 0, sizeof(double) * nnzb * dim * dim, ssr_stream()));
     return assemble_launch(dim, n_p, n_e, n_q, ds_layout, 0, DPhi1, DPhi2, DPhi3, DS, WEIGHT,
                            ssr_pattern_d_emap(P), nullptr, Kvals);

@@ -39,6 +39,11 @@ struct ssr_pattern {
             *d_diag = nullptr, *d_elem = nullptr, *d_brow = nullptr;
     int64_t *d_ref_pos = nullptr;
     double *d_mask = nullptr;  // dim*n_n, 1.0 free / 0.0 Dirichlet
+    // gather-form assembly: the element blocks that sum into each block, in
+    // element order, as code = (e * n_pair + pair) * 2 + transposed
+    std::vector<int32_t> cptr, ccode;
+    int32_t *d_cptr = nullptr, *d_ccode = nullptr;
+    mutable double *d_keblk = nullptr;  // n_e * n_pair * dim * dim element blocks (allocated on first use)
 };
 
 namespace {
@@ -67,6 +72,9 @@ int ssr_pattern_destroy(ssr_pattern *p)
     (void)hipFree(p->d_row_ptr);
     (void)hipFree(p->d_col);
     (void)hipFree(p->d_emap);
+    (void)hipFree(p->d_cptr);
+    (void)hipFree(p->d_ccode);
+    (void)hipFree(p->d_keblk);
     (void)hipFree(p->d_tblk);
     (void)hipFree(p->d_diag);
     (void)hipFree(p->d_elem);
@@ -149,6 +157,28 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
             for (int64_t b = 0; b < n_p; ++b)
                 p->emap[(e * n_p + a) * n_p + b] = find_blk(elem[e * n_p + a], (int32_t)elem[e * n_p + b]);
 
+    // contributions per block for the gather-form assembly: each element
+    // block pair (a <= b) is computed once; block (a,b) takes it as is and
+    // block (b,a) transposed.  Pair numbering: row-major upper triangle.
+    {
+        const int64_t npair = n_p * (n_p + 1) / 2;
+        if (n_e * npair * 2 < ((int64_t)1 << 31) && n_e * n_p * n_p < ((int64_t)1 << 31)) {
+            p->cptr.assign(p->nnzb + 1, 0);
+            for (int64_t k = 0; k < n_e * n_p * n_p; ++k) p->cptr[p->emap[k] + 1]++;
+            for (int64_t b = 0; b < p->nnzb; ++b) p->cptr[b + 1] += p->cptr[b];
+            p->ccode.resize(n_e * n_p * n_p);
+            std::vector<int32_t> fill(p->cptr.begin(), p->cptr.end() - 1);
+            for (int64_t e = 0; e < n_e; ++e)
+                for (int64_t a = 0; a < n_p; ++a)
+                    for (int64_t b = 0; b < n_p; ++b) {
+                        const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+                        const int64_t pr = lo * n_p - lo * (lo - 1) / 2 + (hi - lo);
+                        const int32_t blk = p->emap[(e * n_p + a) * n_p + b];
+                        p->ccode[fill[blk]++] = (int32_t)((e * npair + pr) * 2 + (a > b ? 1 : 0));
+                    }
+        }
+    }
+
     // free DOFs and the reference-order extraction map
     p->mask.assign(Q, Q + dim * n_n);
     for (auto &m : p->mask) m = m ? 1 : 0;
@@ -180,7 +210,8 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
             (rc = upload(&p->d_emap, p->emap)) || (rc = upload(&p->d_tblk, p->tblk)) ||
             (rc = upload(&p->d_diag, p->diag)) || (rc = upload(&p->d_elem, el32)) ||
             (rc = upload(&p->d_ref_pos, p->ref_pos)) || (rc = upload(&p->d_mask, md)) ||
-            (rc = upload(&p->d_brow, p->brow))) {
+            (rc = upload(&p->d_brow, p->brow)) ||
+            (!p->cptr.empty() && ((rc = upload(&p->d_cptr, p->cptr)) || (rc = upload(&p->d_ccode, p->ccode))))) {
             ssr_pattern_destroy(p);
             return rc;
         }
@@ -234,6 +265,18 @@ const int32_t *ssr_pattern_d_elem(const ssr_pattern *p) { return p->d_elem; }
 const int32_t *ssr_pattern_d_brow(const ssr_pattern *p) { return p->d_brow; }
 const int64_t *ssr_pattern_d_ref_pos(const ssr_pattern *p) { return p->d_ref_pos; }
 const double *ssr_pattern_d_mask(const ssr_pattern *p) { return p->d_mask; }
+const int32_t *ssr_pattern_d_cptr(const ssr_pattern *p) { return p->d_cptr; }
+const int32_t *ssr_pattern_d_ccode(const ssr_pattern *p) { return p->d_ccode; }
+// element-block scratch of the gather-form assembly (allocated once)
+int ssr_pattern_keblk(const ssr_pattern *p, double **out)
+{
+    if (!p->d_keblk) {
+        const int64_t len = p->n_e * (p->n_p * (p->n_p + 1) / 2) * p->dim * p->dim;
+        SSR_HIP_CHECK(hipMalloc((void **)&p->d_keblk, sizeof(double) * (len > 0 ? len : 1)));
+    }
+    *out = p->d_keblk;
+    return SSR_OK;
+}
 void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e,
                        int64_t *nnzb, int64_t *nnz_free)
 {

@@ -106,6 +106,35 @@ def test_block_assembly_regularize_extract(dim, size):
                 assert np.array_equal(Kh[b, i], expect)
 
 
+@pytest.mark.parametrize("dim,size", [(3, "small"), (2, "small"), (3, "medium"), (2, "medium")])
+def test_gather_and_atomic_assembly_forms(dim, size):
+    """The default form (element blocks, then a per-block gather in element
+    order) against the oracle and against the FP64-atomic form; the gather
+    form gives the same bits on every run."""
+    from slope_stability_octave_amd import _lib
+    p = problem(dim, size, seed=3)
+    P = A.Pattern(p["elem"], p["Q"], dim)
+    dD = [DeviceArray.from_host(x.ravel(order="F")) for x in p["DPhi"]]
+    dW = DeviceArray.from_host(p["WEIGHT"])
+    dDS = DeviceArray.from_host(np.asfortranarray(p["DS"]).T.copy())
+    out = {}
+    old = _lib.get_tuning(_lib.TUNE_ASSEMBLY_FORM)
+    try:
+        for form in (0, 1, 0):
+            _lib.set_tuning(_lib.TUNE_ASSEMBLY_FORM, form)
+            K = DeviceArray(P.nnzb * P.bs)
+            P.assemble(p["n_q"], dD, dDS, dW, K)
+            out.setdefault(form, []).append(K.get())
+    finally:
+        _lib.set_tuning(_lib.TUNE_ASSEMBLY_FORM, old)
+    g, a = out[0][0], out[1][0]
+    assert np.array_equal(out[0][0], out[0][1])
+    assert np.abs(g - a).max() <= 1e-13 * np.abs(a).max()
+    Vt = DeviceArray(P.nnz_free)
+    P.extract_ref(DeviceArray.from_host(g), Vt)
+    assert relK(Vt.get(), p["R"].global_tangent(p["DS"], p["WEIGHT"])) <= TOL_K
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_strain_and_force(dim):
     p = problem(dim, "medium")
