@@ -256,47 +256,43 @@ __global__ __launch_bounds__(256) void k_strain(int64_t n_e, const int32_t *__re
 // atomics; S has s_rows rows per point (6 in 3D; 4 in 2D where sigma33 is
 // not used), F must be zeroed first.
 template <int DIM, int NP, int NQ>
-__global__ __launch_bounds__(128) void k_force(int64_t n_e, const int32_t *__restrict__ elem,
+__global__ __launch_bounds__(256) void k_force(int64_t n_e, const int32_t *__restrict__ elem,
                                                const double *__restrict__ D1,
                                                const double *__restrict__ D2,
                                                const double *__restrict__ D3,
                                                const double *__restrict__ S, int s_rows,
                                                const double *__restrict__ W, double *__restrict__ F)
 {
+    // one thread per (element, node): the NP threads of an element read its
+    // point gradients as contiguous spans and share the stress rows
     using V = Voigt<DIM>;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = t / NP;
+    const int a = (int)(t - e * NP);
     if (e >= n_e) return;
-    double f[NP][DIM];
+    double f[DIM];
 #pragma unroll
-    for (int a = 0; a < NP; ++a)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) f[a][c] = 0.0;
+    for (int c = 0; c < DIM; ++c) f[c] = 0.0;
     for (int q = 0; q < NQ; ++q) {
         const int64_t g = e * NQ + q;
         double ws[V::NS];
 #pragma unroll
         for (int r = 0; r < V::NS; ++r) ws[r] = W[g] * S[g * s_rows + r];
+        double gr[DIM];
+        gr[0] = D1[g * NP + a];
+        gr[1] = D2[g * NP + a];
+        if (DIM == 3) gr[DIM - 1] = D3[g * NP + a];
 #pragma unroll
-        for (int a = 0; a < NP; ++a) {
-            double gr[DIM];
-            gr[0] = D1[g * NP + a];
-            gr[1] = D2[g * NP + a];
-            if (DIM == 3) gr[DIM - 1] = D3[g * NP + a];
+        for (int c = 0; c < DIM; ++c) {
+            double v = 0.0;
 #pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-                double v = 0.0;
-#pragma unroll
-                for (int k = 0; k < V::NNZ; ++k) v += gr[V::grad(c, k)] * ws[V::row(c, k)];
-                f[a][c] += v;
-            }
+            for (int k = 0; k < V::NNZ; ++k) v += gr[V::grad(c, k)] * ws[V::row(c, k)];
+            f[c] += v;
         }
     }
+    const int64_t node = elem[e * NP + a];
 #pragma unroll
-    for (int a = 0; a < NP; ++a) {
-        const int64_t node = elem[e * NP + a];
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) atomic_add_f64(F + node * DIM + c, f[a][c]);
-    }
+    for (int c = 0; c < DIM; ++c) atomic_add_f64(F + node * DIM + c, f[c]);
 }
 
 // K_r = r K_el + (1-r) K_t, symmetrised through the transpose block, then
@@ -495,14 +491,14 @@ int ssr_dev_force(const ssr_pattern *P, int n_q, const double *DPhi1, const doub
     int64_t n_n, n_p, n_e, nnzb, nnzf;
     ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
     SSR_HIP_CHECK(hipMemsetAsync(F, 0, sizeof(double) * dim * n_n, ssr_stream()));
-    const int th = 128;
+    const int th = 256;
     if (dim == 3) {
         SSR_REQUIRE(n_p == 10 && n_q == 11 && s_rows >= 6, "force: P2 tetrahedra with 11 points supported");
-        hipLaunchKernelGGL((k_force<3, 10, 11>), dim3(ssr_blocks(n_e, th)), dim3(th), 0, ssr_stream(), n_e,
+        hipLaunchKernelGGL((k_force<3, 10, 11>), dim3(ssr_blocks(n_e * 10, th)), dim3(th), 0, ssr_stream(), n_e,
                            ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, S, s_rows, WEIGHT, F);
     } else {
         SSR_REQUIRE(n_p == 6 && n_q == 7 && s_rows >= 3, "force: P2 triangles with 7 points supported");
-        hipLaunchKernelGGL((k_force<2, 6, 7>), dim3(ssr_blocks(n_e, th)), dim3(th), 0, ssr_stream(), n_e,
+        hipLaunchKernelGGL((k_force<2, 6, 7>), dim3(ssr_blocks(n_e * 6, th)), dim3(th), 0, ssr_stream(), n_e,
                            ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, S, s_rows, WEIGHT, F);
     }
     SSR_HIP_CHECK(hipGetLastError());

@@ -327,7 +327,8 @@ __global__ __launch_bounds__(256) void k_spmm_bsr(int64_t n_rows, const int32_t 
 //   k_gemm_tall:    Y = X C for X n x k, C k x m (one 64-row x 64-column
 //     output tile per block).
 // 256 threads as 16 x 16, 4 x 4 outputs each; operands staged through LDS in
-// 32-deep slabs read along the columns (coalesced).
+// 32-deep slabs read along the columns (coalesced), the next slab held in
+// registers while the current one is multiplied.
 // ---------------------------------------------------------------------------
 constexpr int kTile = 64, kSlab = 32;
 
@@ -335,6 +336,7 @@ __global__ __launch_bounds__(256) void k_gram_partial(int64_t n, int k, int tile
                                                       const double *__restrict__ X, const double *__restrict__ Y,
                                                       double *__restrict__ part)
 {
+    constexpr int PER = kSlab * kTile / 256;  // slab elements per thread and operand
     __shared__ double xs[kSlab][kTile + 1];
     __shared__ double ys[kSlab][kTile + 1];
     const int i0 = (blockIdx.x % tiles) * kTile, j0 = (blockIdx.x / tiles) * kTile;
@@ -346,14 +348,29 @@ __global__ __launch_bounds__(256) void k_gram_partial(int64_t n, int k, int tile
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
-    for (int64_t rb = r0; rb < r1; rb += kSlab) {
-        for (int e = threadIdx.x; e < kSlab * kTile; e += 256) {
+    // the next slab is loaded into registers while the current one is used
+    double px[PER], py[PER];
+    auto fetch = [&](int64_t rb) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + 256 * u;
             const int rr = e % kSlab, cc = e / kSlab;
             const int64_t r = rb + rr;
-            xs[rr][cc] = (r < r1 && i0 + cc < k) ? X[(int64_t)(i0 + cc) * n + r] : 0.0;
-            ys[rr][cc] = (r < r1 && j0 + cc < k) ? Y[(int64_t)(j0 + cc) * n + r] : 0.0;
+            px[u] = (r < r1 && i0 + cc < k) ? X[(int64_t)(i0 + cc) * n + r] : 0.0;
+            py[u] = (r < r1 && j0 + cc < k) ? Y[(int64_t)(j0 + cc) * n + r] : 0.0;
+        }
+    };
+    if (r0 < r1) fetch(r0);
+    for (int64_t rb = r0; rb < r1; rb += kSlab) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            xs[e % kSlab][e / kSlab] = px[u];
+            ys[e % kSlab][e / kSlab] = py[u];
         }
         __syncthreads();
+        if (rb + kSlab < r1) fetch(rb + kSlab);
 #pragma unroll 4
         for (int rr = 0; rr < kSlab; ++rr) {
             double a[4], b[4];
@@ -366,7 +383,6 @@ __global__ __launch_bounds__(256) void k_gram_partial(int64_t n, int k, int tile
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
         }
-        __syncthreads();
     }
     double *out = part + (int64_t)blockIdx.y * k * k;
 #pragma unroll
@@ -391,6 +407,7 @@ __global__ void k_sum_chunks(int nc, int64_t len, const double *__restrict__ par
 __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, const double *__restrict__ X,
                                                    const double *__restrict__ C, double *__restrict__ Y)
 {
+    constexpr int PER = kSlab * kTile / 256;
     __shared__ double xs[kSlab][kTile + 1];  // [q][row]
     __shared__ double cs[kSlab][kTile + 1];  // [q][column]
     const int64_t r0 = (int64_t)blockIdx.x * kTile;
@@ -401,14 +418,28 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, cons
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[p][q] = 0.0;
-    for (int q0 = 0; q0 < k; q0 += kSlab) {
-        for (int e = threadIdx.x; e < kSlab * kTile; e += 256) {
+    double px[PER], pc[PER];
+    auto fetch = [&](int q0) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + 256 * u;
             const int rr = e % kTile, qq = e / kTile;
-            xs[qq][rr] = (r0 + rr < n && q0 + qq < k) ? X[(int64_t)(q0 + qq) * n + r0 + rr] : 0.0;
+            px[u] = (r0 + rr < n && q0 + qq < k) ? X[(int64_t)(q0 + qq) * n + r0 + rr] : 0.0;
             const int q2 = e % kSlab, cc = e / kSlab;
-            cs[q2][cc] = (q0 + q2 < k && c0 + cc < m) ? C[(int64_t)(c0 + cc) * k + q0 + q2] : 0.0;
+            pc[u] = (q0 + q2 < k && c0 + cc < m) ? C[(int64_t)(c0 + cc) * k + q0 + q2] : 0.0;
+        }
+    };
+    if (k > 0) fetch(0);
+    for (int q0 = 0; q0 < k; q0 += kSlab) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            xs[e / kTile][e % kTile] = px[u];
+            cs[e % kSlab][e / kSlab] = pc[u];
         }
         __syncthreads();
+        if (q0 + kSlab < k) fetch(q0 + kSlab);
 #pragma unroll 4
         for (int qq = 0; qq < kSlab; ++qq) {
             double a[4], b[4];
@@ -421,7 +452,6 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, cons
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
