@@ -206,43 +206,48 @@ __device__ __forceinline__ double wave_sum_n(double (&s)[N], int l)
     return v;
 }
 
-// Multi-dot h = V(:,0:k)' u, columns stride ld, k <= 128.  Columns in
-// passes of kMdCols; each wavefront reduces its pass sums by shuffles into
-// LDS with no block barrier, and one barrier at the end combines the four
-// wavefronts' sums per column into this block's partial.  Passes of 8
-// columns measured faster than 16; for more than 32 columns the grid gives
-// each thread about one row per pass (many wavefronts in flight), for fewer a
-// 1024-block grid keeps the partials short.
+// Multi-dot h = V(:,0:k)' u, columns stride ld, k <= 128.  The grid is
+// (row blocks) x (8-column groups): every block takes RPT rows per thread of
+// one column group, so all groups run at once instead of one after another
+// (the latency of a small multi-dot is that of one pass).  Each wavefront
+// reduces its 8 column sums by one shuffle transposition, and one barrier
+// combines the four wavefronts into this block's partial for each column.
 constexpr int kMdCols = 8, kMdBlocks = 8192;
 
-template <class T>
+template <class T, int RPT>
 __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int k, const T *__restrict__ V,
                                                                   int64_t ld, const T *__restrict__ u,
                                                                   double *__restrict__ part)
 {
-    __shared__ double sh[kRedThreads / 64][128];
+    __shared__ double sh[kRedThreads / 64][kMdCols];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int ci = md_index<kMdCols>(l);
-    for (int c0 = 0; c0 < k; c0 += kMdCols) {
-        double s[kMdCols];
+    const int c0 = blockIdx.y * kMdCols;
+    const int nc = k - c0 < kMdCols ? k - c0 : kMdCols;
+    double s[kMdCols];
 #pragma unroll
-        for (int c = 0; c < kMdCols; ++c) s[c] = 0.0;
-        const int nc = k - c0 < kMdCols ? k - c0 : kMdCols;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-            const double ui = (double)u[i];
+    for (int c = 0; c < kMdCols; ++c) s[c] = 0.0;
+    const int64_t span = (int64_t)RPT * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * span; base < n; base += (int64_t)gridDim.x * span) {
 #pragma unroll
-            for (int c = 0; c < kMdCols; ++c)
-                if (c < nc) s[c] += (double)V[(int64_t)(c0 + c) * ld + i] * ui;
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int64_t i = base + (int64_t)rr * blockDim.x + threadIdx.x;
+            if (i < n) {
+                const double ui = (double)u[i];
+#pragma unroll
+                for (int c = 0; c < kMdCols; ++c)
+                    if (c < nc) s[c] += (double)V[(int64_t)(c0 + c) * ld + i] * ui;
+            }
         }
-        const double t = wave_sum_n<kMdCols>(s, l);
-        if ((l & (64 / kMdCols - 1)) == 0 && ci < nc) sh[w][c0 + ci] = t;
     }
+    const double t = wave_sum_n<kMdCols>(s, l);
+    if ((l & (64 / kMdCols - 1)) == 0 && ci < nc) sh[w][ci] = t;
     __syncthreads();
-    for (int c = threadIdx.x; c < k; c += blockDim.x) {
-        double t = 0.0;
+    if (threadIdx.x < nc) {
+        double a = 0.0;
 #pragma unroll
-        for (int q = 0; q < kRedThreads / 64; ++q) t += sh[q][c];
-        part[(int64_t)c * gridDim.x + blockIdx.x] = t;
+        for (int q = 0; q < kRedThreads / 64; ++q) a += sh[q][threadIdx.x];
+        part[(int64_t)(c0 + threadIdx.x) * gridDim.x + blockIdx.x] = a;
     }
 }
 
@@ -745,11 +750,18 @@ int multidot(int64_t n, int k, const T *V, int64_t ld, const T *u, double *h)
     RedScratch *sc = nullptr;
     int rc = scratch(&sc);
     if (rc) return rc;
-    const int64_t cap = k <= 32 ? kRedBlocks : kMdBlocks;
-    const int64_t nbl = (n + kRedThreads - 1) / kRedThreads;
-    const unsigned nb = (unsigned)(nbl < 1 ? 1 : (nbl > cap ? cap : nbl));
-    hipLaunchKernelGGL((k_multidot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
-                       sc->part);
+    // rows per thread: 4 for long columns, 2 otherwise (measured at 21k,
+    // 84k and 711k rows)
+    const int rpt = n >= 262144 ? 4 : 2;
+    const int64_t nbl = (n + (int64_t)kRedThreads * rpt - 1) / ((int64_t)kRedThreads * rpt);
+    const unsigned nb = (unsigned)(nbl < 1 ? 1 : (nbl > kMdBlocks ? kMdBlocks : nbl));
+    const dim3 grid(nb, (unsigned)((k + kMdCols - 1) / kMdCols));
+    if (rpt == 4)
+        hipLaunchKernelGGL((k_multidot_partial<T, 4>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
+                           sc->part);
+    else
+        hipLaunchKernelGGL((k_multidot_partial<T, 2>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
+                           sc->part);
     hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, sc->part, h);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
