@@ -77,6 +77,7 @@ def run_fos(args):
     from slope_stability_octave_amd import _lib
 
     _lib.require_gpu()
+    log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
     if args.workload == "fos3d":
         import numpy as np
         mesh_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
@@ -87,15 +88,24 @@ def run_fos(args):
         if args.omega_max != 7e7:
             params["omega_max_stop"] = args.omega_max
         name = "3D homogeneous slope SSR, P2 tetrahedra, mesh SSR_homo_ada_L1 (25591 nodes, 16615 elements)"
-    elif args.workload == "fos3d_slope":
+    elif args.workload in ("fos3d_slope", "fos3d_hetero"):
         n = args.cells_per_metre
         counts = tuple(int(round(c * n)) for c in (20, 20, 20, 10, 20, 20))
-        prob = problems.slope_3d(counts, 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HOMO_2D)
+        hetero = args.workload == "fos3d_hetero"
+        if log:
+            log(f"building the mesh and FE data (cells {counts})")
+        if hetero:
+            prob = problems.slope_3d(counts, 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HETERO_3D_LAYERS,
+                                     layers=problems.HETERO_3D_LEVELS)
+        else:
+            prob = problems.slope_3d(counts, 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HOMO_2D)
+        if log:
+            log(f"mesh ready: {prob['n_dof_free']} free DOFs, {prob['n_int']} integration points")
         params = dict(problems.PARAMS_3D)
         if args.omega_max != 7e7:
             params["omega_max_stop"] = args.omega_max
-        name = (f"3D homogeneous slope SSR, 60x30x20 m box with a 20 m high 1:1 slope, structured P2 tetrahedra "
-                f"(cells {counts})")
+        name = (f"3D {'heterogeneous (4 horizontal layers)' if hetero else 'homogeneous'} slope SSR, 60x30x20 m box "
+                f"with a 20 m high 1:1 slope, structured P2 tetrahedra (cells {counts})")
     elif args.workload == "fos2d_family":
         prob, mat = problems.slope_2d_family(args.seed, h=args.h)
         params = dict(problems.PARAMS_FAMILY)
@@ -105,12 +115,13 @@ def run_fos(args):
     else:
         prob = problems.slope_2d(h=args.h)
         params = dict(problems.PARAMS_2D)
-        params["lambda_init"] = args.lambda_init
+        params["lambda_init"] = args.lambda_init if args.lambda_init is not None else 1.0
         params["omega_max_stop"] = args.omega_max
         name = f"2D homogeneous slope SSR, P2 triangles h={args.h}"
     if args.linear_tol is not None:
         params["linear_tol"] = args.linear_tol
-    log = (lambda s: print(s, file=sys.stderr, flush=True)) if args.verbose else None
+    if args.lambda_init is not None:
+        params["lambda_init"] = args.lambda_init
     amg_opts = amg_options(args)
     res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts,
                            precision=args.precision)
@@ -195,6 +206,81 @@ def run_learned(args):
     print(json.dumps(out))
 
 
+def run_learned3d(args):
+    """Chosen form (learned predictor + FP32 Krylov/AMG with FP64 refinement)
+    against the plain GPU form (secant predictor, FP64) on the 3D homogeneous
+    slope at --cells-per-metre (1.3: about 1.5M DOF).  The network is trained
+    first on trajectories of the plain solver on coarser meshes of the same
+    slope (--train-cpm); training is reported, not part of time-to-FoS."""
+    from slope_stability_octave_amd import _lib, learned, problems
+
+    _lib.require_gpu()
+    log = (lambda s: print(s, file=sys.stderr, flush=True))
+    geom = (20.0, 20.0, 20.0, 10.0, 20.0, 20.0)
+    mat = (45.0, 6.0, 45.0)  # 1:1 slope, c0, phi of the homogeneous material
+    p = dict(problems.PARAMS_3D, linear_tol=args.linear_tol or 1e-2,
+             lambda_init=args.lambda_init if args.lambda_init is not None else 0.7)
+
+    def slope(cpm):
+        counts = tuple(int(round(c * cpm)) for c in (20, 20, 20, 10, 20, 20))
+        return counts, problems.slope_3d(counts, *geom, problems.HOMO_2D)
+
+    def cases(cpms):
+        for cpm in cpms:
+            counts, prob = slope(cpm)
+            yield f"3D slope cells {counts}", prob, mat, p
+    t0 = time.perf_counter()
+    cpms = [float(x) for x in args.train_cpm.split(",")]
+    train_raw = learned.collect(cases(cpms[:-1]), log=log)
+    val_raw = learned.collect(cases(cpms[-1:]), log=log)
+    gen_s = time.perf_counter() - t0
+    net, norm, rep = learned.fit(train_raw, val_raw, epochs=args.epochs, log=log)
+    log(f"[mlp] fit {rep}")
+    counts, prob = slope(args.cells_per_metre)
+    log(f"evaluation mesh cells {counts}: {prob['n_dof_free']} free DOFs")
+    r_plain = problems.run_ssr(prob, p, preconditioner="amg", precision=64, log=log)
+    log(f"plain: FoS {r_plain['FoS']:.7f} time {r_plain['time_to_fos_s']:.2f} s")
+    r_ch = problems.run_ssr(prob, p, preconditioner="amg", precision=32, log=log,
+                            predictor=learned.predictor_factory(net, norm, mat))
+    pr = r_ch["predictor"]
+    out = {
+        "metric": "SSR time-to-FoS reduction, learned predictor + FP32 Krylov/AMG vs plain FP64 GPU form",
+        "value": 1.0 - r_ch["time_to_fos_s"] / r_plain["time_to_fos_s"],
+        "unit": "fraction of plain time-to-FoS saved",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "f64 (Newton, constitutive) / f32 (Krylov, AMG)",
+        "data": "synthetic",
+        "config": {"workload": f"3D homogeneous slope SSR, 60x30x20 m box with a 20 m high 1:1 slope, "
+                               f"structured P2 tetrahedra (cells {counts})",
+                   "free_dof": prob["n_dof_free"], "lambda_init": p["lambda_init"],
+                   "train": f"plain-solver trajectories at {cpms[:-1]} cells/m, validation at {cpms[-1:]}",
+                   "linear_tol": p["linear_tol"]},
+        "t_plain_s": r_plain["time_to_fos_s"], "t_chosen_s": r_ch["time_to_fos_s"],
+        "FoS_plain": r_plain["FoS"], "FoS_chosen": r_ch["FoS"], "dFoS": abs(r_plain["FoS"] - r_ch["FoS"]),
+        "newton_plain": r_plain["newton_iterations"], "newton_chosen": r_ch["newton_iterations"],
+        "steps_plain": len(r_plain["lambda_hist"]), "steps_chosen": len(r_ch["lambda_hist"]),
+        "predictor_used": pr.used, "predictor_rejected": pr.rejected,
+        "curve_max_dlambda": _curve_gap(r_plain, r_ch),
+        "lambda_plain": [float(x) for x in r_plain["lambda_hist"]],
+        "omega_plain": [float(x) for x in r_plain["omega_hist"]],
+        "lambda_chosen": [float(x) for x in r_ch["lambda_hist"]],
+        "omega_chosen": [float(x) for x in r_ch["omega_hist"]],
+        "training": dict(rep, data_generation_s=gen_s),
+    }
+    print(json.dumps(out))
+
+
+def _curve_gap(ra, rb):
+    """Largest |lambda_b(omega) - lambda_a(omega)| over b's accepted steps
+    inside a's omega range (a's curve interpolated linearly): how far the two
+    continuation curves are apart, independent of where each run stopped."""
+    oa, la = np.asarray(ra["omega_hist"]), np.asarray(ra["lambda_hist"])
+    ob, lb = np.asarray(rb["omega_hist"]), np.asarray(rb["lambda_hist"])
+    sel = (ob >= oa[0]) & (ob <= oa[-1])
+    return float(np.abs(np.interp(ob[sel], oa, la) - lb[sel]).max()) if sel.any() else float("nan")
+
+
 def amg_options(args):
     """AMG options named on the command line (None: the defaults)."""
     opts = None
@@ -275,9 +361,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos3d_slope", "fos2d_family", "learned", "batch", "ll2d"])
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
+                             "learned3d", "batch", "ll2d"])
     ap.add_argument("--h", type=float, default=0.5)
-    ap.add_argument("--lambda-init", type=float, default=1.0)
+    ap.add_argument("--lambda-init", type=float, default=None,
+                    help="first lambda of the continuation (default: 1.0 for fos2d, the parameter set's otherwise)")
     ap.add_argument("--omega-max", type=float, default=7e7)
     ap.add_argument("--precond", default="amg", choices=["amg", "block_jacobi"])
     ap.add_argument("--verbose", action="store_true")
@@ -297,6 +385,8 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="instances of the batch workload")
     ap.add_argument("--workers", type=int, default=8, help="host threads (streams) of the batch workload")
     ap.add_argument("--train-seeds", type=int, default=16)
+    ap.add_argument("--train-cpm", default="0.5,0.6,0.7,0.8,0.9",
+                    help="learned3d: cells per metre of the training meshes (the last one validates)")
     ap.add_argument("--eval-seeds", type=int, default=4)
     ap.add_argument("--h-train", type=float, default=1.0)
     ap.add_argument("--epochs", type=int, default=3000)
@@ -326,10 +416,12 @@ def main():
     if args.amg_block_smooth is not None:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_AMG_BLOCK_SMOOTH, args.amg_block_smooth)
-    if args.workload in ("fos2d", "fos3d", "fos3d_slope", "fos2d_family"):
+    if args.workload in ("fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family"):
         return run_fos(args)
     if args.workload == "learned":
         return run_learned(args)
+    if args.workload == "learned3d":
+        return run_learned3d(args)
     if args.workload == "batch":
         return run_batch(args)
     if args.workload == "ll2d":

@@ -369,31 +369,42 @@ def recorder_factory(material):
 # ---------------------------------------------------------------------------
 # end-to-end: trajectories -> training -> use
 # ---------------------------------------------------------------------------
-def generate(seeds, h=1.0, params=None, preconditioner="amg", log=None, linear_tol=1e-2):
-    """Run the plain solver on the seeded 2D slope family and collect the
-    training samples of every trajectory.  Returns (X, G, g, t, h) raw."""
+def collect(cases, preconditioner="amg", log=None):
+    """Run the plain solver on each (name, problem, material, params) case and
+    collect the training samples of every trajectory.  Returns (X, G, g, t, h)
+    raw; cases whose continuation fails are skipped."""
     from . import problems
     from .continuation import ContinuationError
     Xs, Gs, gs, ts, hs = [], [], [], [], []
-    for seed in seeds:
-        prob, mat = problems.slope_2d_family(seed, h=h)
-        p = dict(problems.PARAMS_FAMILY)
-        p.update(params or {})
-        p.setdefault("linear_tol", linear_tol)
+    for name, prob, mat, p in cases:
         try:
             res = problems.run_ssr(prob, p, preconditioner=preconditioner, recorder=recorder_factory(mat))
         except ContinuationError as exc:
             if log:
-                log(f"[mlp] seed {seed} skipped: {exc}")
+                log(f"[mlp] {name} skipped: {exc}")
             continue
         X, G, g, t, hh = res["recorder"].arrays()
         if log:
             lo = np.mean(res["recorder"].L_opt) if len(X) else float("nan")
-            log(f"[mlp] seed {seed} material {tuple(round(x, 2) for x in mat)} FoS {res['FoS']:.5f} "
+            log(f"[mlp] {name} material {tuple(round(x, 2) for x in mat)} FoS {res['FoS']:.5f} "
                 f"samples {len(X)} span optimum L {lo:.3f} time {res['time_to_fos_s']:.2f}s")
         if len(X):
             Xs.append(X), Gs.append(G), gs.append(g), ts.append(t), hs.append(hh)
     return (np.concatenate(Xs), np.concatenate(Gs), np.concatenate(gs), np.concatenate(ts), np.concatenate(hs))
+
+
+def generate(seeds, h=1.0, params=None, preconditioner="amg", log=None, linear_tol=1e-2):
+    """Training samples from the plain solver on the seeded 2D slope family."""
+    from . import problems
+
+    def cases():
+        for seed in seeds:
+            prob, mat = problems.slope_2d_family(seed, h=h)
+            p = dict(problems.PARAMS_FAMILY)
+            p.update(params or {})
+            p.setdefault("linear_tol", linear_tol)
+            yield f"seed {seed}", prob, mat, p
+    return collect(cases(), preconditioner=preconditioner, log=log)
 
 
 def fit(train_raw, val_raw=None, epochs=3000, lr=1e-3, wd=1e-3, w_lam=0.5, seed=0, log=None):

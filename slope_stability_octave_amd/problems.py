@@ -28,6 +28,13 @@ PARAMS_3D = dict(lambda_init=0.9, d_lambda_init=0.1, d_lambda_min=1e-3, d_lambda
                  linear_tol=1e-1, linear_maxit=100, deflation_tol=1e-3)
 
 HOMO_2D = [dict(c0=6, phi=45, psi=0, young=40000, poisson=0.3, gamma_sat=20, gamma_unsat=20)]
+# four horizontal layers of the 3D heterogeneous slope workload, bottom to
+# top (kPa, degrees); equal thickness over the 30 m height
+HETERO_3D_LAYERS = [dict(c0=50, phi=40, psi=0, young=100000, poisson=0.3, gamma_sat=20, gamma_unsat=20),
+                    dict(c0=18, phi=32, psi=0, young=50000, poisson=0.3, gamma_sat=20, gamma_unsat=20),
+                    dict(c0=10, phi=35, psi=0, young=20000, poisson=0.3, gamma_sat=20, gamma_unsat=20),
+                    dict(c0=15, phi=30, psi=0, young=10000, poisson=0.3, gamma_sat=20, gamma_unsat=20)]
+HETERO_3D_LEVELS = [7.5, 15.0, 22.5]
 
 
 def _finish(coord, elem, Q, dim, mats, mat_id):
@@ -84,6 +91,8 @@ def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", 
     p.update(params or {})
     t0 = time.perf_counter()
     cmb = make_constitutive(prob)
+    if log:
+        log(f"pattern and constitutive data ready ({time.perf_counter() - t0:.1f} s)")
     cls = DFGMRES if solver == "dfgmres" else DCG
     if preconditioner == "amg":
         amg_options = dict(amg_options or {})
