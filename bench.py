@@ -332,12 +332,23 @@ def run_batch(args):
 
     _lib.require_gpu()
     rng = np.random.default_rng(7)
-    hs = rng.choice([1.0, 0.75, 0.5], size=args.batch)
+    if args.h_range:
+        lo, hi = (float(x) for x in args.h_range.split(","))
+        hs = rng.uniform(lo, hi, size=args.batch)
+        sizes = f"h ~ U[{lo}, {hi}]"
+    else:
+        hs = rng.choice([1.0, 0.75, 0.5], size=args.batch)
+        sizes = "h in {1, 0.75, 0.5}"
     probs = [problems.slope_2d_family(3000 + i, h=float(hs[i]))[0] for i in range(args.batch)]
+    print(f"{args.batch} instances, {min(p['n_dof_free'] for p in probs)}..{max(p['n_dof_free'] for p in probs)} "
+          f"free DOFs", file=sys.stderr, flush=True)
     p = dict(problems.PARAMS_FAMILY, linear_tol=args.linear_tol or 1e-2)
-    seq, t_seq = problems.run_batch(probs, workers=1, params=p, precision=args.precision)
-    par, t_par = problems.run_batch(probs, workers=args.workers, params=p, precision=args.precision)
-    dfos = max(abs(a["FoS"] - b["FoS"]) for a, b in zip(seq, par))
+    log = (lambda s: print(s, file=sys.stderr, flush=True))
+    seq, t_seq = problems.run_batch(probs, workers=1, params=p, precision=args.precision, log=log)
+    log(f"one after another: {t_seq:.1f} s")
+    par, t_par = problems.run_batch(probs, workers=args.workers, params=p, precision=args.precision, log=log)
+    ok = [i for i in range(len(probs)) if "error" not in seq[i] and "error" not in par[i]]
+    dfos = max((abs(seq[i]["FoS"] - par[i]["FoS"]) for i in ok), default=float("nan"))
     out = {
         "metric": "SSR instances per second, concurrent per-thread streams vs one after another",
         "value": args.batch / t_par,
@@ -346,10 +357,11 @@ def run_batch(args):
         "n_gpus": 1,
         "dtype": "f64" if args.precision == 64 else "f64 (Newton) / f32 (Krylov, AMG)",
         "data": "synthetic",
-        "config": {"workload": f"{args.batch} 2D slope-family instances, h in {{1, 0.75, 0.5}}",
+        "config": {"workload": f"{args.batch} 2D slope-family instances, {sizes}",
                    "free_dof": [int(pr["n_dof_free"]) for pr in probs], "workers": args.workers},
         "wall_sequential_s": t_seq, "wall_concurrent_s": t_par, "speedup": t_seq / t_par,
         "max_dFoS_concurrent_vs_sequential": dfos,
+        "stopped_with_error": [i for i in range(len(probs)) if "error" in seq[i] or "error" in par[i]],
     }
     print(json.dumps(out))
 
@@ -383,6 +395,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2000, help="member of the 2D slope family (fos2d_family)")
     ap.add_argument("--cells-per-metre", type=float, default=1.0, help="mesh density of fos3d_slope")
     ap.add_argument("--batch", type=int, default=16, help="instances of the batch workload")
+    ap.add_argument("--h-range", default=None, help="batch: mesh sizes h ~ U[lo, hi] as 'lo,hi' (0.16,0.5: 20k-200k DOF)")
     ap.add_argument("--workers", type=int, default=8, help="host threads (streams) of the batch workload")
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--train-cpm", default="0.5,0.6,0.7,0.8,0.9",

@@ -181,12 +181,15 @@ def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_option
                 setup_s=t_setup, time_to_fos_s=t_run, timings=timings, counts=dict(cmb.counts))
 
 
-def run_batch(problems, workers=8, params=None, preconditioner="amg", precision=64, predictor_factories=None):
+def run_batch(problems, workers=8, params=None, preconditioner="amg", precision=64, predictor_factories=None,
+              log=None):
     """Independent SSR instances (of any sizes) solved concurrently: each
     worker thread drives its instances on a HIP stream of its own, so the
     kernels of different instances overlap on the device (the library
     releases the Python lock inside every call).  Returns the per-instance
-    results (in input order) and the wall time of the whole batch."""
+    results (in input order; an instance whose continuation stops with an
+    error gives FoS nan and the message) and the wall time of the whole
+    batch."""
     from concurrent.futures import ThreadPoolExecutor
     from ._lib import ThreadStream
     import threading
@@ -201,7 +204,14 @@ def run_batch(problems, workers=8, params=None, preconditioner="amg", precision=
             with lock:
                 streams.append(local.ts)
         pf = predictor_factories[i] if predictor_factories else None
-        return run_ssr(problems[i], params, preconditioner=preconditioner, precision=precision, predictor=pf)
+        from .continuation import ContinuationError
+        try:
+            res = run_ssr(problems[i], params, preconditioner=preconditioner, precision=precision, predictor=pf)
+        except ContinuationError as exc:  # the instance's own outcome, reported like the reference's error
+            res = dict(FoS=float("nan"), error=str(exc))
+        if log:
+            log(f"instance {i}: FoS {res['FoS']:.7f}" + (f" ({res['error']})" if "error" in res else ""))
+        return res
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=workers) as ex:
