@@ -63,3 +63,23 @@ def test_fos_matches_cpu_oracle_coarse_2d(coarse2d):
     # the lambda-omega curve is monotone (slope_stability_2D_homo_SSR.m:229-234)
     assert np.all(np.diff(res["lambda_hist"]) >= -1e-10)
     assert np.all(np.diff(res["omega_hist"]) >= -1e-10)
+
+
+def test_fos_heterogeneous_3d_matches_cpu_oracle():
+    """The four-layer 3D slope (per-point materials through reduction, return
+    mapping and assembly) on a coarse mesh: GPU continuation, with one AMG
+    hierarchy shared by the steps, against the CPU oracle path."""
+    prob = problems.slope_3d((2, 2, 2, 1, 2, 2), 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HETERO_3D_LAYERS,
+                             layers=problems.HETERO_3D_LEVELS)
+    assert len(np.unique(prob["mats"]["c0"])) == 4
+    cpu = NP.ConstitutiveCPU(prob["coord"], prob["elem"], prob["Q"], 3, prob["DPhi"], prob["WEIGHT"], prob["mats"])
+    p = dict(problems.PARAMS_3D)
+    p["lambda_init"] = 0.7
+    p["linear_tol"] = 1e-10
+    p["linear_maxit"] = 1500
+    res = problems.run_ssr(prob, p, preconditioner="amg")
+    U0, lh0, oh0 = NP.SSR_indirect_continuation(
+        0.7, p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
+        p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], prob["f"], cpu)
+    print("GPU FoS", res["FoS"], "CPU FoS", lh0[-1], "steps", len(res["lambda_hist"]), len(lh0), flush=True)
+    assert abs(res["FoS"] - lh0[-1]) <= 1e-4

@@ -157,6 +157,36 @@ def test_multidot_and_gemv_update_match_numpy(k):
     assert np.abs(du.get() - ref_u).max() <= 1e-12 * np.abs(ref_u).max() * k
 
 
+def test_shared_amg_hierarchy_outlives_its_owner_and_rebuilds_on_request():
+    """Copies of a solver share its AMG object: each copy solves to the
+    tolerance with it, the object survives the original's destruction, and
+    an invalidated hierarchy is rebuilt on the next setup (same solution)."""
+    import gc
+    cmb, K, Kq, qf = operator(3, counts=(2, 2, 2, 2, 2, 2))
+    rng = np.random.default_rng(7)
+    n = cmb.pattern.n_dof
+    from slope_stability_octave_amd import mesh
+    coord, _, _ = mesh.mesh_P2_3D(2, 2, 2, 2, 2, 2, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0)
+    orig = DFGMRES(cmb.pattern, 1e-8, 300, 1e-3, "amg", amg_options=dict(coord=coord))
+    bq = rng.normal(size=qf.size)
+    b = DeviceArray.from_host(_full(bq, qf, n, 3))
+    c1 = orig.copy()
+    c1.setup_preconditioner(K)
+    x1, it1 = c1.solve(K, b)
+    c2 = orig.copy()
+    del orig, c1
+    gc.collect()
+    c2.setup_preconditioner(K)  # shared hierarchy: kept
+    x2, it2 = c2.solve(K, b)
+    assert it2 == it1 and np.array_equal(x1.get(), x2.get())
+    c2.invalidate_preconditioner()
+    c2.setup_preconditioner(K)  # rebuilt from the same operator
+    x3, it3 = c2.solve(K, b)
+    assert it3 == it1 and np.abs(x3.get() - x1.get()).max() <= 1e-12 * np.abs(x1.get()).max()
+    for x in (x1, x3):
+        assert np.linalg.norm(Kq @ x.get().ravel()[qf] - bq) <= 1.1e-8 * np.linalg.norm(bq)
+
+
 def test_fgmres_relative_tolerance_stop():
     """With tol 1e-1 the solver stops at the first iterate whose relative
     residual estimate is below 0.1 and reports it."""
