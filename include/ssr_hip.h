@@ -202,6 +202,11 @@ int ssr_solver_set_matrix(ssr_solver *s, const double *Kvals, int setup_precond)
  * residual meets the tolerance.  No reference counterpart (the reference runs
  * FP64 only); call before set_matrix. */
 int ssr_solver_set_precision(ssr_solver *s, int bits);
+/* The operator is symmetric (as K_r is, CONSTITUTIVE.m:515-532): the
+ * deflation projector x - W W'A x then takes W'A x as (A W)' x with the A W
+ * left by ssr_solver_A_orthogonalize, one SpMV less per Krylov iteration.
+ * No reference counterpart (dfgmres_solver.m:59-64 applies A each time). */
+int ssr_solver_set_symmetric(ssr_solver *s, int symmetric);
 int ssr_solver_get_precision(const ssr_solver *s, int *bits);
 /* near_null: n_null columns over all dim*n_n DOFs (host, column-major);
  * opts: NULL or double[8] {theta, max_levels, max_coarse, cheb_degree, cheb_lower,

@@ -145,7 +145,14 @@ struct ssr_solver {
     DevBuf<double> V, Z, t1, t2, r;
     DevBuf<float> V32, Z32, t2f;
     DevBuf<double> hcol, hW, ydev, scal;  // Arnoldi columns, deflation coefficients, LSQ solution, scalars
-    DevBuf<double> AW, W2, Gd, Td, gwork;  // re-orthogonalisation: A W, new basis, Gram matrix, coefficients
+    DevBuf<double> AW, W2, AW2, Gd, Td, gwork;  // re-orthogonalisation: A W, new basis and its A W, Gram, coefficients
+    // A W of the current basis as left by the re-orthogonalisation (valid
+    // while kAW == kW: set_matrix, expand, clear and copy invalidate it).  For
+    // a symmetric operator W'A x = (A W)' x, so the projector skips its SpMV.
+    int kAW = -1;
+    bool symmetric = false;
+    DevBuf<float> AW32;
+    int kAW32 = -1;
     HostBuf<double> hhost, yhost;
     std::vector<hipEvent_t> ev;
     // stats
@@ -181,15 +188,17 @@ double hnorm(ssr_solver *s, const double *x, int *rc)
     return sqrt(v);
 }
 
-// u = alpha*u + beta*W (W' x) with the basis of precision T; coefficients in s->hW
+// u = alpha*u + beta*W (H' x) with the basis of precision T, H = W unless
+// given; coefficients in s->hW
 template <class T>
-int defl_apply(ssr_solver *s, const T *x, double alpha, double beta, T *u)
+int defl_apply(ssr_solver *s, const T *x, double alpha, double beta, T *u, const T *H = nullptr)
 {
     int rc;
     const T *W = defl_basis<T>(s);
+    if (!H) H = W;
     for (int c0 = 0; c0 < s->kW; c0 += 128) {
         const int k = s->kW - c0 < 128 ? s->kW - c0 : 128;
-        if ((rc = ssr::multidot<T>(s->n, k, W + (int64_t)c0 * s->n, s->n, x, s->hW.p + c0))) return rc;
+        if ((rc = ssr::multidot<T>(s->n, k, H + (int64_t)c0 * s->n, s->n, x, s->hW.p + c0))) return rc;
     }
     for (int c0 = 0; c0 < s->kW; c0 += 128) {
         const int k = s->kW - c0 < 128 ? s->kW - c0 : 128;
@@ -200,11 +209,21 @@ int defl_apply(ssr_solver *s, const T *x, double alpha, double beta, T *u)
     return SSR_OK;
 }
 
-// x <- x - W W' A x   (scratch t)
+template <class T> const T *aw_basis(const ssr_solver *s);
+template <> const double *aw_basis<double>(const ssr_solver *s) { return s->kAW == s->kW ? s->AW.p : nullptr; }
+template <> const float *aw_basis<float>(const ssr_solver *s)
+{
+    return s->kAW == s->kW && s->kAW32 == s->kAW ? s->AW32.p : nullptr;
+}
+
+// x <- x - W W' A x   (scratch t); with a symmetric operator and A W at hand,
+// W'A x = (A W)' x needs no SpMV
 template <class T>
 int project(ssr_solver *s, T *x, T *t)
 {
     if (s->kW == 0) return SSR_OK;
+    const T *AWb = s->symmetric ? aw_basis<T>(s) : nullptr;
+    if (AWb) return defl_apply<T>(s, x, 1.0, -1.0, x, AWb);
     int rc;
     if ((rc = spmv<T>(s, x, t))) return rc;
     return defl_apply<T>(s, t, 1.0, -1.0, x);
@@ -226,6 +245,16 @@ int ensure_w32(ssr_solver *s)
     if ((rc = s->W32.alloc(s->n * (int64_t)(s->capW > 0 ? s->capW : 1)))) return rc;
     if (s->kW > 0 && (rc = ssr::convert<double, float>(s->n * (int64_t)s->kW, 1.0, s->W.p, s->W32.p))) return rc;
     s->kW32 = s->kW;
+    return SSR_OK;
+}
+
+int ensure_aw32(ssr_solver *s)
+{
+    if (!s->symmetric || s->kAW != s->kW || s->kAW32 == s->kAW) return SSR_OK;
+    int rc;
+    if ((rc = s->AW32.alloc(s->n * (int64_t)(s->capW > 0 ? s->capW : 1)))) return rc;
+    if (s->kAW > 0 && (rc = ssr::convert<double, float>(s->n * (int64_t)s->kAW, 1.0, s->AW.p, s->AW32.p))) return rc;
+    s->kAW32 = s->kAW;
     return SSR_OK;
 }
 
@@ -404,6 +433,17 @@ int ssr_solver_set_precond(ssr_solver *s, ssr_precond *M)
     return SSR_OK;
 }
 
+// The operator is symmetric (K_r is symmetrised by construction): the
+// deflation projector then uses (A W)' x for W'A x.
+int ssr_solver_set_symmetric(ssr_solver *s, int symmetric)
+{
+    SSR_REQUIRE(s, "set_symmetric: null");
+    s->symmetric = symmetric != 0;
+    s->kAW = -1;
+    s->kAW32 = -1;
+    return SSR_OK;
+}
+
 // The next setup of the solver's preconditioner rebuilds it in full.
 int ssr_solver_invalidate_preconditioner(ssr_solver *s)
 {
@@ -430,6 +470,7 @@ int ssr_solver_set_precision(ssr_solver *s, int bits)
     s->precision = bits;
     s->K32_valid = false;
     s->kW32 = -1;
+    s->kAW32 = -1;
     return SSR_OK;
 }
 
@@ -446,6 +487,7 @@ int ssr_solver_set_matrix(ssr_solver *s, const double *Kvals, int setup_precond)
 {
     SSR_REQUIRE(s && Kvals, "solver_set_matrix: null");
     s->K = Kvals;
+    s->kAW = -1;  // values may change behind the same pointer
     int rc;
     s->K32_valid = false;
     if (s->precision == 32) {
@@ -468,6 +510,7 @@ int ssr_solver_clear_deflation(ssr_solver *s)
 {
     s->kW = 0;
     s->kW32 = -1;
+    s->kAW = -1;
     return SSR_OK;
 }
 
@@ -494,6 +537,7 @@ int ssr_solver_expand_deflation(ssr_solver *s, const double *v)
     if (rc) return rc;
     s->kW++;
     s->kW32 = -1;
+    s->kAW = -1;
     return SSR_OK;
 }
 
@@ -503,6 +547,7 @@ int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src)
     SSR_REQUIRE(dst && src && dst->n == src->n, "copy_deflation: mismatched solvers");
     dst->kW = 0;
     dst->kW32 = -1;
+    dst->kAW = -1;
     for (int c = 0; c < src->kW; ++c) {
         int rc = ssr_solver_expand_deflation(dst, src->W.p + (int64_t)c * src->n);
         if (rc) return rc;
@@ -591,6 +636,7 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
     if (acc == 0) {
         s->kW = 0;
         s->kW32 = -1;
+        s->kAW = -1;
         return SSR_OK;
     }
     // W_new(:, c) = W * T[acc-1-c]  (accepted columns back in forward order)
@@ -603,9 +649,18 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
     if ((rc = ssr::gemm_tall(n, k, acc, s->W.p, s->Td.p, s->W2.p))) return rc;
     std::swap(s->W.p, s->W2.p);
     std::swap(s->W.n, s->W2.n);
+    if (s->symmetric) {
+        // A (W T) = (A W) T: the projector of the solves that follow reads it
+        if ((rc = s->AW2.alloc(n * (int64_t)s->capW))) return rc;
+        if ((rc = ssr::gemm_tall(n, k, acc, s->AW.p, s->Td.p, s->AW2.p))) return rc;
+        std::swap(s->AW.p, s->AW2.p);
+        std::swap(s->AW.n, s->AW2.n);
+    }
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));  // Th leaves scope
     s->kW = acc;
     s->kW32 = -1;
+    s->kAW = s->symmetric ? acc : -1;
+    s->kAW32 = -1;
     return SSR_OK;
 }
 
@@ -646,7 +701,7 @@ int ssr_solver_dfgmres(ssr_solver *s, const double *b, double *x, int *iters, do
     } else {
         if ((rc = s->V32.alloc(n * (int64_t)(m + 1))) || (rc = s->Z32.alloc(n * (int64_t)m)) || (rc = s->t2f.alloc(n)))
             return rc;
-        if ((rc = ensure_w32(s))) return rc;
+        if ((rc = ensure_w32(s)) || (rc = ensure_aw32(s))) return rc;
         while (it_total < m && refine < s->max_refine) {
             if (refine > 0 && s->kW > 0) {
                 // The projected Krylov space has no component in span(W), so a

@@ -40,6 +40,7 @@ _lib.extend_signatures({
     "ssr_solver_stats": [_vp, _vp],
     "ssr_solver_set_amg": [_vp, _vp, _int, _vp],
     "ssr_solver_set_precision": [_vp, _int],
+    "ssr_solver_set_symmetric": [_vp, _int],
     "ssr_solver_get_precision": [_vp, _vp],
     "ssr_dev_spmv_bsr": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
     "ssr_dev_dot": [_i64, _vp, _vp, _vp, _vp],
@@ -98,7 +99,7 @@ class DFGMRES:
     method = "dfgmres"
 
     def __init__(self, pattern, tolerance=1e-1, max_iterations=100, tolerance_deflation_basis=1e-3,
-                 preconditioner="block_jacobi", verbose=0, amg_options=None, precision=64):
+                 preconditioner="block_jacobi", verbose=0, amg_options=None, precision=64, symmetric=True):
         self.pattern = pattern
         self.precision = int(precision)
         self.tolerance = float(tolerance)
@@ -116,6 +117,9 @@ class DFGMRES:
         self.handle = h
         if self.precision != 64:
             call("ssr_solver_set_precision", self.handle, self.precision)
+        # the operators of the Newton solves (K_r) are symmetric
+        self.symmetric = bool(symmetric)
+        call("ssr_solver_set_symmetric", self.handle, 1 if self.symmetric else 0)
         if preconditioner == "amg":
             from . import amg
             amg.attach(self, amg_options)
@@ -193,7 +197,7 @@ class DFGMRES:
         with amg_options['reuse_hierarchy'] (the default), the copy shares
         this solver's AMG object so one hierarchy serves every copy."""
         new = type(self)(self.pattern, self.tolerance, self.max_iterations, self.tolerance_deflation_basis,
-                         self.preconditioner_name, self.verbose, self.amg_options, self.precision)
+                         self.preconditioner_name, self.verbose, self.amg_options, self.precision, self.symmetric)
         call("ssr_solver_copy_deflation", new.handle, self.handle)
         if self.preconditioner_name == "amg" and _amg_option(self.amg_options, "reuse_hierarchy"):
             call("ssr_solver_share_preconditioner", new.handle, self.handle)

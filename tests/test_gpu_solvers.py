@@ -187,6 +187,31 @@ def test_shared_amg_hierarchy_outlives_its_owner_and_rebuilds_on_request():
         assert np.linalg.norm(Kq @ x.get().ravel()[qf] - bq) <= 1.1e-8 * np.linalg.norm(bq)
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_symmetric_projector_matches_spmv_projector(prec):
+    """With a symmetric operator the deflation projector takes W'A x as
+    (A W)' x from the A-orthogonalisation; the solves must match the form that
+    applies A every iteration, in double and in float."""
+    cmb, K, Kq, qf = operator(3, r=0.3, counts=(2, 2, 2, 2, 2, 2))
+    rng = np.random.default_rng(9)
+    n = cmb.pattern.n_dof
+    vecs = [rng.normal(size=qf.size) for _ in range(12)]
+    bq = rng.normal(size=qf.size)
+    b = DeviceArray.from_host(_full(bq, qf, n, 3))
+    out = {}
+    for sym in (True, False):
+        s = DFGMRES(cmb.pattern, 1e-9, 400, 1e-3, precision=prec, symmetric=sym)
+        s.setup_preconditioner(K)
+        for v in vecs:
+            s.expand_deflation_basis(DeviceArray.from_host(_full(v, qf, n, 3)))
+        s.A_orthogonalize(K)
+        x, it = s.solve(K, b)
+        out[sym] = (x.get().ravel()[qf], it)
+        assert np.linalg.norm(Kq @ out[sym][0] - bq) <= 1.1e-9 * np.linalg.norm(bq)
+    assert abs(out[True][1] - out[False][1]) <= 2
+    assert np.linalg.norm(out[True][0] - out[False][0]) <= 1e-7 * np.linalg.norm(out[False][0])
+
+
 def test_fgmres_relative_tolerance_stop():
     """With tol 1e-1 the solver stops at the first iterate whose relative
     residual estimate is below 0.1 and reports it."""
