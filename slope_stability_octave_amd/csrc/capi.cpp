@@ -320,6 +320,9 @@ extern "C" int ssr_set_tuning(int key, int value)
     if (key == SSR_TUNE_SPMV_GROUP)
         SSR_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64,
                     "set_tuning: SpMV group must be 0 (by dimension), 4, 8, 16, 32 or 64");
+    if (key == SSR_TUNE_SPMM_GROUP)
+        SSR_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
+                    "set_tuning: multi-vector SpMV group must be 0 (by dimension), 4, 8, 16 or 32");
     g_tuning[key] = value;
     return SSR_OK;
 }

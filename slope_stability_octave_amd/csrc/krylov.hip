@@ -867,14 +867,22 @@ int spmm_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
     SSR_REQUIRE(nv >= 0 && nv <= kSpmmNV, "spmm_bsr: nv must be in [0,8]");
     SSR_REQUIRE(ldx >= n_rows * dim && ldy >= n_rows * dim, "spmm_bsr: leading dimension too small");
     if (n_rows == 0 || nv == 0) return SSR_OK;
-    constexpr int G = 8;
+    const int G = ssr_tuning(SSR_TUNE_SPMM_GROUP) > 0 ? ssr_tuning(SSR_TUNE_SPMM_GROUP) : (dim == 3 ? 16 : 4);
     const dim3 grid(ssr_blocks(n_rows * G, 256)), blk(256);
-    if (dim == 3)
-        hipLaunchKernelGGL((k_spmm_bsr<3, G, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, X, ldx, Y,
-                           ldy, nv);
-    else
-        hipLaunchKernelGGL((k_spmm_bsr<2, G, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, X, ldx, Y,
-                           ldy, nv);
+#define SPMM(D_, G_) hipLaunchKernelGGL((k_spmm_bsr<D_, G_, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, \
+                                        val, X, ldx, Y, ldy, nv)
+    if (dim == 3) {
+        if (G == 4) SPMM(3, 4);
+        else if (G == 8) SPMM(3, 8);
+        else if (G == 32) SPMM(3, 32);
+        else SPMM(3, 16);
+    } else {
+        if (G == 8) SPMM(2, 8);
+        else if (G == 16) SPMM(2, 16);
+        else if (G == 32) SPMM(2, 32);
+        else SPMM(2, 4);
+    }
+#undef SPMM
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }
