@@ -11,7 +11,9 @@ from ._lib import call, ptr
 # object, so the hierarchy is built once (then every rebuild_every setups);
 # the finest level always applies the current tangent; a failed Newton solve
 # of the continuation asks for a rebuild before its retry.
-DEFAULTS = dict(theta=0.0, max_levels=12, max_coarse=600, cheb_degree=2, cheb_lower=1.0 / 30.0, cheb_upper=1.1,
+# cheb_degree None: 3 in 3D, 2 in 2D (measured with the node-block smoother of
+# the finest level, which is the default)
+DEFAULTS = dict(theta=0.0, max_levels=12, max_coarse=600, cheb_degree=None, cheb_lower=1.0 / 30.0, cheb_upper=1.1,
                 refresh_every=0, rebuild_every=0, reuse_hierarchy=True)
 
 
@@ -46,6 +48,8 @@ def attach(solver, options=None):
     if coord is None:
         raise ValueError("AMG needs the node coordinates (options['coord'])")
     B = np.asfortranarray(rigid_body_modes(np.asarray(coord, dtype=np.float64)))
+    if o["cheb_degree"] is None:
+        o["cheb_degree"] = 3 if np.asarray(coord).shape[0] == 3 else 2
     vec = np.array([o["theta"], o["max_levels"], o["max_coarse"], o["cheb_degree"], o["cheb_lower"],
                     o["cheb_upper"], o["refresh_every"], o["rebuild_every"]], dtype=np.float64)
     call("ssr_solver_set_amg", solver.handle, ptr(B), B.shape[1], ptr(vec))
