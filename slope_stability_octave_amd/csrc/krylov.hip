@@ -339,11 +339,12 @@ constexpr int kTile = 64, kSlab = 32;
 
 __global__ __launch_bounds__(256) void k_gram_partial(int64_t n, int k, int tiles, int64_t rows_per_chunk,
                                                       const double *__restrict__ X, const double *__restrict__ Y,
-                                                      double *__restrict__ part)
+                                                      double *__restrict__ part, int upper_only)
 {
     constexpr int PER = kSlab * kTile / 256;  // slab elements per thread and operand
     __shared__ double xs[kSlab][kTile + 1];
     __shared__ double ys[kSlab][kTile + 1];
+    if (upper_only && blockIdx.x % tiles > blockIdx.x / tiles) return;  // mirrored by k_sum_chunks
     const int i0 = (blockIdx.x % tiles) * kTile, j0 = (blockIdx.x / tiles) * kTile;
     const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
     const int64_t r1 = r0 + rows_per_chunk < n ? r0 + rows_per_chunk : n;
@@ -400,17 +401,23 @@ __global__ __launch_bounds__(256) void k_gram_partial(int64_t n, int k, int tile
 }
 
 // out[e] = sum over c < nc of part[c*len + e], in chunk order
-__global__ void k_sum_chunks(int nc, int64_t len, const double *__restrict__ part, double *__restrict__ out)
+// (k x k result; with upper_only the tiles below the diagonal were skipped
+// and their entries are read from the transposed position)
+__global__ void k_sum_chunks(int nc, int k, const double *__restrict__ part, double *__restrict__ out, int upper_only)
 {
+    const int64_t len = (int64_t)k * k;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= len) return;
+    const int i = (int)(e % k), j = (int)(e / k);
+    const int64_t src = (upper_only && i / kTile > j / kTile) ? (int64_t)i * k + j : e;
     double s = 0.0;
-    for (int c = 0; c < nc; ++c) s += part[(int64_t)c * len + e];
+    for (int c = 0; c < nc; ++c) s += part[(int64_t)c * len + src];
     out[e] = s;
 }
 
 __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, const double *__restrict__ X,
-                                                   const double *__restrict__ C, double *__restrict__ Y)
+                                                   const double *__restrict__ C, double *__restrict__ Y,
+                                                   const int *__restrict__ qstart)
 {
     constexpr int PER = kSlab * kTile / 256;
     __shared__ double xs[kSlab][kTile + 1];  // [q][row]
@@ -434,8 +441,10 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, cons
             pc[u] = (q0 + q2 < k && c0 + cc < m) ? C[(int64_t)(c0 + cc) * k + q0 + q2] : 0.0;
         }
     };
-    if (k > 0) fetch(0);
-    for (int q0 = 0; q0 < k; q0 += kSlab) {
+    // rows of C above qstart[column tile] are zero in every column of the tile
+    const int qs = qstart ? (qstart[blockIdx.y] / kSlab) * kSlab : 0;
+    if (qs < k) fetch(qs);
+    for (int q0 = qs; q0 < k; q0 += kSlab) {
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -912,7 +921,8 @@ int64_t gram_work_len(int64_t n, int k)
     return (int64_t)nc * k * k;
 }
 
-int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *work, int64_t work_len)
+int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *work, int64_t work_len,
+         int symmetric)
 {
     SSR_REQUIRE(k >= 0 && k <= 16384, "gram: k out of range");
     if (k == 0) return SSR_OK;
@@ -924,13 +934,14 @@ int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *
     SSR_REQUIRE(work && work_len >= (int64_t)nc * kk, "gram: workspace too small");
     SSR_REQUIRE((int64_t)t * t <= 0x7fffffff && nc <= 65535, "gram: grid too large");
     hipLaunchKernelGGL(k_gram_partial, dim3((unsigned)(t * t), (unsigned)nc), dim3(256), 0, ssr_stream(), n, k, t,
-                       rpc, X, Y, work);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(ssr_blocks(kk, 256)), dim3(256), 0, ssr_stream(), nc, kk, work, G);
+                       rpc, X, Y, work, symmetric ? 1 : 0);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(ssr_blocks(kk, 256)), dim3(256), 0, ssr_stream(), nc, k, work, G,
+                       symmetric ? 1 : 0);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }
 
-int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y)
+int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y, const int *qstart)
 {
     SSR_REQUIRE(k >= 0 && m >= 0 && m <= 65535 * kTile, "gemm_tall: sizes out of range");
     if (n == 0 || m == 0) return SSR_OK;
@@ -938,7 +949,7 @@ int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double 
     const int64_t rb = (n + kTile - 1) / kTile;
     SSR_REQUIRE(rb <= 0x7fffffff, "gemm_tall: too many rows");
     hipLaunchKernelGGL(k_gemm_tall, dim3((unsigned)rb, (unsigned)((m + kTile - 1) / kTile)), dim3(256), 0,
-                       ssr_stream(), n, k, m, X, C, Y);
+                       ssr_stream(), n, k, m, X, C, Y, qstart);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

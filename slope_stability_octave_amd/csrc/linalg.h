@@ -19,9 +19,14 @@ int spmm_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
 // holds gram_work_len(n, k) doubles of per-chunk partial sums, added in a
 // fixed order (deterministic)
 int64_t gram_work_len(int64_t n, int k);
-int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *work, int64_t work_len);
-// Y (n x m) = X (n x k) C (k x m), all column-major (ld n, k, n)
-int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y);
+// (symmetric: X'Y is known to be symmetric, so only the tiles on and above
+// the diagonal are formed and the rest mirrored)
+int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *work, int64_t work_len,
+         int symmetric = 0);
+// Y (n x m) = X (n x k) C (k x m), all column-major (ld n, k, n); qstart
+// (device, one int per 64-column tile of C, or null) gives the first row of
+// C that is nonzero in that tile
+int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y, const int *qstart = nullptr);
 // y = alpha*A*x + beta*z on scalar CSR (64-bit row pointers); nnz (when
 // known) picks the lanes per row
 template <class T>

@@ -146,6 +146,7 @@ struct ssr_solver {
     DevBuf<float> V32, Z32, t2f;
     DevBuf<double> hcol, hW, ydev, scal;  // Arnoldi columns, deflation coefficients, LSQ solution, scalars
     DevBuf<double> AW, W2, AW2, Gd, Td, gwork;  // re-orthogonalisation: A W, new basis and its A W, Gram, coefficients
+    DevBuf<int> Tq;  // first nonzero row of the coefficients per 64-column tile
     // A W of the current basis as left by the re-orthogonalisation (valid
     // while kAW == kW: set_matrix, expand, clear and copy invalidate it).  For
     // a symmetric operator W'A x = (A W)' x, so the projector skips its SpMV.
@@ -592,7 +593,7 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
             return rc;
         s->n_spmv += nv;
     }
-    if ((rc = ssr::gram(n, k, s->W.p, s->AW.p, s->Gd.p, s->gwork.p, wl))) return rc;
+    if ((rc = ssr::gram(n, k, s->W.p, s->AW.p, s->Gd.p, s->gwork.p, wl, s->symmetric ? 1 : 0))) return rc;
     std::vector<double> G((size_t)k * k);  // column j = W'A w_j
     SSR_HIP_CHECK(hipMemcpyAsync(G.data(), s->Gd.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, ssr_stream()));
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
@@ -639,20 +640,29 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
         s->kAW = -1;
         return SSR_OK;
     }
-    // W_new(:, c) = W * T[acc-1-c]  (accepted columns back in forward order)
+    // W_new(:, c) = W * T[acc-1-c]  (accepted columns back in forward order);
+    // T is zero above each column's starting row, so each 64-column tile of
+    // the product starts at the smallest starting row of its columns
     std::vector<double> Th((size_t)k * acc);
-    for (int c = 0; c < acc; ++c)
+    const int nct = (acc + 63) / 64;
+    std::vector<int> qst(nct, k);
+    for (int c = 0; c < acc; ++c) {
         std::copy(Tm.begin() + (size_t)(acc - 1 - c) * k, Tm.begin() + (size_t)(acc - c) * k,
                   Th.begin() + (size_t)c * k);
-    if ((rc = s->Td.alloc((int64_t)k * acc)) || (rc = s->W2.alloc(n * (int64_t)s->capW))) return rc;
+        qst[c / 64] = std::min(qst[c / 64], lo[acc - 1 - c]);
+    }
+    if ((rc = s->Td.alloc((int64_t)k * acc)) || (rc = s->W2.alloc(n * (int64_t)s->capW)) ||
+        (rc = s->Tq.alloc(nct)))
+        return rc;
     SSR_HIP_CHECK(hipMemcpyAsync(s->Td.p, Th.data(), sizeof(double) * Th.size(), hipMemcpyHostToDevice, ssr_stream()));
-    if ((rc = ssr::gemm_tall(n, k, acc, s->W.p, s->Td.p, s->W2.p))) return rc;
+    SSR_HIP_CHECK(hipMemcpyAsync(s->Tq.p, qst.data(), sizeof(int) * qst.size(), hipMemcpyHostToDevice, ssr_stream()));
+    if ((rc = ssr::gemm_tall(n, k, acc, s->W.p, s->Td.p, s->W2.p, s->Tq.p))) return rc;
     std::swap(s->W.p, s->W2.p);
     std::swap(s->W.n, s->W2.n);
     if (s->symmetric) {
         // A (W T) = (A W) T: the projector of the solves that follow reads it
         if ((rc = s->AW2.alloc(n * (int64_t)s->capW))) return rc;
-        if ((rc = ssr::gemm_tall(n, k, acc, s->AW.p, s->Td.p, s->AW2.p))) return rc;
+        if ((rc = ssr::gemm_tall(n, k, acc, s->AW.p, s->Td.p, s->AW2.p, s->Tq.p))) return rc;
         std::swap(s->AW.p, s->AW2.p);
         std::swap(s->AW.n, s->AW2.n);
     }
