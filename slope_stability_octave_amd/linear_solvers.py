@@ -94,7 +94,11 @@ def spmv(pattern, K, x, y, alpha=1.0, beta=0.0):
 
 
 class DFGMRES:
-    """Deflated flexible GMRES with a GPU preconditioner (DFGMRES.m)."""
+    """Deflated flexible GMRES with a GPU preconditioner (DFGMRES.m).
+
+    ``symmetric`` (default True: every operator of the SSR path, K_r, is
+    symmetrised by construction) lets the deflation projector use the A W kept
+    by A_orthogonalize; pass False for a non-symmetric operator."""
 
     method = "dfgmres"
 
