@@ -26,6 +26,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <memory>
 #include <vector>
@@ -252,6 +253,7 @@ template <> DevLevel<float> &Level::dev<float>() { return d32; }
 double gershgorin_rho(const HCsr &A, const std::vector<double> &dinv)
 {
     double m = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : m)
     for (int64_t r = 0; r < A.nr; ++r) {
         double s = 0.0;
         for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) s += fabs(A.v[k]);
@@ -260,25 +262,42 @@ double gershgorin_rho(const HCsr &A, const std::vector<double> &dinv)
     return m > 0 ? m : 1.0;
 }
 
+// sum_i f(i) over fixed chunks of rows, the chunk sums added in order: the
+// same value for any thread count
+template <class F>
+double chunked_sum(int64_t n, F f)
+{
+    constexpr int64_t CH = 16384;
+    const int64_t nc = (n + CH - 1) / CH;
+    std::vector<double> part(nc > 0 ? nc : 1, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < nc; ++c) {
+        double a = 0.0;
+        const int64_t e = std::min(n, (c + 1) * CH);
+        for (int64_t i = c * CH; i < e; ++i) a += f(i);
+        part[c] = a;
+    }
+    double t = 0.0;
+    for (int64_t c = 0; c < nc; ++c) t += part[c];
+    return t;
+}
+
 double power_rho(const HCsr &A, const std::vector<double> &dinv, int its)
 {
     const int64_t n = A.nr;
     std::vector<double> x(n), y(n);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) x[i] = 1.0 + 0.1 * (double)((i * 7919) % 13);
     double lam = 0.0;
     for (int k = 0; k < its; ++k) {
-        double nx = 0.0;
-        for (double t : x) nx += t * t;
-        nx = sqrt(nx);
+        const double nx = sqrt(chunked_sum(n, [&](int64_t i) { return x[i] * x[i]; }));
         if (nx == 0.0) return 1.0;
-        for (auto &t : x) t /= nx;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) x[i] /= nx;
         spmv_host(A, x.data(), y.data());
-        double num = 0.0;
-        for (int64_t i = 0; i < n; ++i) {
-            y[i] *= dinv[i];
-            num += x[i] * y[i];
-        }
-        lam = num;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) y[i] *= dinv[i];
+        lam = chunked_sum(n, [&](int64_t i) { return x[i] * y[i]; });
         x.swap(y);
     }
     return lam > 0 ? lam : 1.0;
@@ -355,26 +374,37 @@ struct Amg : ssr_precond {
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
     }
 
+    // scalar CSR of the downloaded node-block operator (zeros dropped, the
+    // diagonal kept): count per row, prefix sum, fill, rows in parallel
     void fine_from_bsr(HCsr &A)
     {
         const int bs = dim * dim;
         A.nr = A.nc = n;
         A.rp.assign(n + 1, 0);
-        A.ci.clear();
-        A.v.clear();
-        A.ci.reserve((size_t)nnzb * bs);
-        A.v.reserve((size_t)nnzb * bs);
+        auto keep = [&](int64_t r, int i, int32_t b, int j, double v) { return v != 0.0 || (col[b] == r && i == j); };
+#pragma omp parallel for schedule(dynamic, 1024)
         for (int64_t r = 0; r < n_n; ++r)
             for (int i = 0; i < dim; ++i) {
+                int64_t c = 0;
+                for (int32_t b = row_ptr[r]; b < row_ptr[r + 1]; ++b)
+                    for (int j = 0; j < dim; ++j) c += keep(r, i, b, j, Kh[(size_t)b * bs + i * dim + j]) ? 1 : 0;
+                A.rp[r * dim + i + 1] = c;
+            }
+        for (int64_t q = 0; q < n; ++q) A.rp[q + 1] += A.rp[q];
+        A.ci.resize(A.rp[n]);
+        A.v.resize(A.rp[n]);
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t r = 0; r < n_n; ++r)
+            for (int i = 0; i < dim; ++i) {
+                int64_t o = A.rp[r * dim + i];
                 for (int32_t b = row_ptr[r]; b < row_ptr[r + 1]; ++b)
                     for (int j = 0; j < dim; ++j) {
                         const double v = Kh[(size_t)b * bs + i * dim + j];
-                        if (v != 0.0 || (col[b] == r && i == j)) {
-                            A.ci.push_back(col[b] * dim + j);
-                            A.v.push_back(v);
+                        if (keep(r, i, b, j, v)) {
+                            A.ci[o] = col[b] * dim + j;
+                            A.v[o++] = v;
                         }
                     }
-                A.rp[r * dim + i + 1] = (int64_t)A.ci.size();
             }
     }
 
@@ -384,8 +414,10 @@ struct Amg : ssr_precond {
     {
         const int bs = dim * dim;
         double mx = 0.0;
-        std::vector<double> Bi(bs), acc(dim);
+#pragma omp parallel for schedule(static) reduction(max : mx)
         for (int64_t r = 0; r < n_n; ++r) {
+            std::array<double, 9> Bi{};
+            std::array<double, 3> acc{};
             const double *D = nullptr;
             for (int32_t b = row_ptr[r]; b < row_ptr[r + 1]; ++b)
                 if (col[b] == r) D = Kh.data() + (size_t)b * bs;
@@ -419,6 +451,7 @@ struct Amg : ssr_precond {
     static void diag_inv(const HCsr &A, std::vector<double> &d)
     {
         d.assign(A.nr, 1.0);
+#pragma omp parallel for schedule(static)
         for (int64_t r = 0; r < A.nr; ++r)
             for (int64_t t = A.rp[r]; t < A.rp[r + 1]; ++t)
                 if (A.ci[t] == r) d[r] = A.v[t] != 0.0 ? 1.0 / A.v[t] : 1.0;
