@@ -124,7 +124,7 @@ def run_fos(args):
         params["lambda_init"] = args.lambda_init
     amg_opts = amg_options(args)
     res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts,
-                           precision=args.precision)
+                           precision=args.precision, driver=args.newton_driver)
     out = {
         "metric": "SSR time-to-FoS (%s)" % ("plain GPU form, FP64" if args.precision == 64
                                              else "FP32 Krylov/AMG + FP64 refinement"),
@@ -138,7 +138,8 @@ def run_fos(args):
                    "free_dof": prob["n_dof_free"], "int_points": prob["n_int"],
                    "lambda_init": params["lambda_init"], "omega_max_stop": params["omega_max_stop"],
                    "preconditioner": args.precond, "krylov_precision": args.precision,
-                   "linear_tol": params["linear_tol"], "amg_options": dict(amg_opts or {})},
+                   "linear_tol": params["linear_tol"], "amg_options": dict(amg_opts or {}),
+                   "newton_driver": args.newton_driver},
         "FoS": res["FoS"],
         "steps": len(res["lambda_hist"]),
         "newton_iterations": res["newton_iterations"],
@@ -381,7 +382,9 @@ def main():
     ap.add_argument("--omega-max", type=float, default=7e7)
     ap.add_argument("--precond", default="amg", choices=["amg", "block_jacobi"])
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--variant", type=int, default=None, help="3D constitutive kernel variant (0 direct, 1 LDS)")
+    ap.add_argument("--newton-driver", default="python", choices=["python", "native"],
+                    help="FoS workloads: Newton loop in Python over the C ABI, or inside the library")
+    ap.add_argument("--variant", type=int, default=None, help="3D constitutive kernel form (tuning key 0; 11 is the default)")
     ap.add_argument("--linear-tol", type=float, default=None)
     ap.add_argument("--amg-rebuild-every", type=int, default=None,
                     help="rebuild the AMG hierarchy every n-th Newton iteration (0: once per continuation step)")

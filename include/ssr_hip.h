@@ -104,9 +104,20 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
                       const double *shear, const double *bulk, const double *lame, double *Psi);
 
 /* ---- tuning ------------------------------------------------------------- */
-/* key 0: 3D constitutive kernel variant (0 direct point-major stores,
- *        1 LDS-staged coalesced, default); key 1: lanes per block row in the
- *        node-block SpMV (4, 8 = default, 16).  No reference counterpart. */
+/* Kernel forms and launch shapes, all parity-identical (no reference
+ * counterpart; autotune.py picks them on the caller's data):
+ *  key 0: 3D constitutive kernel form (11 = default: register inputs, LDS-staged
+ *         S and DS, non-temporal stores; the other forms are listed in
+ *         constitutive.hip);
+ *  key 1: lanes per block row of the node-block SpMV (0 = by dimension,
+ *         4, 8, 16, 32; 64 = one wavefront per row);
+ *  key 2: wavefronts per compute unit of persistent grids (0 = automatic);
+ *  key 3: 1 = replay the AMG V-cycle as a captured HIP graph (0 = eager, default);
+ *  key 4: 1 = node-block Jacobi scaling in the finest-level Chebyshev smoother
+ *         (default), 0 = scalar;
+ *  key 5: assembly and internal-force form (0 = element contributions, then
+ *         a deterministic per-block / per-node gather, default; 1 = FP64 atomics);
+ *  key 6: lanes per block row of the multi-vector SpMV (0 = by dimension). */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 
@@ -233,6 +244,54 @@ int ssr_solver_dcg(ssr_solver *s, const double *b, double *x, int *iters, double
 int ssr_solver_precond_apply(ssr_solver *s, const double *x, double *y);
 /* out[4] = total iterations, solves, preconditioner applications, SpMVs */
 int ssr_solver_stats(const ssr_solver *s, int64_t *out);
+
+/* ---- Newton step of the indirect SSR continuation ---------------------------- */
+/* The device state of the CONSTITUTIVE handle class (CONSTITUTIVE.m) that the
+ * Newton loop touches.  All pointers are device memory owned by the caller
+ * except phi_host/psi_host (host, n_int each, read only for Davis C). */
+typedef struct ssr_constitutive {
+    int dim;     /* 2 or 3 */
+    int n_q;     /* integration points per element; n_int = n_e * n_q */
+    int davis;   /* 'A', 'B' or 'C' (reduction.m) */
+    int s_rows;  /* rows of S per point: 6 in 3D, 4 in 2D */
+    int64_t n_int;
+    const ssr_pattern *pattern;               /* with the Dirichlet mask of Q */
+    const double *DPhi1, *DPhi2, *DPhi3;      /* n_p x n_int; DPhi3 NULL in 2D */
+    const double *WEIGHT;                     /* n_int */
+    const double *c0, *phi, *psi;             /* n_int, unreduced material */
+    const double *shear, *bulk, *lame;        /* n_int */
+    const double *phi_host, *psi_host;        /* host copies of phi, psi */
+    double *lambda;                           /* 1 double: lambda of the reduction */
+    double *c_bar, *sin_phi;                  /* n_int, reduced material */
+    double *E, *S, *DS;                       /* n_strain, s_rows, n_strain^2 per point */
+    const double *K_elast;                    /* block values of the elastic operator */
+    double *K_tang, *K_r;                     /* block values, overwritten */
+    const double *mask;                       /* dim*n_n: 1 at free DOFs, 0 at Dirichlet DOFs */
+    const double *f;                          /* dim*n_n load vector */
+} ssr_constitutive;
+
+#define SSR_NEWTON_UNKNOWN 0
+#define SSR_NEWTON_CONVERGED 1
+#define SSR_NEWTON_STALLED 2 /* 'stalled_regularization' */
+#define SSR_NEWTON_MAXIT 3
+#define SSR_NEWTON_NAN 4
+
+/* [U_it, lambda_it, flag_N, it, history] = newton_ind_SSR(U_ini, omega, lambda_ini,
+ *     it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, constitutive_matrix_builder,
+ *     linear_solver)
+ * replaces slope_stability/+NEWTON/newton_ind_SSR.m:1-206 with damping_ALG5.m:1-60;
+ * the whole Newton loop runs in the library.  method 0 = DFGMRES, 1 = DCG on
+ * `solver` (whose deflation basis it expands, DFGMRES.m:163-170).  U_ini and
+ * U_it are device dim*n_n fields (they may be the same array); work is device
+ * scratch of ssr_newton_work_len doubles, disjoint from every other argument.  hist (host, 6 x it_newt_max, or
+ * NULL) receives per iteration: relative residual, r, alpha, Krylov
+ * iterations for W and for V, deflation basis size; NaN where an iteration
+ * stopped before that point.  A NaN residual fails the solve (flag_N = 1). */
+int ssr_newton_work_len(const ssr_constitutive *C, int64_t *len);
+int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini,
+                       double omega, double lambda_ini, int it_newt_max, int it_damp_max, double tol,
+                       double r_min, double *work, int64_t work_len, double *U_it, double *lambda_out,
+                       int *flag_N, int *it, int *status, double *hist);
 
 /* ---- learned continuation predictor (small MLP) --------------------------- */
 /* No reference counterpart: the reference continuation has the secant

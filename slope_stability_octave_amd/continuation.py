@@ -91,8 +91,10 @@ def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_mi
 
 def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
                               omega_max_stop, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
-                              linear_system_solver, log=None, predictor=None, recorder=None):
-    """Returns (U, lambda_hist, omega_hist, Umax_hist, stats)."""
+                              linear_system_solver, log=None, predictor=None, recorder=None, driver="python"):
+    """Returns (U, lambda_hist, omega_hist, Umax_hist, stats).  ``driver``
+    picks the Newton loop: "python" (newton.py) or "native" (the same loop in
+    libssr_hip.so)."""
     t_total = time.perf_counter()
     ws = Workspace(cmb.n_n, cmb.dim)
     mask_ready = cmb.pattern_initialized or cmb.init_K_r_pattern(Q) or True  # noqa: F841
@@ -142,7 +144,7 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
         t0 = time.perf_counter()
         U_it, lambda_it, flag, it_newt, hist = newton_ind_SSR(
             U_ini, omega_it, lambda_ini, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
-            linear_system_solver.copy(), ws, log)
+            linear_system_solver.copy(), ws, log, driver=driver)
         wall = time.perf_counter() - t0
         stats["attempt_step"].append(step + 1)
         stats["attempt_success"].append(flag == 0)

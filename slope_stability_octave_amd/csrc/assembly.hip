@@ -34,6 +34,9 @@ const double *ssr_pattern_d_mask(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_cptr(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_ccode(const ssr_pattern *p);
 int ssr_pattern_keblk(const ssr_pattern *p, double **out);
+const int32_t *ssr_pattern_d_nptr(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_ncode(const ssr_pattern *p);
+int ssr_pattern_fe(const ssr_pattern *p, double **out);
 void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e,
                        int64_t *nnzb, int64_t *nnz_free);
 
@@ -252,10 +255,11 @@ __global__ __launch_bounds__(256) void k_strain(int64_t n_e, const int32_t *__re
     for (int r = 0; r < V::NS; ++r) E[g * V::NS + r] = eps[r];
 }
 
-// Internal force: one thread per element, F (dim x n_n) accumulated with
-// atomics; S has s_rows rows per point (6 in 3D; 4 in 2D where sigma33 is
-// not used), F must be zeroed first.
-template <int DIM, int NP, int NQ>
+// Internal force: one thread per (element, node); S has s_rows rows per
+// point (6 in 3D; 4 in 2D where sigma33 is not used).  GATHER: the thread
+// writes its node force to out[(e*NP + a)*DIM + c] for k_gather_force;
+// otherwise it adds it to F (dim x n_n, zeroed first) with atomics.
+template <int DIM, int NP, int NQ, bool GATHER>
 __global__ __launch_bounds__(256) void k_force(int64_t n_e, const int32_t *__restrict__ elem,
                                                const double *__restrict__ D1,
                                                const double *__restrict__ D2,
@@ -290,9 +294,31 @@ __global__ __launch_bounds__(256) void k_force(int64_t n_e, const int32_t *__res
             f[c] += v;
         }
     }
-    const int64_t node = elem[e * NP + a];
+    if (GATHER) {
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) atomic_add_f64(F + node * DIM + c, f[c]);
+        for (int c = 0; c < DIM; ++c) F[t * DIM + c] = f[c];
+    } else {
+        const int64_t node = elem[e * NP + a];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) atomic_add_f64(F + node * DIM + c, f[c]);
+    }
+}
+
+// F(c, v) = sum of the element-node forces fe of node v in element order
+// (nptr/ncode from the pattern): the same sum, bit for bit, on every run.
+// One thread per (node, component).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_gather_force(int64_t n_n, const int32_t *__restrict__ nptr,
+                                                      const int32_t *__restrict__ ncode,
+                                                      const double *__restrict__ fe, double *__restrict__ F)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_n * DIM) return;
+    const int64_t v = t / DIM;
+    const int c = (int)(t - v * DIM);
+    double s = 0.0;
+    for (int32_t k = nptr[v]; k < nptr[v + 1]; ++k) s += fe[(int64_t)ncode[k] * DIM + c];
+    F[t] = s;
 }
 
 // K_r = r K_el + (1-r) K_t, symmetrised through the transpose block, then
@@ -490,17 +516,42 @@ int ssr_dev_force(const ssr_pattern *P, int n_q, const double *DPhi1, const doub
     int dim;
     int64_t n_n, n_p, n_e, nnzb, nnzf;
     ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
-    SSR_HIP_CHECK(hipMemsetAsync(F, 0, sizeof(double) * dim * n_n, ssr_stream()));
-    const int th = 256;
-    if (dim == 3) {
+    if (dim == 3)
         SSR_REQUIRE(n_p == 10 && n_q == 11 && s_rows >= 6, "force: P2 tetrahedra with 11 points supported");
-        hipLaunchKernelGGL((k_force<3, 10, 11>), dim3(ssr_blocks(n_e * 10, th)), dim3(th), 0, ssr_stream(), n_e,
-                           ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, S, s_rows, WEIGHT, F);
-    } else {
+    else
         SSR_REQUIRE(n_p == 6 && n_q == 7 && s_rows >= 3, "force: P2 triangles with 7 points supported");
-        hipLaunchKernelGGL((k_force<2, 6, 7>), dim3(ssr_blocks(n_e * 6, th)), dim3(th), 0, ssr_stream(), n_e,
-                           ssr_pattern_d_elem(P), DPhi1, DPhi2, DPhi3, S, s_rows, WEIGHT, F);
+    const int th = 256;
+    const unsigned grid = ssr_blocks(n_e * n_p, th);
+    const int32_t *elem = ssr_pattern_d_elem(P);
+    if (ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) == 0 && ssr_pattern_d_nptr(P)) {
+        // deterministic: element-node forces, then one ordered sum per node
+        double *fe = nullptr;
+        int rc = ssr_pattern_fe(P, &fe);
+        if (rc) return rc;
+        if (dim == 3)
+            hipLaunchKernelGGL((k_force<3, 10, 11, true>), dim3(grid), dim3(th), 0, ssr_stream(), n_e, elem, DPhi1,
+                               DPhi2, DPhi3, S, s_rows, WEIGHT, fe);
+        else
+            hipLaunchKernelGGL((k_force<2, 6, 7, true>), dim3(grid), dim3(th), 0, ssr_stream(), n_e, elem, DPhi1,
+                               DPhi2, DPhi3, S, s_rows, WEIGHT, fe);
+        SSR_HIP_CHECK(hipGetLastError());
+        const unsigned g2 = ssr_blocks(n_n * dim, th);
+        if (dim == 3)
+            hipLaunchKernelGGL(k_gather_force<3>, dim3(g2), dim3(th), 0, ssr_stream(), n_n, ssr_pattern_d_nptr(P),
+                               ssr_pattern_d_ncode(P), fe, F);
+        else
+            hipLaunchKernelGGL(k_gather_force<2>, dim3(g2), dim3(th), 0, ssr_stream(), n_n, ssr_pattern_d_nptr(P),
+                               ssr_pattern_d_ncode(P), fe, F);
+        SSR_HIP_CHECK(hipGetLastError());
+        return SSR_OK;
     }
+    SSR_HIP_CHECK(hipMemsetAsync(F, 0, sizeof(double) * dim * n_n, ssr_stream()));
+    if (dim == 3)
+        hipLaunchKernelGGL((k_force<3, 10, 11, false>), dim3(grid), dim3(th), 0, ssr_stream(), n_e, elem, DPhi1,
+                           DPhi2, DPhi3, S, s_rows, WEIGHT, F);
+    else
+        hipLaunchKernelGGL((k_force<2, 6, 7, false>), dim3(grid), dim3(th), 0, ssr_stream(), n_e, elem, DPhi1,
+                           DPhi2, DPhi3, S, s_rows, WEIGHT, F);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

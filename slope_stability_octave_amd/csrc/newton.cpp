@@ -1,0 +1,268 @@
+// Bordered semismooth Newton of the indirect SSR continuation, driven from
+// the library: newton_ind_SSR.m:1-206 with damping_ALG5.m:1-60.
+//
+// Every field (U, F, f, V, W, d_U and the constitutive work arrays) stays in
+// device memory; the host keeps only the scalars the control flow needs (the
+// residual norms, f'V, f'W, f'U and the damping factor), each read back with
+// one synchronising reduction, as the Octave loop reads them.  The calls are
+// the library's own C entry points, so the kernels and their order are those
+// of the Python driver (newton.py) and the two give the same numbers.
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "ssr_common.h"
+#include "../../include/ssr_hip.h"
+
+namespace {
+
+#define NCHECK(expr)               \
+    do {                           \
+        int _rc = (expr);          \
+        if (_rc != SSR_OK) return _rc; \
+    } while (0)
+
+struct NewtonCtx {
+    const ssr_constitutive *C;
+    ssr_solver *solver;
+    int method;
+    int64_t n_dof;
+    double *F, *F_eps, *G, *rhs, *V, *W, *dU, *Ua, *Fa, *tmp;
+};
+
+// CONSTITUTIVE.m:178-200 / reduction.m:25-58 at a scalar lambda
+int reduction(const NewtonCtx &x, double lam)
+{
+    const ssr_constitutive *C = x.C;
+    int c_all = 1;
+    if (C->davis == 'C') {
+        // Octave's vector `if` of reduction.m:47 holds only when every point has
+        // atan(tan(phi)/lambda) > psi
+        for (int64_t i = 0; i < C->n_int; ++i)
+            if (!(atan(tan(C->phi_host[i]) / lam) > C->psi_host[i])) {
+                c_all = 0;
+                break;
+            }
+    }
+    NCHECK(ssr_htod(C->lambda, &lam, sizeof(double)));
+    return ssr_dev_reduction(C->n_int, C->davis, C->c0, C->phi, C->psi, C->lambda, 0, c_all, C->c_bar,
+                             C->sin_phi);
+}
+
+// build_F_all / build_F_and_DS_all (CONSTITUTIVE.m:202-285): reduction, E = B U,
+// S (and DS), F = B'(w S)
+int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with_ds)
+{
+    const ssr_constitutive *C = x.C;
+    NCHECK(reduction(x, lam));
+    NCHECK(ssr_dev_strain(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, U, C->E));
+    if (C->dim == 3)
+        NCHECK(ssr_dev_constitutive_3D(C->n_int, C->E, C->c_bar, C->sin_phi, C->shear, C->bulk, C->lame, C->S,
+                                       with_ds ? C->DS : nullptr, nullptr));
+    else
+        NCHECK(ssr_dev_constitutive_2D(C->n_int, C->E, C->c_bar, C->sin_phi, C->shear, C->bulk, C->lame, C->S,
+                                       with_ds ? C->DS : nullptr, nullptr));
+    return ssr_dev_force(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->S, C->s_rows, C->WEIGHT, F);
+}
+
+// ||a(Q) - b(Q)||
+int norm_Q(const NewtonCtx &x, const double *a, const double *b, double *out)
+{
+    NCHECK(ssr_dev_lincomb(x.n_dof, 1.0, a, -1.0, b, x.tmp));
+    double s = 0.0;
+    NCHECK(ssr_dot(x.n_dof, x.tmp, x.tmp, x.C->mask, &s));
+    *out = sqrt(s > 0.0 ? s : 0.0);
+    return SSR_OK;
+}
+
+// damping_ALG5.m:26-58: halve alpha until the residual norm decreases.  A NaN
+// trial residual counts as no decrease (Octave's `>=` would accept it).
+int damping_ALG5(const NewtonCtx &x, int it_damp_max, const double *U_it, double lambda_it, double d_l,
+                 double criterion, double *alpha_out)
+{
+    if (isnan(d_l)) {
+        *alpha_out = 0.0;
+        return SSR_OK;
+    }
+    double alpha = 1.0;
+    int it_damp = 0;
+    while (true) {
+        ++it_damp;
+        NCHECK(ssr_dev_lincomb(x.n_dof, 1.0, U_it, alpha, x.dU, x.Ua));
+        NCHECK(build_F(x, lambda_it + alpha * d_l, x.Ua, x.Fa, 0));
+        double crit_alpha;
+        NCHECK(norm_Q(x, x.Fa, x.C->f, &crit_alpha));
+        if (crit_alpha >= criterion || isnan(crit_alpha))
+            alpha = alpha / 2;
+        else
+            break;
+        if (it_damp >= it_damp_max) {
+            alpha = 0.0;
+            break;
+        }
+    }
+    *alpha_out = alpha;
+    return SSR_OK;
+}
+
+int solve(const NewtonCtx &x, const double *b, double *sol, int *iters)
+{
+    double rr = 0.0;
+    if (x.method == 1) return ssr_solver_dcg(x.solver, b, sol, iters, &rr);
+    return ssr_solver_dfgmres(x.solver, b, sol, iters, &rr);
+}
+
+}  // namespace
+
+extern "C" int ssr_newton_work_len(const ssr_constitutive *C, int64_t *len)
+{
+    SSR_REQUIRE(C && C->pattern && len, "newton_work_len: null argument");
+    int64_t info[8];
+    NCHECK(ssr_pattern_info(C->pattern, info));
+    *len = 10 * info[0] * info[1];
+    return SSR_OK;
+}
+
+extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini,
+                                  double omega, double lambda_ini, int it_newt_max, int it_damp_max, double tol,
+                                  double r_min, double *work, int64_t work_len, double *U_it, double *lambda_out,
+                                  int *flag_out, int *it_out, int *status_out, double *hist)
+{
+    SSR_REQUIRE(C && solver && U_ini && work && U_it && lambda_out && flag_out && it_out && status_out,
+                "newton_ind_SSR: null argument");
+    SSR_REQUIRE(C->pattern && C->DPhi1 && C->DPhi2 && (C->dim == 2 || C->DPhi3) && C->WEIGHT && C->c0 &&
+                    C->phi && C->psi && C->shear && C->bulk && C->lame && C->lambda && C->c_bar && C->sin_phi &&
+                    C->E && C->S && C->DS && C->K_elast && C->K_tang && C->K_r && C->mask && C->f,
+                "newton_ind_SSR: incomplete constitutive description");
+    SSR_REQUIRE(C->dim == 2 || C->dim == 3, "newton_ind_SSR: dim must be 2 or 3");
+    SSR_REQUIRE(C->s_rows == (C->dim == 3 ? 6 : 4), "newton_ind_SSR: S has 6 rows in 3D, 4 in 2D");
+    SSR_REQUIRE(C->davis == 'A' || C->davis == 'B' || C->davis == 'C', "Incorrect choice of the Davis approach.");
+    SSR_REQUIRE(C->davis != 'C' || (C->phi_host && C->psi_host), "newton_ind_SSR: Davis C needs phi, psi on the host");
+    SSR_REQUIRE(method == 0 || method == 1, "newton_ind_SSR: method 0 (DFGMRES) or 1 (DCG)");
+    SSR_REQUIRE(it_newt_max >= 1 && it_damp_max >= 1, "newton_ind_SSR: iteration limits must be positive");
+    int64_t info[8];
+    NCHECK(ssr_pattern_info(C->pattern, info));
+    SSR_REQUIRE(info[0] == C->dim, "newton_ind_SSR: pattern dimension %lld != %d", (long long)info[0], C->dim);
+    SSR_REQUIRE(info[3] > 0 && C->n_q > 0 && C->n_int == info[3] * C->n_q,
+                "newton_ind_SSR: n_int %lld != n_e %lld x n_q %d", (long long)C->n_int, (long long)info[3], C->n_q);
+    const int64_t n_dof = info[0] * info[1];
+    SSR_REQUIRE(work_len >= 10 * n_dof, "newton_ind_SSR: work holds %lld doubles, %lld needed",
+                (long long)work_len, (long long)(10 * n_dof));
+
+    NewtonCtx x;
+    x.C = C;
+    x.solver = solver;
+    x.method = method;
+    x.n_dof = n_dof;
+    double **slots[10] = {&x.F, &x.F_eps, &x.G, &x.rhs, &x.V, &x.W, &x.dU, &x.Ua, &x.Fa, &x.tmp};
+    for (int k = 0; k < 10; ++k) *slots[k] = work + k * n_dof;
+
+    // newton_ind_SSR.m:22-33
+    NCHECK(ssr_dev_lincomb(n_dof, 1.0, U_ini, 0.0, U_ini, U_it));
+    double lambda_it = lambda_ini;
+    const double eps = tol / 1000;
+    double ff = 0.0;
+    NCHECK(ssr_dot(n_dof, C->f, C->f, C->mask, &ff));
+    const double norm_f = sqrt(ff);
+    int flag_N = 0;
+    bool compute_diffs = true;
+    double r = r_min;
+    int it = 0;
+    double rel_resid = NAN, criterion = NAN;
+    int status = SSR_NEWTON_UNKNOWN;
+    if (hist)
+        for (int64_t k = 0; k < 6 * (int64_t)it_newt_max; ++k) hist[k] = NAN;
+
+    // newton_ind_SSR.m:68-175
+    while (true) {
+        ++it;
+        double *h = hist ? hist + 6 * (int64_t)(it - 1) : nullptr;
+        if (compute_diffs) {
+            NCHECK(build_F(x, lambda_it, U_it, x.F, 1));
+            NCHECK(norm_Q(x, x.F, C->f, &criterion));
+            rel_resid = criterion / norm_f;
+            if (h) h[0] = rel_resid;
+            if (rel_resid < tol && it > 1) {
+                status = SSR_NEWTON_CONVERGED;
+                break;
+            }
+        } else if (h) {
+            h[0] = rel_resid;
+        }
+        if (h) h[1] = r;
+        // K_r (CONSTITUTIVE.m:487-532)
+        NCHECK(ssr_dev_assemble_bsr(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT,
+                                    C->K_tang));
+        NCHECK(ssr_dev_regularize(C->pattern, r, C->K_elast, C->K_tang, C->K_r));
+        // G = (F(lambda + eps) - F) / eps   (:94-96)
+        NCHECK(build_F(x, lambda_it + eps, U_it, x.F_eps, 0));
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0 / eps, x.F_eps, -1.0 / eps, x.F, x.G));
+        // preconditioner and deflation basis for K_r (:99-110)
+        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
+        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 0));
+        NCHECK(ssr_solver_A_orthogonalize(solver));
+        if (h) {
+            int k = 0;
+            NCHECK(ssr_solver_deflation_size(solver, &k));
+            h[5] = k;
+        }
+        // W = K \ (-G(Q)),  V = K \ (f(Q) - F(Q))   (:112-115)
+        int lin_w = 0, lin_v = 0;
+        NCHECK(ssr_dev_lincomb(n_dof, -1.0, x.G, 0.0, x.G, x.rhs));
+        NCHECK(ssr_dev_mul(n_dof, C->mask, x.rhs));
+        NCHECK(solve(x, x.rhs, x.W, &lin_w));
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0, C->f, -1.0, x.F, x.rhs));
+        NCHECK(ssr_dev_mul(n_dof, C->mask, x.rhs));
+        NCHECK(solve(x, x.rhs, x.V, &lin_v));
+        if (h) {
+            h[3] = lin_w;
+            h[4] = lin_v;
+        }
+        // d_l = -f'V / f'W,  d_U = V + d_l W   (:121-122)
+        double fV = 0.0, fW = 0.0;
+        NCHECK(ssr_dot(n_dof, C->f, x.V, C->mask, &fV));
+        NCHECK(ssr_dot(n_dof, C->f, x.W, C->mask, &fW));
+        const double d_l = fW != 0.0 ? -fV / fW : NAN;
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0, x.V, d_l, x.W, x.dU));
+        double alpha = 0.0;
+        NCHECK(damping_ALG5(x, it_damp_max, U_it, lambda_it, d_l, criterion, &alpha));
+        if (h) h[2] = alpha;
+        // regularisation and deflation-basis update (:129-143)
+        compute_diffs = true;
+        if (alpha < 1e-1) {
+            if (alpha == 0) {
+                compute_diffs = false;
+                r = r * 2;
+            } else {
+                r = r * pow(2.0, 1.0 / 4);
+            }
+        } else {
+            NCHECK(ssr_solver_expand_deflation(solver, x.W));
+            NCHECK(ssr_solver_expand_deflation(solver, x.V));
+            if (alpha > 0.5) r = fmax(r / sqrt(2.0), r_min);
+        }
+        if (alpha == 0 && r > 1) {
+            status = SSR_NEWTON_STALLED;
+            if (rel_resid > 10 * tol) flag_N = 1;
+            break;
+        }
+        // U += alpha d_U; U = omega U / (f(Q)'U(Q)); lambda += alpha d_l   (:154-156)
+        NCHECK(ssr_dev_axpby(n_dof, alpha, x.dU, 1.0, U_it));
+        double fU = 0.0;
+        NCHECK(ssr_dot(n_dof, C->f, U_it, C->mask, &fU));
+        NCHECK(ssr_dev_axpby(n_dof, 0.0, U_it, omega / fU, U_it));
+        lambda_it = lambda_it + alpha * d_l;
+        // (:164-174; a NaN residual fails the solve)
+        if (isnan(rel_resid) || it == it_newt_max) {
+            if (rel_resid > 10 * tol || isnan(rel_resid)) flag_N = 1;
+            status = isnan(rel_resid) ? SSR_NEWTON_NAN : SSR_NEWTON_MAXIT;
+            break;
+        }
+    }
+    NCHECK(ssr_synchronize());
+    *lambda_out = lambda_it;
+    *flag_out = flag_N;
+    *it_out = it;
+    *status_out = status;
+    return SSR_OK;
+}

@@ -44,6 +44,11 @@ struct ssr_pattern {
     std::vector<int32_t> cptr, ccode;
     int32_t *d_cptr = nullptr, *d_ccode = nullptr;
     mutable double *d_keblk = nullptr;  // n_e * n_pair * dim * dim element blocks (allocated on first use)
+    // gather-form internal force: the element nodes (code = e * n_p + a) that
+    // sum into each node, in element order
+    std::vector<int32_t> nptr, ncode;
+    int32_t *d_nptr = nullptr, *d_ncode = nullptr;
+    mutable double *d_fe = nullptr;  // n_e * n_p * dim element-node forces (allocated on first use)
 };
 
 namespace {
@@ -75,6 +80,9 @@ int ssr_pattern_destroy(ssr_pattern *p)
     (void)hipFree(p->d_cptr);
     (void)hipFree(p->d_ccode);
     (void)hipFree(p->d_keblk);
+    (void)hipFree(p->d_nptr);
+    (void)hipFree(p->d_ncode);
+    (void)hipFree(p->d_fe);
     (void)hipFree(p->d_tblk);
     (void)hipFree(p->d_diag);
     (void)hipFree(p->d_elem);
@@ -179,6 +187,16 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
         }
     }
 
+    // node incidences for the gather-form internal force
+    if (n_e * n_p < ((int64_t)1 << 31)) {
+        p->nptr.assign(n_n + 1, 0);
+        for (int64_t k = 0; k < n_e * n_p; ++k) p->nptr[elem[k] + 1]++;
+        for (int64_t v = 0; v < n_n; ++v) p->nptr[v + 1] += p->nptr[v];
+        p->ncode.resize(n_e * n_p);
+        std::vector<int32_t> fill(p->nptr.begin(), p->nptr.end() - 1);
+        for (int64_t k = 0; k < n_e * n_p; ++k) p->ncode[fill[elem[k]]++] = (int32_t)k;
+    }
+
     // free DOFs and the reference-order extraction map
     p->mask.assign(Q, Q + dim * n_n);
     for (auto &m : p->mask) m = m ? 1 : 0;
@@ -211,7 +229,8 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
             (rc = upload(&p->d_diag, p->diag)) || (rc = upload(&p->d_elem, el32)) ||
             (rc = upload(&p->d_ref_pos, p->ref_pos)) || (rc = upload(&p->d_mask, md)) ||
             (rc = upload(&p->d_brow, p->brow)) ||
-            (!p->cptr.empty() && ((rc = upload(&p->d_cptr, p->cptr)) || (rc = upload(&p->d_ccode, p->ccode))))) {
+            (!p->cptr.empty() && ((rc = upload(&p->d_cptr, p->cptr)) || (rc = upload(&p->d_ccode, p->ccode)))) ||
+            (!p->nptr.empty() && ((rc = upload(&p->d_nptr, p->nptr)) || (rc = upload(&p->d_ncode, p->ncode))))) {
             ssr_pattern_destroy(p);
             return rc;
         }
@@ -275,6 +294,18 @@ int ssr_pattern_keblk(const ssr_pattern *p, double **out)
         SSR_HIP_CHECK(hipMalloc((void **)&p->d_keblk, sizeof(double) * (len > 0 ? len : 1)));
     }
     *out = p->d_keblk;
+    return SSR_OK;
+}
+const int32_t *ssr_pattern_d_nptr(const ssr_pattern *p) { return p->d_nptr; }
+const int32_t *ssr_pattern_d_ncode(const ssr_pattern *p) { return p->d_ncode; }
+// element-node force scratch of the gather-form internal force (allocated once)
+int ssr_pattern_fe(const ssr_pattern *p, double **out)
+{
+    if (!p->d_fe) {
+        const int64_t len = p->n_e * p->n_p * p->dim;
+        SSR_HIP_CHECK(hipMalloc((void **)&p->d_fe, sizeof(double) * (len > 0 ? len : 1)));
+    }
+    *out = p->d_fe;
     return SSR_OK;
 }
 void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e,

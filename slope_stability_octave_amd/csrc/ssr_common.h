@@ -65,8 +65,9 @@ struct ssr_precond {
 #define SSR_TUNE_PERSIST_WAVES 2        /* wavefronts per compute unit of persistent grids (0: auto) */
 #define SSR_TUNE_AMG_GRAPH 3            /* 1: replay the V-cycle as a captured HIP graph, 0: eager launches */
 #define SSR_TUNE_AMG_BLOCK_SMOOTH 4     /* 1: node-block Jacobi scaling in the finest-level Chebyshev smoother */
-#define SSR_TUNE_ASSEMBLY_FORM 5        /* 0: element blocks, then one deterministic gather per block;
-                                           1: FP64 atomic adds of the element blocks */
+#define SSR_TUNE_ASSEMBLY_FORM 5        /* 0: element blocks (and element-node forces), then one
+                                           deterministic gather per block (per node);
+                                           1: FP64 atomic adds of the element contributions */
 #define SSR_TUNE_SPMM_GROUP 6           /* lanes per block row of the multi-vector SpMV (0: by dimension) */
 #define SSR_TUNE_COUNT 8
 int ssr_tuning(int key);

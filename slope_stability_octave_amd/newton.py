@@ -11,16 +11,83 @@ Every vector (U, F, f, V, W, d_U) is a DeviceArray of shape (n_n, dim);
 reductions (norms, f'V) return scalars to the host, which owns the control
 flow exactly as the reference's Octave loop does.  Restriction to the free
 DOFs Q is the Dirichlet mask (zero at fixed DOFs).
+
+``newton_ind_SSR(..., driver="native")`` runs the same loop inside
+libssr_hip.so (``ssr_newton_ind_SSR``): one library call per Newton solve
+instead of a few dozen per iteration, with the same kernels in the same
+order, so both drivers give the same iterates.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 
 import numpy as np
 
-from ._lib import DeviceArray
+from . import _lib
+from ._lib import DeviceArray, call, ptr
 from .linear_solvers import axpby, dot, lincomb
+
+_vp = ctypes.c_void_p
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_i64 = ctypes.c_int64
+
+_lib.extend_signatures({
+    "ssr_newton_work_len": [_vp, _vp],
+    "ssr_newton_ind_SSR": [_vp, _vp, _int, _vp, _dbl, _dbl, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp,
+                           _vp, _vp, _vp],
+})
+
+DRIVERS = ("python", "native")
+NEWTON_STATUS = {0: "unknown", 1: "converged", 2: "stalled_regularization", 3: "max_iterations",
+                 4: "nan_residual"}
+
+
+class _Constitutive(ctypes.Structure):
+    """ctypes image of ssr_constitutive (include/ssr_hip.h)."""
+
+    _fields_ = [("dim", _int), ("n_q", _int), ("davis", _int), ("s_rows", _int), ("n_int", _i64),
+                ("pattern", _vp), ("DPhi1", _vp), ("DPhi2", _vp), ("DPhi3", _vp), ("WEIGHT", _vp),
+                ("c0", _vp), ("phi", _vp), ("psi", _vp), ("shear", _vp), ("bulk", _vp), ("lame", _vp),
+                ("phi_host", _vp), ("psi_host", _vp), ("lambda_", _vp), ("c_bar", _vp), ("sin_phi", _vp),
+                ("E", _vp), ("S", _vp), ("DS", _vp), ("K_elast", _vp), ("K_tang", _vp), ("K_r", _vp),
+                ("mask", _vp), ("f", _vp)]
+
+
+def native_description(cmb, f):
+    """The ssr_constitutive view of a CONSTITUTIVE object and load vector f.
+    The arrays stay owned by ``cmb`` and ``f``; the host copies of phi and psi
+    are kept on ``cmb`` for as long as it lives."""
+    if not cmb.pattern_initialized:
+        raise RuntimeError("init_K_r_pattern(Q) must be called first")
+    n, ns, P = cmb.n_int, cmb.n_strain, cmb.pattern
+    need = dict(WEIGHT=n, c0=n, phi=n, psi=n, shear=n, bulk=n, lame=n, c_bar=n, sin_phi=n, E=n * ns,
+                S=n * cmb.s_rows, DS=n * ns * ns, K_elast=P.nnzb * P.bs, K_tang=P.nnzb * P.bs, K_r=P.nnzb * P.bs)
+    for name, count in need.items():
+        if getattr(cmb, name).nbytes < 8 * count:
+            raise ValueError(f"native Newton driver: {name} holds fewer than {count} doubles")
+    if any(d.nbytes < 8 * cmb.n_p * n for d in cmb.DPhi) or f.nbytes < 8 * P.n_dof:
+        raise ValueError("native Newton driver: DPhi or f too small for the mesh")
+    host = getattr(cmb, "_native_host", None)
+    if host is None:
+        host = cmb._native_host = (np.ascontiguousarray(cmb._phi, dtype=np.float64).ravel(),
+                                   np.ascontiguousarray(cmb._psi, dtype=np.float64).ravel())
+    if host[0].size != n or host[1].size != n:
+        raise ValueError("native Newton driver: phi and psi must be given per integration point")
+    d = _Constitutive()
+    d.dim, d.n_q, d.davis, d.s_rows, d.n_int = cmb.dim, cmb.n_q, ord(cmb.Davis_type), cmb.s_rows, cmb.n_int
+    d.pattern = cmb.pattern.handle.value
+    d.DPhi1, d.DPhi2 = ptr(cmb.DPhi[0]).value, ptr(cmb.DPhi[1]).value
+    d.DPhi3 = ptr(cmb.DPhi[2]).value if cmb.dim == 3 else None
+    for name in ("WEIGHT", "c0", "phi", "psi", "shear", "bulk", "lame", "c_bar", "sin_phi", "E", "S", "DS",
+                 "K_elast", "K_tang", "K_r", "mask"):
+        setattr(d, name, ptr(getattr(cmb, name)).value)
+    d.lambda_ = ptr(cmb._lambda).value
+    d.phi_host, d.psi_host = host[0].ctypes.data, host[1].ctypes.data
+    d.f = ptr(f).value
+    return d
 
 
 class Workspace:
@@ -39,6 +106,13 @@ class Workspace:
         self.Ua = DeviceArray(shape)
         self.Fa = DeviceArray(shape)
         self.tmp = DeviceArray(shape)
+        self._native = None
+
+    def native_work(self, n):
+        """Device scratch of the native Newton driver (n doubles, kept)."""
+        if self._native is None or self._native.shape[0] < n:
+            self._native = DeviceArray(n)
+        return self._native
 
 
 def norm_Q(a, b, mask, tmp):
@@ -68,10 +142,15 @@ def damping_ALG5(it_damp_max, U_it, lambda_it, d_U, d_l, f, criterion, mask, cmb
 
 
 def newton_ind_SSR(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
-                   linear_solver, ws=None, log=None):
+                   linear_solver, ws=None, log=None, driver="python"):
     """Returns (U_it, lambda_it, flag_N, it, history)."""
     if not cmb.pattern_initialized:
         cmb.init_K_r_pattern(Q)
+    if driver == "native":
+        return _newton_ind_SSR_native(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, tol, r_min, f, cmb,
+                                      linear_solver, ws, log)
+    if driver != "python":
+        raise ValueError(f"unknown Newton driver {driver!r} (expected one of {DRIVERS})")
     mask = cmb.mask
     if ws is None:
         ws = Workspace(cmb.n_n, cmb.dim)
@@ -158,6 +237,57 @@ def newton_ind_SSR(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, tol, r_mi
     hist["status"] = status
     hist["wall_time"] = time.perf_counter() - t_start
     return U_it, lambda_it, flag_N, it, hist
+
+
+def _newton_ind_SSR_native(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, tol, r_min, f, cmb, linear_solver,
+                           ws, log):
+    """newton_ind_SSR.m:1-206 as one ssr_newton_ind_SSR call; the history and
+    the solver's counters are filled from the per-iteration record it returns."""
+    if ws is None:
+        ws = Workspace(cmb.n_n, cmb.dim)
+    desc = native_description(cmb, f)
+    wl = _i64(0)
+    call("ssr_newton_work_len", ctypes.byref(desc), ctypes.byref(wl))
+    work = ws.native_work(wl.value)
+    U_it = DeviceArray(U_ini.shape)
+    rec = np.full((int(it_newt_max), 6), np.nan)
+    lam, flag, it, st = _dbl(0.0), _int(0), _int(0), _int(0)
+    method = 1 if getattr(linear_solver, "method", "dfgmres") == "dcg" else 0
+    t_start = time.perf_counter()
+    call("ssr_newton_ind_SSR", ctypes.byref(desc), linear_solver.handle, method, ptr(U_ini), float(omega),
+         float(lambda_ini), int(it_newt_max), int(it_damp_max), float(tol), float(r_min), ptr(work), wl.value,
+         ptr(U_it), ctypes.byref(lam), ctypes.byref(flag), ctypes.byref(it), ctypes.byref(st), ptr(rec))
+    wall = time.perf_counter() - t_start
+    linear_solver._K = cmb.K_r
+    n_it = it.value
+    hist = dict(residual=[float(x) for x in rec[:n_it, 0]], r=[], alpha=[], linear_iters_W=[], linear_iters_V=[])
+    c = linear_solver.counters
+    for i in range(n_it):
+        row = rec[i]
+        if math.isnan(row[1]):  # stopped at the convergence test
+            continue
+        hist["r"].append(float(row[1]))
+        hist["alpha"].append(float(row[2]))
+        lin_w, lin_v = int(row[3]), int(row[4])
+        hist["linear_iters_W"].append(lin_w)
+        hist["linear_iters_V"].append(lin_v)
+        if c is not None:
+            k = int(row[5])
+            c["DFGMRES.deflation_size.max"] = max(c.get("DFGMRES.deflation_size.max", 0), k)
+            c["DFGMRES.deflation_size.sum"] = c.get("DFGMRES.deflation_size.sum", 0) + k
+            c["DFGMRES.solve.n_iters"] = c.get("DFGMRES.solve.n_iters", 0) + lin_w + lin_v
+            c["DFGMRES.solve.calls"] = c.get("DFGMRES.solve.calls", 0) + 2
+            at_max = (lin_w >= linear_solver.max_iterations) + (lin_v >= linear_solver.max_iterations)
+            if at_max:
+                c["DFGMRES.solve.at_maxit"] = c.get("DFGMRES.solve.at_maxit", 0) + at_max
+        if log and not (i == n_it - 1 and st.value == 2):
+            log(f"  newton_ind_SSR it={i + 1}  rel_resid={row[0]:.2e}  alpha={row[2]:.2f}  r={row[1]:.3g}  "
+                f"lin_it=[W:{lin_w},V:{lin_v}]")
+    if linear_solver.timings is not None:
+        linear_solver.timings["newton_ind_SSR.native"] = linear_solver.timings.get("newton_ind_SSR.native", 0.0) + wall
+    hist["status"] = NEWTON_STATUS.get(st.value, "unknown")
+    hist["wall_time"] = wall
+    return U_it, lam.value, flag.value, n_it, hist
 
 
 def _mask_inplace(x, mask):

@@ -81,10 +81,10 @@ def make_constitutive(prob, Davis_type="B"):
 
 
 def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", log=None, predictor=None,
-            amg_options=None, precision=64, recorder=None):
+            amg_options=None, precision=64, recorder=None, driver="python"):
     """``predictor`` may also be a factory called as predictor(cmb, f) once the
     device problem exists (see learned.LearnedPredictor); ``recorder`` likewise
-    (factory(cmb) or a hook)."""
+    (factory(cmb) or a hook).  ``driver``: "python" or "native" Newton loop."""
     from .continuation import SSR_indirect_continuation
     from .linear_solvers import DCG, DFGMRES
     p = dict(PARAMS_2D if prob["dim"] == 2 else PARAMS_3D)
@@ -109,7 +109,7 @@ def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", 
     U, lam_h, om_h, umax_h, stats = SSR_indirect_continuation(
         p["lambda_init"], p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
         p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], None, prob["Q"], f, cmb,
-        lin, log=log, predictor=predictor, recorder=recorder)
+        lin, log=log, predictor=predictor, recorder=recorder, driver=driver)
     t_run = time.perf_counter() - t1
     timings = dict(cmb.timings)
     timings.update(lin.timings)

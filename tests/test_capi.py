@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_signatures_cover_header():
-    from slope_stability_octave_amd import assembly, learned, linear_solvers  # noqa: F401  (register signatures)
+    from slope_stability_octave_amd import assembly, learned, linear_solvers, newton  # noqa: F401  (signatures)
     missing = [n for n in declared_symbols() if n not in _lib.SIGNATURES]
     assert not missing, missing
 
@@ -47,6 +47,28 @@ def test_error_reporting_without_compute():
                                            None, None, None, None)
     assert rc == 1
     assert b"3 rows" in lib.ssr_last_error()
+
+
+def test_newton_driver_argument_checks():
+    """ssr_newton_ind_SSR rejects an incomplete problem description before
+    touching the device."""
+    from slope_stability_octave_amd import newton
+    lib = _lib.load()
+    d = newton._Constitutive()
+    out = [ctypes.c_double(0.0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)]
+    fake = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
+    rc = lib.ssr_newton_ind_SSR(ctypes.byref(d), fake, 0, fake, ctypes.c_double(1.0), ctypes.c_double(1.0), 10, 10,
+                                ctypes.c_double(1e-4), ctypes.c_double(1e-4), fake, ctypes.c_int64(0), fake,
+                                *[ctypes.byref(o) for o in out], None)
+    assert rc == 1
+    assert b"incomplete constitutive description" in lib.ssr_last_error()
+    rc = lib.ssr_newton_ind_SSR(None, fake, 0, fake, ctypes.c_double(1.0), ctypes.c_double(1.0), 10, 10,
+                                ctypes.c_double(1e-4), ctypes.c_double(1e-4), fake, ctypes.c_int64(0), fake,
+                                *[ctypes.byref(o) for o in out], None)
+    assert rc == 1
+    assert b"null argument" in lib.ssr_last_error()
+    n = ctypes.c_int64(0)
+    assert lib.ssr_newton_work_len(ctypes.byref(d), ctypes.byref(n)) == 1
 
 
 def test_no_cpu_fallback_when_no_gpu():

@@ -152,7 +152,22 @@ def test_strain_and_force(dim):
     assert np.abs(dE.get().T - E0).max() <= 1e-12 * np.abs(E0).max()
     srows = 6 if dim == 3 else 4
     dS = DeviceArray.from_host(np.asfortranarray(p["S"]).T.copy())
-    dF = DeviceArray((n_n, dim))
-    P.force(p["n_q"], dD, dS, srows, DeviceArray.from_host(p["WEIGHT"]), dF)
+    dWt = DeviceArray.from_host(p["WEIGHT"])
     F0 = O.force(dim, p["elem"], p["DPhi"], p["S"], p["WEIGHT"], p["n_q"], n_n)
-    assert np.abs(dF.get().T - F0).max() <= 1e-12 * np.abs(F0).max()
+    # the default form (element-node forces, then one ordered sum per node)
+    # gives the same bits on every run; the FP64-atomic form agrees to rounding
+    from slope_stability_octave_amd import _lib
+    out = {}
+    old = _lib.get_tuning(_lib.TUNE_ASSEMBLY_FORM)
+    try:
+        for form in (0, 1, 0):
+            _lib.set_tuning(_lib.TUNE_ASSEMBLY_FORM, form)
+            dF = DeviceArray((n_n, dim))
+            dF.set(np.full((n_n, dim), np.nan))  # every entry must be written
+            P.force(p["n_q"], dD, dS, srows, dWt, dF)
+            out.setdefault(form, []).append(dF.get().T)
+    finally:
+        _lib.set_tuning(_lib.TUNE_ASSEMBLY_FORM, old)
+    for F in (out[0][0], out[1][0]):
+        assert np.abs(F - F0).max() <= 1e-12 * np.abs(F0).max()
+    assert np.array_equal(out[0][0], out[0][1])

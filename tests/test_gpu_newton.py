@@ -83,3 +83,52 @@ def test_fos_heterogeneous_3d_matches_cpu_oracle():
         p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], prob["f"], cpu)
     print("GPU FoS", res["FoS"], "CPU FoS", lh0[-1], "steps", len(res["lambda_hist"]), len(lh0), flush=True)
     assert abs(res["FoS"] - lh0[-1]) <= 1e-4
+
+
+@pytest.mark.parametrize("solver,precond,precision", [("dfgmres", "block_jacobi", 64), ("dcg", "block_jacobi", 64),
+                                                      ("dfgmres", "amg", 32)])
+def test_native_newton_driver_matches_python_driver(coarse2d, solver, precond, precision):
+    """ssr_newton_ind_SSR (the Newton loop inside the library) against the
+    Python loop over the same C entry points: the same kernels run in the same
+    order, so every accepted (omega, lambda), the Newton and Krylov iteration
+    counts and the final field agree to rounding (in practice exactly)."""
+    prob, _ = coarse2d
+    p = dict(problems.PARAMS_2D)
+    p["lambda_init"] = 1.0
+    runs = {}
+    for driver in ("python", "native"):
+        print(f"driver={driver} solver={solver} precond={precond} precision={precision}", flush=True)
+        runs[driver] = problems.run_ssr(prob, p, solver=solver, preconditioner=precond, precision=precision,
+                                        driver=driver)
+    a, b = runs["python"], runs["native"]
+    print("FoS", a["FoS"], b["FoS"], "Newton", a["newton_iterations"], b["newton_iterations"], flush=True)
+    assert len(a["lambda_hist"]) == len(b["lambda_hist"])
+    assert np.abs(np.asarray(a["lambda_hist"]) - b["lambda_hist"]).max() <= 1e-12
+    assert np.abs(np.asarray(a["omega_hist"]) - b["omega_hist"]).max() <= 1e-12 * np.abs(a["omega_hist"]).max()
+    assert a["newton_iterations"] == b["newton_iterations"]
+    assert a["counts"]["DFGMRES.solve.n_iters"] == b["counts"]["DFGMRES.solve.n_iters"]
+    assert a["counts"]["DFGMRES.deflation_size.max"] == b["counts"]["DFGMRES.deflation_size.max"]
+    Ua, Ub = a["U"].get(), b["U"].get()
+    assert np.abs(Ua - Ub).max() <= 1e-12 * np.abs(Ua).max()
+    for ha, hb in zip(a["stats"]["newton_histories"], b["stats"]["newton_histories"]):
+        assert ha["status"] == hb["status"]
+        assert ha["linear_iters_W"] == hb["linear_iters_W"] and ha["linear_iters_V"] == hb["linear_iters_V"]
+        assert ha["alpha"] == hb["alpha"] and ha["r"] == hb["r"]
+        assert np.allclose(ha["residual"], hb["residual"], rtol=1e-12, atol=0)
+
+
+def test_native_newton_driver_heterogeneous_3d():
+    """The four-layer 3D slope (per-point materials) with the shared AMG
+    hierarchy: both Newton drivers give the same factor of safety."""
+    prob = problems.slope_3d((2, 2, 2, 1, 2, 2), 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HETERO_3D_LAYERS,
+                             layers=problems.HETERO_3D_LEVELS)
+    p = dict(problems.PARAMS_3D)
+    p["lambda_init"] = 0.7
+    fos = {}
+    for driver in ("python", "native"):
+        print(f"3D heterogeneous, driver={driver}", flush=True)
+        res = problems.run_ssr(prob, p, preconditioner="amg", driver=driver)
+        fos[driver] = (res["FoS"], res["newton_iterations"], len(res["lambda_hist"]))
+    print(fos, flush=True)
+    assert abs(fos["python"][0] - fos["native"][0]) <= 1e-12
+    assert fos["python"][1:] == fos["native"][1:]
