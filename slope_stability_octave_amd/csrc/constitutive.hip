@@ -78,7 +78,11 @@ __host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, 
     e2 = -2.0 * rq * cos(th - kTwoPiOver3) + I1 / 3.0;
     e3 = -2.0 * rq * cos(th) + I1 / 3.0;
 
-    k = Coef3{};
+    k.sP1 = k.sP3 = k.sI = k.sT = 0.0;
+    k.a = k.b = 0.0;
+    k.c11 = k.c33 = k.c13 = k.cII = k.c1I = k.c3I = k.cIT = k.c1T = k.c3T = 0.0;
+    k.g1 = k.g3 = 0.0;
+    k.use1 = k.use3 = false;
     const double f_tr = 2.0 * G * ((1.0 + sphi) * e1 - (1.0 - sphi) * e3) + 2.0 * lam * sphi * I1 - cb;
     if (f_tr <= 0.0) {
         k.sI = lam * I1;
