@@ -286,12 +286,14 @@ typedef struct ssr_constitutive {
  * scratch of ssr_newton_work_len doubles, disjoint from every other argument.  hist (host, 6 x it_newt_max, or
  * NULL) receives per iteration: relative residual, r, alpha, Krylov
  * iterations for W and for V, deflation basis size; NaN where an iteration
- * stopped before that point.  A NaN residual fails the solve (flag_N = 1). */
+ * stopped before that point.  times (host, 4 doubles, or NULL) accumulates host
+ * seconds: Krylov solves, A-orthogonalisation, preconditioner setup, the rest.
+ * A NaN residual fails the solve (flag_N = 1). */
 int ssr_newton_work_len(const ssr_constitutive *C, int64_t *len);
 int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini,
                        double omega, double lambda_ini, int it_newt_max, int it_damp_max, double tol,
                        double r_min, double *work, int64_t work_len, double *U_it, double *lambda_out,
-                       int *flag_N, int *it, int *status, double *hist);
+                       int *flag_N, int *it, int *status, double *hist, double *times);
 
 /* ---- learned continuation predictor (small MLP) --------------------------- */
 /* No reference counterpart: the reference continuation has the secant

@@ -11,6 +11,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <chrono>
+
 #include "ssr_common.h"
 #include "../../include/ssr_hip.h"
 
@@ -28,6 +30,19 @@ struct NewtonCtx {
     int method;
     int64_t n_dof;
     double *F, *F_eps, *G, *rhs, *V, *W, *dU, *Ua, *Fa, *tmp;
+};
+
+// host seconds spent in a phase (the calls that end in a synchronisation
+// absorb the device work queued before them)
+struct Lap {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double take()
+    {
+        const auto t1 = std::chrono::steady_clock::now();
+        const double s = std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+        return s;
+    }
 };
 
 // CONSTITUTIVE.m:178-200 / reduction.m:25-58 at a scalar lambda
@@ -126,7 +141,7 @@ extern "C" int ssr_newton_work_len(const ssr_constitutive *C, int64_t *len)
 extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini,
                                   double omega, double lambda_ini, int it_newt_max, int it_damp_max, double tol,
                                   double r_min, double *work, int64_t work_len, double *U_it, double *lambda_out,
-                                  int *flag_out, int *it_out, int *status_out, double *hist)
+                                  int *flag_out, int *it_out, int *status_out, double *hist, double *times)
 {
     SSR_REQUIRE(C && solver && U_ini && work && U_it && lambda_out && flag_out && it_out && status_out,
                 "newton_ind_SSR: null argument");
@@ -172,6 +187,8 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
     int status = SSR_NEWTON_UNKNOWN;
     if (hist)
         for (int64_t k = 0; k < 6 * (int64_t)it_newt_max; ++k) hist[k] = NAN;
+    double tm[4] = {0.0, 0.0, 0.0, 0.0};  // solves, A-orthogonalisation, preconditioner, the rest
+    Lap lap;
 
     // newton_ind_SSR.m:68-175
     while (true) {
@@ -198,9 +215,12 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
         NCHECK(build_F(x, lambda_it + eps, U_it, x.F_eps, 0));
         NCHECK(ssr_dev_lincomb(n_dof, 1.0 / eps, x.F_eps, -1.0 / eps, x.F, x.G));
         // preconditioner and deflation basis for K_r (:99-110)
+        tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
+        tm[2] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 0));
         NCHECK(ssr_solver_A_orthogonalize(solver));
+        tm[1] += lap.take();
         if (h) {
             int k = 0;
             NCHECK(ssr_solver_deflation_size(solver, &k));
@@ -214,6 +234,7 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
         NCHECK(ssr_dev_lincomb(n_dof, 1.0, C->f, -1.0, x.F, x.rhs));
         NCHECK(ssr_dev_mul(n_dof, C->mask, x.rhs));
         NCHECK(solve(x, x.rhs, x.V, &lin_v));
+        tm[0] += lap.take();
         if (h) {
             h[3] = lin_w;
             h[4] = lin_v;
@@ -260,6 +281,9 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
         }
     }
     NCHECK(ssr_synchronize());
+    tm[3] += lap.take();
+    if (times)
+        for (int q = 0; q < 4; ++q) times[q] += tm[q];
     *lambda_out = lambda_it;
     *flag_out = flag_N;
     *it_out = it;

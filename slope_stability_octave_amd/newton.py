@@ -37,7 +37,7 @@ _i64 = ctypes.c_int64
 _lib.extend_signatures({
     "ssr_newton_work_len": [_vp, _vp],
     "ssr_newton_ind_SSR": [_vp, _vp, _int, _vp, _dbl, _dbl, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp,
-                           _vp, _vp, _vp],
+                           _vp, _vp, _vp, _vp],
 })
 
 DRIVERS = ("python", "native")
@@ -251,12 +251,13 @@ def _newton_ind_SSR_native(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, t
     work = ws.native_work(wl.value)
     U_it = DeviceArray(U_ini.shape)
     rec = np.full((int(it_newt_max), 6), np.nan)
+    times = np.zeros(4)
     lam, flag, it, st = _dbl(0.0), _int(0), _int(0), _int(0)
     method = 1 if getattr(linear_solver, "method", "dfgmres") == "dcg" else 0
     t_start = time.perf_counter()
     call("ssr_newton_ind_SSR", ctypes.byref(desc), linear_solver.handle, method, ptr(U_ini), float(omega),
          float(lambda_ini), int(it_newt_max), int(it_damp_max), float(tol), float(r_min), ptr(work), wl.value,
-         ptr(U_it), ctypes.byref(lam), ctypes.byref(flag), ctypes.byref(it), ctypes.byref(st), ptr(rec))
+         ptr(U_it), ctypes.byref(lam), ctypes.byref(flag), ctypes.byref(it), ctypes.byref(st), ptr(rec), ptr(times))
     wall = time.perf_counter() - t_start
     linear_solver._K = cmb.K_r
     n_it = it.value
@@ -283,8 +284,12 @@ def _newton_ind_SSR_native(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, t
         if log and not (i == n_it - 1 and st.value == 2):
             log(f"  newton_ind_SSR it={i + 1}  rel_resid={row[0]:.2e}  alpha={row[2]:.2f}  r={row[1]:.3g}  "
                 f"lin_it=[W:{lin_w},V:{lin_v}]")
-    if linear_solver.timings is not None:
-        linear_solver.timings["newton_ind_SSR.native"] = linear_solver.timings.get("newton_ind_SSR.native", 0.0) + wall
+    tt = linear_solver.timings
+    if tt is not None:
+        for key, v in zip(("DFGMRES.solve", "DFGMRES.A_orthogonalize", "DFGMRES.update_preconditioner",
+                           "newton_ind_SSR.native_other"), times):
+            tt[key] = tt.get(key, 0.0) + float(v)
+        tt["newton_ind_SSR.native"] = tt.get("newton_ind_SSR.native", 0.0) + wall
     hist["status"] = NEWTON_STATUS.get(st.value, "unknown")
     hist["wall_time"] = wall
     return U_it, lam.value, flag.value, n_it, hist

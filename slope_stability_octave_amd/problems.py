@@ -81,10 +81,11 @@ def make_constitutive(prob, Davis_type="B"):
 
 
 def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", log=None, predictor=None,
-            amg_options=None, precision=64, recorder=None, driver="python"):
+            amg_options=None, precision=64, recorder=None, driver="native"):
     """``predictor`` may also be a factory called as predictor(cmb, f) once the
     device problem exists (see learned.LearnedPredictor); ``recorder`` likewise
-    (factory(cmb) or a hook).  ``driver``: "python" or "native" Newton loop."""
+    (factory(cmb) or a hook).  ``driver``: "native" (the Newton loop inside
+    libssr_hip.so, default) or "python" (newton.py); both give the same bits."""
     from .continuation import SSR_indirect_continuation
     from .linear_solvers import DCG, DFGMRES
     p = dict(PARAMS_2D if prob["dim"] == 2 else PARAMS_3D)

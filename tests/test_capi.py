@@ -59,12 +59,12 @@ def test_newton_driver_argument_checks():
     fake = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
     rc = lib.ssr_newton_ind_SSR(ctypes.byref(d), fake, 0, fake, ctypes.c_double(1.0), ctypes.c_double(1.0), 10, 10,
                                 ctypes.c_double(1e-4), ctypes.c_double(1e-4), fake, ctypes.c_int64(0), fake,
-                                *[ctypes.byref(o) for o in out], None)
+                                *[ctypes.byref(o) for o in out], None, None)
     assert rc == 1
     assert b"incomplete constitutive description" in lib.ssr_last_error()
     rc = lib.ssr_newton_ind_SSR(None, fake, 0, fake, ctypes.c_double(1.0), ctypes.c_double(1.0), 10, 10,
                                 ctypes.c_double(1e-4), ctypes.c_double(1e-4), fake, ctypes.c_int64(0), fake,
-                                *[ctypes.byref(o) for o in out], None)
+                                *[ctypes.byref(o) for o in out], None, None)
     assert rc == 1
     assert b"null argument" in lib.ssr_last_error()
     n = ctypes.c_int64(0)
