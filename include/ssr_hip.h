@@ -294,6 +294,16 @@ int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver, int method
                        double omega, double lambda_ini, int it_newt_max, int it_damp_max, double tol,
                        double r_min, double *work, int64_t work_len, double *U_it, double *lambda_out,
                        int *flag_N, int *it, int *status, double *hist, double *times);
+/* [U_it, flag_N, it, history] = newton(U_ini, tol, it_newt_max, it_damp_max, r_min,
+ *     K_elast, Q, f, constitutive_matrix_builder, linear_system_solver)
+ * replaces slope_stability/+NEWTON/newton.m:1-192 with damping.m:1-56: Newton at
+ * the material already reduced on C (c_bar, sin_phi), as the initial phase of
+ * the continuation calls it.  Arguments as for ssr_newton_ind_SSR; hist rows
+ * (5 x it_newt_max): relative residual, r, alpha, Krylov iterations,
+ * deflation basis size. */
+int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini, int it_newt_max,
+               int it_damp_max, double tol, double r_min, double *work, int64_t work_len, double *U_it,
+               int *flag_N, int *it, int *status, double *hist, double *times);
 
 /* ---- learned continuation predictor (small MLP) --------------------------- */
 /* No reference counterpart: the reference continuation has the secant

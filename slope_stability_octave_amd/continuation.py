@@ -45,7 +45,7 @@ def _umax(U):
 
 
 def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max,
-                                         tol, r_min, K_elast, Q, f, cmb, solver, ws, log=None):
+                                         tol, r_min, K_elast, Q, f, cmb, solver, ws, log=None, driver="python"):
     if not cmb.pattern_initialized:
         cmb.init_K_r_pattern(Q)
     mask = cmb.mask
@@ -55,7 +55,7 @@ def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_mi
     cmb.reduction(lambda_init)
     its = []
     U_it, flag_N, it_newt, _ = newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver,
-                                      ws, log)
+                                      ws, log, driver=driver)
     its.append(it_newt)
     if flag_N == 1:
         raise ContinuationError("Initial choice of lambda seems to be too large.")
@@ -68,7 +68,7 @@ def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_mi
         lambda_it = lambda1 + d_lambda
         cmb.reduction(lambda_it)
         U_it, flag_N, it_newt, _ = newton(U1, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver,
-                                          ws, log)
+                                          ws, log, driver=driver)
         its.append(it_newt)
         if flag_N == 1:
             d_lambda = d_lambda / 2
@@ -93,8 +93,8 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
                               omega_max_stop, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
                               linear_system_solver, log=None, predictor=None, recorder=None, driver="python"):
     """Returns (U, lambda_hist, omega_hist, Umax_hist, stats).  ``driver``
-    picks the Newton loop: "python" (newton.py) or "native" (the same loop in
-    libssr_hip.so)."""
+    picks the Newton loops (newton_ind_SSR and the initial phase's newton):
+    "python" (newton.py) or "native" (the same loops in libssr_hip.so)."""
     t_total = time.perf_counter()
     ws = Workspace(cmb.n_n, cmb.dim)
     mask_ready = cmb.pattern_initialized or cmb.init_K_r_pattern(Q) or True  # noqa: F841
@@ -105,7 +105,7 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
                  predictor_used=[])
     U_old, U, omega_old, omega, lambda_old, lam, init_its = init_phase_SSR_indirect_continuation(
         lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
-        linear_system_solver.copy(), ws, log)
+        linear_system_solver.copy(), ws, log, driver=driver)
     stats["init_newton_iterations"] = init_its
     linear_system_solver.expand_deflation_basis(U_old)
     lambda_hist = [lambda_old, lam]

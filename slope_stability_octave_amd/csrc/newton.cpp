@@ -65,11 +65,12 @@ int reduction(const NewtonCtx &x, double lam)
 }
 
 // build_F_all / build_F_and_DS_all (CONSTITUTIVE.m:202-285): reduction, E = B U,
-// S (and DS), F = B'(w S)
-int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with_ds)
+// S (and DS), F = B'(w S); the *_reduced forms (reduce = false) keep the
+// material the caller reduced
+int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with_ds, bool reduce = true)
 {
     const ssr_constitutive *C = x.C;
-    NCHECK(reduction(x, lam));
+    if (reduce) NCHECK(reduction(x, lam));
     NCHECK(ssr_dev_strain(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, U, C->E));
     if (C->dim == 3)
         NCHECK(ssr_dev_constitutive_3D(C->n_int, C->E, C->c_bar, C->sin_phi, C->shear, C->bulk, C->lame, C->S,
@@ -120,11 +121,75 @@ int damping_ALG5(const NewtonCtx &x, int it_damp_max, const double *U_it, double
     return SSR_OK;
 }
 
+// damping.m:25-54: bisection on the directional derivative (F(U + a dU) - f)'dU
+// over all DOFs, at the material the caller reduced
+int damping(const NewtonCtx &x, int it_damp_max, const double *U_it, const double *F, double *alpha_out)
+{
+    const int64_t n = x.n_dof;
+    const double *f = x.C->f;
+    double decrease = 0.0, dd = 0.0;
+    NCHECK(ssr_dev_lincomb(n, 1.0, F, -1.0, f, x.tmp));
+    NCHECK(ssr_dot(n, x.tmp, x.dU, nullptr, &decrease));
+    NCHECK(ssr_dot(n, x.dU, x.dU, nullptr, &dd));
+    if (isnan(decrease) || isinf(sqrt(dd)) || decrease >= 0) {
+        *alpha_out = 0.0;
+        return SSR_OK;
+    }
+    double alpha = 1.0, a_min = 0.0, a_max = 1.0;
+    for (int it_damp = 0; it_damp < it_damp_max;) {
+        ++it_damp;
+        NCHECK(ssr_dev_lincomb(n, 1.0, U_it, alpha, x.dU, x.Ua));
+        NCHECK(build_F(x, 0.0, x.Ua, x.Fa, 0, false));
+        NCHECK(ssr_dev_lincomb(n, 1.0, x.Fa, -1.0, f, x.tmp));
+        NCHECK(ssr_dot(n, x.tmp, x.dU, nullptr, &decrease));
+        if (decrease < 0) {
+            if (alpha == 1) break;
+            a_min = alpha;
+        } else {
+            a_max = alpha;
+        }
+        alpha = (a_min + a_max) / 2;
+    }
+    *alpha_out = alpha;
+    return SSR_OK;
+}
+
 int solve(const NewtonCtx &x, const double *b, double *sol, int *iters)
 {
     double rr = 0.0;
     if (x.method == 1) return ssr_solver_dcg(x.solver, b, sol, iters, &rr);
     return ssr_solver_dfgmres(x.solver, b, sol, iters, &rr);
+}
+
+// argument checks shared by the entry points; fills the context
+int prepare(const ssr_constitutive *C, ssr_solver *solver, int method, int it_newt_max, int it_damp_max,
+            double *work, int64_t work_len, NewtonCtx *x)
+{
+    SSR_REQUIRE(C->pattern && C->DPhi1 && C->DPhi2 && (C->dim == 2 || C->DPhi3) && C->WEIGHT && C->c0 &&
+                    C->phi && C->psi && C->shear && C->bulk && C->lame && C->lambda && C->c_bar && C->sin_phi &&
+                    C->E && C->S && C->DS && C->K_elast && C->K_tang && C->K_r && C->mask && C->f,
+                "newton: incomplete constitutive description");
+    SSR_REQUIRE(C->dim == 2 || C->dim == 3, "newton: dim must be 2 or 3");
+    SSR_REQUIRE(C->s_rows == (C->dim == 3 ? 6 : 4), "newton: S has 6 rows in 3D, 4 in 2D");
+    SSR_REQUIRE(C->davis == 'A' || C->davis == 'B' || C->davis == 'C', "Incorrect choice of the Davis approach.");
+    SSR_REQUIRE(C->davis != 'C' || (C->phi_host && C->psi_host), "newton: Davis C needs phi, psi on the host");
+    SSR_REQUIRE(method == 0 || method == 1, "newton: method 0 (DFGMRES) or 1 (DCG)");
+    SSR_REQUIRE(it_newt_max >= 1 && it_damp_max >= 1, "newton: iteration limits must be positive");
+    int64_t info[8];
+    NCHECK(ssr_pattern_info(C->pattern, info));
+    SSR_REQUIRE(info[0] == C->dim, "newton: pattern dimension %lld != %d", (long long)info[0], C->dim);
+    SSR_REQUIRE(info[3] > 0 && C->n_q > 0 && C->n_int == info[3] * C->n_q,
+                "newton: n_int %lld != n_e %lld x n_q %d", (long long)C->n_int, (long long)info[3], C->n_q);
+    const int64_t n_dof = info[0] * info[1];
+    SSR_REQUIRE(work_len >= 10 * n_dof, "newton: work holds %lld doubles, %lld needed", (long long)work_len,
+                (long long)(10 * n_dof));
+    x->C = C;
+    x->solver = solver;
+    x->method = method;
+    x->n_dof = n_dof;
+    double **slots[10] = {&x->F, &x->F_eps, &x->G, &x->rhs, &x->V, &x->W, &x->dU, &x->Ua, &x->Fa, &x->tmp};
+    for (int k = 0; k < 10; ++k) *slots[k] = work + k * n_dof;
+    return SSR_OK;
 }
 
 }  // namespace
@@ -145,32 +210,9 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
 {
     SSR_REQUIRE(C && solver && U_ini && work && U_it && lambda_out && flag_out && it_out && status_out,
                 "newton_ind_SSR: null argument");
-    SSR_REQUIRE(C->pattern && C->DPhi1 && C->DPhi2 && (C->dim == 2 || C->DPhi3) && C->WEIGHT && C->c0 &&
-                    C->phi && C->psi && C->shear && C->bulk && C->lame && C->lambda && C->c_bar && C->sin_phi &&
-                    C->E && C->S && C->DS && C->K_elast && C->K_tang && C->K_r && C->mask && C->f,
-                "newton_ind_SSR: incomplete constitutive description");
-    SSR_REQUIRE(C->dim == 2 || C->dim == 3, "newton_ind_SSR: dim must be 2 or 3");
-    SSR_REQUIRE(C->s_rows == (C->dim == 3 ? 6 : 4), "newton_ind_SSR: S has 6 rows in 3D, 4 in 2D");
-    SSR_REQUIRE(C->davis == 'A' || C->davis == 'B' || C->davis == 'C', "Incorrect choice of the Davis approach.");
-    SSR_REQUIRE(C->davis != 'C' || (C->phi_host && C->psi_host), "newton_ind_SSR: Davis C needs phi, psi on the host");
-    SSR_REQUIRE(method == 0 || method == 1, "newton_ind_SSR: method 0 (DFGMRES) or 1 (DCG)");
-    SSR_REQUIRE(it_newt_max >= 1 && it_damp_max >= 1, "newton_ind_SSR: iteration limits must be positive");
-    int64_t info[8];
-    NCHECK(ssr_pattern_info(C->pattern, info));
-    SSR_REQUIRE(info[0] == C->dim, "newton_ind_SSR: pattern dimension %lld != %d", (long long)info[0], C->dim);
-    SSR_REQUIRE(info[3] > 0 && C->n_q > 0 && C->n_int == info[3] * C->n_q,
-                "newton_ind_SSR: n_int %lld != n_e %lld x n_q %d", (long long)C->n_int, (long long)info[3], C->n_q);
-    const int64_t n_dof = info[0] * info[1];
-    SSR_REQUIRE(work_len >= 10 * n_dof, "newton_ind_SSR: work holds %lld doubles, %lld needed",
-                (long long)work_len, (long long)(10 * n_dof));
-
     NewtonCtx x;
-    x.C = C;
-    x.solver = solver;
-    x.method = method;
-    x.n_dof = n_dof;
-    double **slots[10] = {&x.F, &x.F_eps, &x.G, &x.rhs, &x.V, &x.W, &x.dU, &x.Ua, &x.Fa, &x.tmp};
-    for (int k = 0; k < 10; ++k) *slots[k] = work + k * n_dof;
+    NCHECK(prepare(C, solver, method, it_newt_max, it_damp_max, work, work_len, &x));
+    const int64_t n_dof = x.n_dof;
 
     // newton_ind_SSR.m:22-33
     NCHECK(ssr_dev_lincomb(n_dof, 1.0, U_ini, 0.0, U_ini, U_it));
@@ -285,6 +327,108 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
     if (times)
         for (int q = 0; q < 4; ++q) times[q] += tm[q];
     *lambda_out = lambda_it;
+    *flag_out = flag_N;
+    *it_out = it;
+    *status_out = status;
+    return SSR_OK;
+}
+
+extern "C" int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini,
+                          int it_newt_max, int it_damp_max, double tol, double r_min, double *work, int64_t work_len,
+                          double *U_it, int *flag_out, int *it_out, int *status_out, double *hist, double *times)
+{
+    SSR_REQUIRE(C && solver && U_ini && work && U_it && flag_out && it_out && status_out, "newton: null argument");
+    NewtonCtx x;
+    NCHECK(prepare(C, solver, method, it_newt_max, it_damp_max, work, work_len, &x));
+    const int64_t n_dof = x.n_dof;
+
+    // newton.m:36-44
+    NCHECK(ssr_dev_lincomb(n_dof, 1.0, U_ini, 0.0, U_ini, U_it));
+    double ff = 0.0;
+    NCHECK(ssr_dot(n_dof, C->f, C->f, C->mask, &ff));
+    const double norm_f = sqrt(ff);
+    int flag_N = 0;
+    bool compute_diffs = true;
+    double r = r_min;
+    int it = 0;
+    double rel_resid = NAN, criterion = NAN;
+    int status = SSR_NEWTON_UNKNOWN;
+    if (hist)
+        for (int64_t k = 0; k < 5 * (int64_t)it_newt_max; ++k) hist[k] = NAN;
+    double tm[4] = {0.0, 0.0, 0.0, 0.0};
+    Lap lap;
+
+    // newton.m:74-160
+    while (true) {
+        ++it;
+        double *h = hist ? hist + 5 * (int64_t)(it - 1) : nullptr;
+        if (compute_diffs) {
+            NCHECK(build_F(x, 0.0, U_it, x.F, 1, false));
+            NCHECK(norm_Q(x, x.F, C->f, &criterion));
+            rel_resid = criterion / norm_f;
+            if (h) h[0] = rel_resid;
+            if (rel_resid < tol) {
+                status = SSR_NEWTON_CONVERGED;
+                break;
+            }
+        } else if (h) {
+            h[0] = rel_resid;
+        }
+        if (h) h[1] = r;
+        NCHECK(ssr_dev_assemble_bsr(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT,
+                                    C->K_tang));
+        NCHECK(ssr_dev_regularize(C->pattern, r, C->K_elast, C->K_tang, C->K_r));
+        tm[3] += lap.take();
+        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
+        tm[2] += lap.take();
+        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 0));
+        NCHECK(ssr_solver_A_orthogonalize(solver));
+        tm[1] += lap.take();
+        if (h) {
+            int k = 0;
+            NCHECK(ssr_solver_deflation_size(solver, &k));
+            h[4] = k;
+        }
+        // dU = K \ (f(Q) - F(Q))   (:112-113)
+        int lin_it = 0;
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0, C->f, -1.0, x.F, x.rhs));
+        NCHECK(ssr_dev_mul(n_dof, C->mask, x.rhs));
+        NCHECK(solve(x, x.rhs, x.dU, &lin_it));
+        tm[0] += lap.take();
+        if (h) h[3] = lin_it;
+        double alpha = 0.0;
+        NCHECK(damping(x, it_damp_max, U_it, x.F, &alpha));
+        if (h) h[2] = alpha;
+        // (:121-143)
+        compute_diffs = true;
+        if (alpha < 1e-1) {
+            if (alpha == 0) {
+                compute_diffs = false;
+                r = r * 2;
+            } else {
+                r = r * pow(2.0, 1.0 / 4);
+            }
+        } else {
+            NCHECK(ssr_solver_expand_deflation(solver, x.dU));
+            if (alpha > 0.5) r = fmax(r / sqrt(2.0), r_min);
+        }
+        if (alpha == 0 && r > 1) {
+            status = SSR_NEWTON_STALLED;
+            if (rel_resid > 10 * tol) flag_N = 1;
+            break;
+        }
+        NCHECK(ssr_dev_axpby(n_dof, alpha, x.dU, 1.0, U_it));
+        // (:153-160; a NaN residual fails the solve)
+        if (isnan(rel_resid) || it == it_newt_max) {
+            if (rel_resid > 10 * tol || isnan(rel_resid)) flag_N = 1;
+            status = isnan(rel_resid) ? SSR_NEWTON_NAN : SSR_NEWTON_MAXIT;
+            break;
+        }
+    }
+    NCHECK(ssr_synchronize());
+    tm[3] += lap.take();
+    if (times)
+        for (int q = 0; q < 4; ++q) times[q] += tm[q];
     *flag_out = flag_N;
     *it_out = it;
     *status_out = status;

@@ -38,6 +38,7 @@ _lib.extend_signatures({
     "ssr_newton_work_len": [_vp, _vp],
     "ssr_newton_ind_SSR": [_vp, _vp, _int, _vp, _dbl, _dbl, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp,
                            _vp, _vp, _vp, _vp],
+    "ssr_newton": [_vp, _vp, _int, _vp, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
 })
 
 DRIVERS = ("python", "native")
@@ -262,7 +263,6 @@ def _newton_ind_SSR_native(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, t
     linear_solver._K = cmb.K_r
     n_it = it.value
     hist = dict(residual=[float(x) for x in rec[:n_it, 0]], r=[], alpha=[], linear_iters_W=[], linear_iters_V=[])
-    c = linear_solver.counters
     for i in range(n_it):
         row = rec[i]
         if math.isnan(row[1]):  # stopped at the convergence test
@@ -272,27 +272,75 @@ def _newton_ind_SSR_native(U_ini, omega, lambda_ini, it_newt_max, it_damp_max, t
         lin_w, lin_v = int(row[3]), int(row[4])
         hist["linear_iters_W"].append(lin_w)
         hist["linear_iters_V"].append(lin_v)
-        if c is not None:
-            k = int(row[5])
-            c["DFGMRES.deflation_size.max"] = max(c.get("DFGMRES.deflation_size.max", 0), k)
-            c["DFGMRES.deflation_size.sum"] = c.get("DFGMRES.deflation_size.sum", 0) + k
-            c["DFGMRES.solve.n_iters"] = c.get("DFGMRES.solve.n_iters", 0) + lin_w + lin_v
-            c["DFGMRES.solve.calls"] = c.get("DFGMRES.solve.calls", 0) + 2
-            at_max = (lin_w >= linear_solver.max_iterations) + (lin_v >= linear_solver.max_iterations)
-            if at_max:
-                c["DFGMRES.solve.at_maxit"] = c.get("DFGMRES.solve.at_maxit", 0) + at_max
+        _count_native(linear_solver, int(row[5]), [lin_w, lin_v])
         if log and not (i == n_it - 1 and st.value == 2):
             log(f"  newton_ind_SSR it={i + 1}  rel_resid={row[0]:.2e}  alpha={row[2]:.2f}  r={row[1]:.3g}  "
                 f"lin_it=[W:{lin_w},V:{lin_v}]")
+    _record_native(linear_solver, times, wall)
+    hist["status"] = NEWTON_STATUS.get(st.value, "unknown")
+    hist["wall_time"] = wall
+    return U_it, lam.value, flag.value, n_it, hist
+
+
+def _record_native(linear_solver, times, wall):
     tt = linear_solver.timings
     if tt is not None:
         for key, v in zip(("DFGMRES.solve", "DFGMRES.A_orthogonalize", "DFGMRES.update_preconditioner",
                            "newton_ind_SSR.native_other"), times):
             tt[key] = tt.get(key, 0.0) + float(v)
         tt["newton_ind_SSR.native"] = tt.get("newton_ind_SSR.native", 0.0) + wall
+
+
+def _count_native(linear_solver, k, lin_its):
+    c = linear_solver.counters
+    if c is None:
+        return
+    c["DFGMRES.deflation_size.max"] = max(c.get("DFGMRES.deflation_size.max", 0), k)
+    c["DFGMRES.deflation_size.sum"] = c.get("DFGMRES.deflation_size.sum", 0) + k
+    c["DFGMRES.solve.n_iters"] = c.get("DFGMRES.solve.n_iters", 0) + sum(lin_its)
+    c["DFGMRES.solve.calls"] = c.get("DFGMRES.solve.calls", 0) + len(lin_its)
+    at_max = sum(1 for x in lin_its if x >= linear_solver.max_iterations)
+    if at_max:
+        c["DFGMRES.solve.at_maxit"] = c.get("DFGMRES.solve.at_maxit", 0) + at_max
+
+
+def _newton_native(U_ini, tol, it_newt_max, it_damp_max, r_min, f, cmb, linear_solver, ws, log):
+    """newton.m:1-192 as one ssr_newton call (material reduced beforehand)."""
+    if ws is None:
+        ws = Workspace(cmb.n_n, cmb.dim)
+    desc = native_description(cmb, f)
+    wl = _i64(0)
+    call("ssr_newton_work_len", ctypes.byref(desc), ctypes.byref(wl))
+    work = ws.native_work(wl.value)
+    U_it = DeviceArray(U_ini.shape)
+    rec = np.full((int(it_newt_max), 5), np.nan)
+    times = np.zeros(4)
+    flag, it, st = _int(0), _int(0), _int(0)
+    method = 1 if getattr(linear_solver, "method", "dfgmres") == "dcg" else 0
+    t_start = time.perf_counter()
+    call("ssr_newton", ctypes.byref(desc), linear_solver.handle, method, ptr(U_ini), int(it_newt_max),
+         int(it_damp_max), float(tol), float(r_min), ptr(work), wl.value, ptr(U_it), ctypes.byref(flag),
+         ctypes.byref(it), ctypes.byref(st), ptr(rec), ptr(times))
+    wall = time.perf_counter() - t_start
+    linear_solver._K = cmb.K_r
+    n_it = it.value
+    hist = dict(residual=[], r=[], alpha=[], linear_iters=[])
+    for i in range(n_it):
+        row = rec[i]
+        if i == 0 or hist["alpha"] and hist["alpha"][-1] != 0:  # a fresh residual was computed
+            hist["residual"].append(float(row[0]))
+        if math.isnan(row[1]):  # stopped at the convergence test
+            continue
+        hist["r"].append(float(row[1]))
+        hist["alpha"].append(float(row[2]))
+        hist["linear_iters"].append(int(row[3]))
+        _count_native(linear_solver, int(row[4]), [int(row[3])])
+        if log and not (i == n_it - 1 and st.value == 2):
+            log(f"  newton it={i + 1}  rel_resid={row[0]:.2e}  alpha={row[2]:.2f}  r={row[1]:.3g}  "
+                f"lin_it={int(row[3])}")
+    _record_native(linear_solver, times, wall)
     hist["status"] = NEWTON_STATUS.get(st.value, "unknown")
-    hist["wall_time"] = wall
-    return U_it, lam.value, flag.value, n_it, hist
+    return U_it, flag.value, n_it, hist
 
 
 def _mask_inplace(x, mask):
@@ -325,10 +373,15 @@ def damping(it_damp_max, U_it, dU, F, f, cmb, ws):
     return alpha
 
 
-def newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, linear_solver, ws=None, log=None):
+def newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, linear_solver, ws=None, log=None,
+           driver="python"):
     """Newton at the current (reduced) material; returns (U, flag_N, it, history)."""
     if not cmb.pattern_initialized:
         cmb.init_K_r_pattern(Q)
+    if driver == "native":
+        return _newton_native(U_ini, tol, it_newt_max, it_damp_max, r_min, f, cmb, linear_solver, ws, log)
+    if driver != "python":
+        raise ValueError(f"unknown Newton driver {driver!r} (expected one of {DRIVERS})")
     mask = cmb.mask
     if ws is None:
         ws = Workspace(cmb.n_n, cmb.dim)

@@ -69,6 +69,10 @@ def test_newton_driver_argument_checks():
     assert b"null argument" in lib.ssr_last_error()
     n = ctypes.c_int64(0)
     assert lib.ssr_newton_work_len(ctypes.byref(d), ctypes.byref(n)) == 1
+    rc = lib.ssr_newton(ctypes.byref(d), fake, 0, fake, 10, 10, ctypes.c_double(1e-4), ctypes.c_double(1e-4), fake,
+                        ctypes.c_int64(0), fake, *[ctypes.byref(o) for o in out[1:]], None, None)
+    assert rc == 1
+    assert b"incomplete constitutive description" in lib.ssr_last_error()
 
 
 def test_no_cpu_fallback_when_no_gpu():

@@ -106,6 +106,7 @@ def test_native_newton_driver_matches_python_driver(coarse2d, solver, precond, p
     assert np.abs(np.asarray(a["lambda_hist"]) - b["lambda_hist"]).max() <= 1e-12
     assert np.abs(np.asarray(a["omega_hist"]) - b["omega_hist"]).max() <= 1e-12 * np.abs(a["omega_hist"]).max()
     assert a["newton_iterations"] == b["newton_iterations"]
+    assert a["stats"]["init_newton_iterations"] == b["stats"]["init_newton_iterations"]
     assert a["counts"]["DFGMRES.solve.n_iters"] == b["counts"]["DFGMRES.solve.n_iters"]
     assert a["counts"]["DFGMRES.deflation_size.max"] == b["counts"]["DFGMRES.deflation_size.max"]
     Ua, Ub = a["U"].get(), b["U"].get()
