@@ -78,6 +78,8 @@ __host__ __device__ __forceinline__ int mc3d_coef(const double t[6], double cb, 
     e2 = -2.0 * rq * cos(th - kTwoPiOver3) + I1 / 3.0;
     e3 = -2.0 * rq * cos(th) + I1 / 3.0;
 
+    // field by field: an aggregate `k = Coef3{}` was left partly in scratch
+    // memory (32 bytes per lane stored and reloaded through HBM)
     k.sP1 = k.sP3 = k.sI = k.sT = 0.0;
     k.a = k.b = 0.0;
     k.c11 = k.c33 = k.c13 = k.cII = k.c1I = k.c3I = k.cIT = k.c1T = k.c3T = 0.0;
