@@ -304,6 +304,17 @@ int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver, int method
 int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini, int it_newt_max,
                int it_damp_max, double tol, double r_min, double *work, int64_t work_len, double *U_it,
                int *flag_N, int *it, int *status, double *hist, double *times);
+/* [U_it, t_it, flag_N, it] = newton_ind_LL(U_ini, t_ini, omega, it_newt_max, it_damp_max,
+ *     tol, r_min, K_elast, Q, f, constitutive_matrix_builder, linear_solver)
+ * replaces slope_stability/+NEWTON/newton_ind_LL.m:1-198 (limit-load method): bordered
+ * Newton for F(U) = t f, f'U = omega at the material on C, damped by
+ * damping.m against a zero field; t takes the full increment and reaching
+ * it_newt_max is a failure.  Arguments and hist rows as for ssr_newton_ind_SSR
+ * (relative residual, r, alpha, Krylov iterations for W and V, deflation size). */
+int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini,
+                      double t_ini, double omega, int it_newt_max, int it_damp_max, double tol, double r_min,
+                      double *work, int64_t work_len, double *U_it, double *t_out, int *flag_N, int *it,
+                      int *status, double *hist, double *times);
 
 /* ---- learned continuation predictor (small MLP) --------------------------- */
 /* No reference counterpart: the reference continuation has the secant

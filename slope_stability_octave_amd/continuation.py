@@ -333,13 +333,14 @@ def SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_d
 # Limit-load (LL) indirect continuation
 # ---------------------------------------------------------------------------
 def LL_indirect_continuation(d_omega_ini, d_t_min, step_max, omega_max, it_newt_max, it_damp_max, tol, r_min,
-                             K_elast, U_elast, Q, f, cmb, linear_system_solver, log=None):
+                             K_elast, U_elast, Q, f, cmb, linear_system_solver, log=None, driver="python"):
     """LL_indirect_continuation.m:1-160: omega-stepping of the load factor t
     from the scaled elastic field, secant predictor after the first step,
     halved d_omega on failure (at most five times in a row), doubled when the
     t increments shrink and Newton took fewer than 20 iterations; stops on
     d_t < d_t_min, omega_max or step_max.  Returns (U, t_hist, omega_hist,
-    Umax_hist, newton_iterations)."""
+    Umax_hist, newton_iterations).  ``driver``: "python" or "native" Newton
+    loop (newton.newton_ind_LL)."""
     from .newton import newton_ind_LL
     if not cmb.pattern_initialized:
         cmb.init_K_r_pattern(Q)
@@ -364,7 +365,7 @@ def LL_indirect_continuation(d_omega_ini, d_t_min, step_max, omega_max, it_newt_
             a = d_omega / (omega - omega_old)
             lincomb(1.0 + a, U, -a, U_old, U_ini)
         U_it, t_it, flag, its = newton_ind_LL(U_ini, t, omega_it, it_newt_max, it_damp_max, tol, r_min, K_elast, Q,
-                                              f, cmb, linear_system_solver.copy(), ws, log)
+                                              f, cmb, linear_system_solver.copy(), ws, log, driver=driver)
         newton_total += its
         if flag == 1:
             if log:

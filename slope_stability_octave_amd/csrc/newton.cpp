@@ -122,11 +122,12 @@ int damping_ALG5(const NewtonCtx &x, int it_damp_max, const double *U_it, double
 }
 
 // damping.m:25-54: bisection on the directional derivative (F(U + a dU) - f)'dU
-// over all DOFs, at the material the caller reduced
-int damping(const NewtonCtx &x, int it_damp_max, const double *U_it, const double *F, double *alpha_out)
+// over all DOFs, at the material the caller reduced (f: the load vector, or
+// a zero field for the limit-load Newton)
+int damping(const NewtonCtx &x, int it_damp_max, const double *U_it, const double *F, const double *f,
+            double *alpha_out)
 {
     const int64_t n = x.n_dof;
-    const double *f = x.C->f;
     double decrease = 0.0, dd = 0.0;
     NCHECK(ssr_dev_lincomb(n, 1.0, F, -1.0, f, x.tmp));
     NCHECK(ssr_dot(n, x.tmp, x.dU, nullptr, &decrease));
@@ -397,7 +398,7 @@ extern "C" int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int met
         tm[0] += lap.take();
         if (h) h[3] = lin_it;
         double alpha = 0.0;
-        NCHECK(damping(x, it_damp_max, U_it, x.F, &alpha));
+        NCHECK(damping(x, it_damp_max, U_it, x.F, C->f, &alpha));
         if (h) h[2] = alpha;
         // (:121-143)
         compute_diffs = true;
@@ -429,6 +430,130 @@ extern "C" int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int met
     tm[3] += lap.take();
     if (times)
         for (int q = 0; q < 4; ++q) times[q] += tm[q];
+    *flag_out = flag_N;
+    *it_out = it;
+    *status_out = status;
+    return SSR_OK;
+}
+
+extern "C" int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, int method, const double *U_ini,
+                                 double t_ini, double omega, int it_newt_max, int it_damp_max, double tol,
+                                 double r_min, double *work, int64_t work_len, double *U_it, double *t_out,
+                                 int *flag_out, int *it_out, int *status_out, double *hist, double *times)
+{
+    SSR_REQUIRE(C && solver && U_ini && work && U_it && t_out && flag_out && it_out && status_out,
+                "newton_ind_LL: null argument");
+    NewtonCtx x;
+    NCHECK(prepare(C, solver, method, it_newt_max, it_damp_max, work, work_len, &x));
+    const int64_t n_dof = x.n_dof;
+    double *zero = x.G;  // the damping's target: F(U) against f = 0
+    NCHECK(ssr_dev_fill(n_dof, 0.0, zero));
+
+    // newton_ind_LL.m setup (the lines before the loop at :80)
+    NCHECK(ssr_dev_lincomb(n_dof, 1.0, U_ini, 0.0, U_ini, U_it));
+    double t_it = t_ini;
+    double ff = 0.0;
+    NCHECK(ssr_dot(n_dof, C->f, C->f, C->mask, &ff));
+    const double norm_f = sqrt(ff);
+    int flag_N = 0;
+    bool compute_diffs = true;
+    double r = r_min;
+    int it = 0;
+    double rel_resid = NAN;
+    int status = SSR_NEWTON_UNKNOWN;
+    if (hist)
+        for (int64_t k = 0; k < 6 * (int64_t)it_newt_max; ++k) hist[k] = NAN;
+    double tm[4] = {0.0, 0.0, 0.0, 0.0};
+    Lap lap;
+
+    // newton_ind_LL.m:80-177: F(U) = t f, f'U = omega at the unreduced material
+    while (true) {
+        ++it;
+        double *h = hist ? hist + 6 * (int64_t)(it - 1) : nullptr;
+        if (compute_diffs) NCHECK(build_F(x, 0.0, U_it, x.F, 1, false));
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0, x.F, -t_it, C->f, x.tmp));
+        double s2 = 0.0;
+        NCHECK(ssr_dot(n_dof, x.tmp, x.tmp, C->mask, &s2));
+        const double criterion = sqrt(s2 > 0.0 ? s2 : 0.0);
+        rel_resid = criterion / norm_f;
+        if (h) h[0] = rel_resid;
+        if (criterion < tol * norm_f && it > 1) {
+            status = SSR_NEWTON_CONVERGED;
+            break;
+        }
+        if (h) h[1] = r;
+        NCHECK(ssr_dev_assemble_bsr(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT,
+                                    C->K_tang));
+        NCHECK(ssr_dev_regularize(C->pattern, r, C->K_elast, C->K_tang, C->K_r));
+        tm[3] += lap.take();
+        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
+        tm[2] += lap.take();
+        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 0));
+        NCHECK(ssr_solver_A_orthogonalize(solver));
+        tm[1] += lap.take();
+        if (h) {
+            int k = 0;
+            NCHECK(ssr_solver_deflation_size(solver, &k));
+            h[5] = k;
+        }
+        // W = K \ f(Q),  V = K \ (t f(Q) - F(Q))
+        int lin_w = 0, lin_v = 0;
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0, C->f, 0.0, C->f, x.rhs));
+        NCHECK(ssr_dev_mul(n_dof, C->mask, x.rhs));
+        NCHECK(solve(x, x.rhs, x.W, &lin_w));
+        NCHECK(ssr_dev_lincomb(n_dof, t_it, C->f, -1.0, x.F, x.rhs));
+        NCHECK(ssr_dev_mul(n_dof, C->mask, x.rhs));
+        NCHECK(solve(x, x.rhs, x.V, &lin_v));
+        tm[0] += lap.take();
+        if (h) {
+            h[3] = lin_w;
+            h[4] = lin_v;
+        }
+        // d_t = -f'V / f'W,  d_U = V + d_t W
+        double fV = 0.0, fW = 0.0;
+        NCHECK(ssr_dot(n_dof, C->f, x.V, C->mask, &fV));
+        NCHECK(ssr_dot(n_dof, C->f, x.W, C->mask, &fW));
+        const double d_t = fW != 0.0 ? -fV / fW : NAN;
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0, x.V, d_t, x.W, x.dU));
+        double alpha = 0.0;
+        NCHECK(damping(x, it_damp_max, U_it, x.F, zero, &alpha));  // :130, against F_int*0
+        if (h) h[2] = alpha;
+        compute_diffs = true;
+        if (alpha < 1e-1) {
+            if (alpha == 0) {
+                compute_diffs = false;
+                r = r * 2;
+            } else {
+                r = r * pow(2.0, 1.0 / 4);
+            }
+        } else {
+            NCHECK(ssr_solver_expand_deflation(solver, x.W));
+            NCHECK(ssr_solver_expand_deflation(solver, x.V));
+            if (alpha > 0.5) r = fmax(r / sqrt(2.0), r_min);
+        }
+        if (alpha == 0 && r > 1) {
+            status = SSR_NEWTON_STALLED;
+            flag_N = 1;
+            break;
+        }
+        // U += alpha d_U; U = omega U / (f(Q)'U(Q)); t takes the full d_t
+        NCHECK(ssr_dev_axpby(n_dof, alpha, x.dU, 1.0, U_it));
+        double fU = 0.0;
+        NCHECK(ssr_dot(n_dof, C->f, U_it, C->mask, &fU));
+        NCHECK(ssr_dev_axpby(n_dof, 0.0, U_it, omega / fU, U_it));
+        t_it = t_it + d_t;
+        // reaching it_newt_max is a failure here
+        if (isnan(rel_resid) || it == it_newt_max) {
+            flag_N = 1;
+            status = isnan(rel_resid) ? SSR_NEWTON_NAN : SSR_NEWTON_MAXIT;
+            break;
+        }
+    }
+    NCHECK(ssr_synchronize());
+    tm[3] += lap.take();
+    if (times)
+        for (int q = 0; q < 4; ++q) times[q] += tm[q];
+    *t_out = t_it;
     *flag_out = flag_N;
     *it_out = it;
     *status_out = status;

@@ -39,6 +39,8 @@ _lib.extend_signatures({
     "ssr_newton_ind_SSR": [_vp, _vp, _int, _vp, _dbl, _dbl, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp,
                            _vp, _vp, _vp, _vp],
     "ssr_newton": [_vp, _vp, _int, _vp, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ssr_newton_ind_LL": [_vp, _vp, _int, _vp, _dbl, _dbl, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp, _vp,
+                          _vp, _vp, _vp],
 })
 
 DRIVERS = ("python", "native")
@@ -445,7 +447,7 @@ def newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, line
 
 
 def newton_ind_LL(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb, linear_solver,
-                  ws=None, log=None):
+                  ws=None, log=None, driver="python"):
     """newton_ind_LL.m:1-198: bordered Newton for F(U) = t f, f'U = omega at
     the unreduced material (unknowns U and the load factor t), with the
     directional-derivative damping of damping.m against f = 0 (f'dU = 0 by
@@ -454,6 +456,11 @@ def newton_ind_LL(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, K_e
     Returns (U_it, t_it, flag_N, it)."""
     if not cmb.pattern_initialized:
         cmb.init_K_r_pattern(Q)
+    if driver == "native":
+        return _newton_ind_LL_native(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, f, cmb,
+                                     linear_solver, ws, log)
+    if driver != "python":
+        raise ValueError(f"unknown Newton driver {driver!r} (expected one of {DRIVERS})")
     mask = cmb.mask
     if ws is None:
         ws = Workspace(cmb.n_n, cmb.dim)
@@ -517,3 +524,37 @@ def newton_ind_LL(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, K_e
             flag_N = 1
             break
     return U_it, t_it, flag_N, it
+
+
+def _newton_ind_LL_native(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, f, cmb, linear_solver, ws,
+                          log):
+    """newton_ind_LL.m:1-198 as one ssr_newton_ind_LL call."""
+    if ws is None:
+        ws = Workspace(cmb.n_n, cmb.dim)
+    desc = native_description(cmb, f)
+    wl = _i64(0)
+    call("ssr_newton_work_len", ctypes.byref(desc), ctypes.byref(wl))
+    work = ws.native_work(wl.value)
+    U_it = DeviceArray(U_ini.shape)
+    rec = np.full((int(it_newt_max), 6), np.nan)
+    times = np.zeros(4)
+    t, flag, it, st = _dbl(0.0), _int(0), _int(0), _int(0)
+    method = 1 if getattr(linear_solver, "method", "dfgmres") == "dcg" else 0
+    t_start = time.perf_counter()
+    call("ssr_newton_ind_LL", ctypes.byref(desc), linear_solver.handle, method, ptr(U_ini), float(t_ini),
+         float(omega), int(it_newt_max), int(it_damp_max), float(tol), float(r_min), ptr(work), wl.value, ptr(U_it),
+         ctypes.byref(t), ctypes.byref(flag), ctypes.byref(it), ctypes.byref(st), ptr(rec), ptr(times))
+    wall = time.perf_counter() - t_start
+    linear_solver._K = cmb.K_r
+    n_it = it.value
+    for i in range(n_it):
+        row = rec[i]
+        if math.isnan(row[1]):  # stopped at the convergence test
+            continue
+        lin_w, lin_v = int(row[3]), int(row[4])
+        _count_native(linear_solver, int(row[5]), [lin_w, lin_v])
+        if log and not (i == n_it - 1 and st.value == 2):
+            log(f"  newton_ind_LL it={i + 1}  rel_resid={row[0]:.2e}  alpha={row[2]:.2f}  r={row[1]:.3g}  "
+                f"lin_it=[W:{lin_w},V:{lin_v}]")
+    _record_native(linear_solver, times, wall)
+    return U_it, t.value, flag.value, n_it

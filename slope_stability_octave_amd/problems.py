@@ -228,11 +228,12 @@ PARAMS_LL_2D = dict(d_t_min=1e-3, step_max=100, omega_max=2000.0, it_newt_max=10
                     r_min=1e-4, linear_tol=1e-1, linear_maxit=100, deflation_tol=1e-3)
 
 
-def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, precision=64, elastic_tol=1e-10):
+def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, precision=64, elastic_tol=1e-10,
+           driver="native"):
     """Limit-load analysis (slope_stability_2D_homo_LL.m:205-223): the elastic
     field at the unreduced material, d_omega_ini = omega_el / 5, the elastic
     field scaled by 1/5 as the start, then the LL indirect continuation.  The
-    limit load is t_hist[-1]."""
+    limit load is t_hist[-1].  ``driver``: "native" or "python" Newton loop."""
     from .continuation import LL_indirect_continuation
     from .linear_solvers import DFGMRES, dot, lincomb
     p = dict(PARAMS_LL_2D)
@@ -266,7 +267,7 @@ def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, 
     t1 = time.perf_counter()
     U, t_h, om_h, umax_h, its = LL_indirect_continuation(
         omega_el / 5, p["d_t_min"], p["step_max"], p["omega_max"], p["it_newt_max"], p["it_damp_max"], p["tol"],
-        p["r_min"], None, U_el, prob["Q"], f, cmb, lin.copy(), log=log)
+        p["r_min"], None, U_el, prob["Q"], f, cmb, lin.copy(), log=log, driver=driver)
     t_run = time.perf_counter() - t1
     timings = dict(cmb.timings)
     timings.update(lin.timings)

@@ -39,3 +39,22 @@ def test_ll_2d_recorded_curve():
     # the curve flattens at the limit load (last increments below 1e-3)
     assert t[-1] - t[-2] < 1e-3
     assert 1.529 < res["limit_load"] < 1.535
+
+
+def test_ll_native_driver_matches_python_driver():
+    """ssr_newton_ind_LL (the LL Newton loop inside the library) against the
+    Python loop over the same C entry points, on a coarse mesh: the same
+    kernels in the same order give the same t(omega) curve and counts."""
+    prob = problems.slope_2d(h=2.0)
+    runs = {}
+    for driver in ("python", "native"):
+        print(f"LL driver={driver}", flush=True)
+        runs[driver] = problems.run_ll(prob, dict(linear_tol=1e-2), preconditioner="amg", driver=driver)
+    a, b = runs["python"], runs["native"]
+    print("limit load", a["limit_load"], b["limit_load"], "Newton", a["newton_iterations"], b["newton_iterations"],
+          flush=True)
+    assert len(a["t_hist"]) == len(b["t_hist"]) and len(a["t_hist"]) > 3
+    assert np.abs(np.asarray(a["t_hist"]) - b["t_hist"]).max() <= 1e-12
+    assert np.abs(np.asarray(a["omega_hist"]) - b["omega_hist"]).max() <= 1e-12 * np.abs(a["omega_hist"]).max()
+    assert a["newton_iterations"] == b["newton_iterations"]
+    assert a["counts"]["DFGMRES.solve.n_iters"] == b["counts"]["DFGMRES.solve.n_iters"]
