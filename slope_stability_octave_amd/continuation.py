@@ -207,13 +207,14 @@ def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
 # Direct SSR continuation (lambda-stepping with omega = -dJ/dlambda)
 # ---------------------------------------------------------------------------
 def omega_SSR_direct_continuation(lambda_ini, U_ini, eps, d_lambda, it_newt_max, it_damp_max, tol, r_min, K_elast,
-                                  Q, f, cmb, solver, ws, log=None):
+                                  Q, f, cmb, solver, ws, log=None, driver="python"):
     """omega_SSR_direct_continuation.m:49-81: U at lambda, and omega as the
     backward difference of the energy J = Psi - f'U over eps.  Returns
     (U, omega, flag, newton iterations)."""
     mask = cmb.mask
     cmb.reduction(lambda_ini)
-    U, flag, it1, _ = newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver, ws, log)
+    U, flag, it1, _ = newton(U_ini, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver, ws, log,
+                             driver=driver)
     if flag == 1:
         return U, 0.0, 1, it1
     J = cmb.potential_energy(U) - dot(f, U, mask)
@@ -221,7 +222,8 @@ def omega_SSR_direct_continuation(lambda_ini, U_ini, eps, d_lambda, it_newt_max,
     U_beta = DeviceArray(U.shape)
     lincomb(beta, U_ini, 1.0 - beta, U, U_beta)
     cmb.reduction(lambda_ini - eps)
-    U_eps, flag, it2, _ = newton(U_beta, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver, ws, log)
+    U_eps, flag, it2, _ = newton(U_beta, tol, it_newt_max, it_damp_max, r_min, K_elast, Q, f, cmb, solver, ws, log,
+                                 driver=driver)
     if flag == 1:
         return U, 0.0, 1, it1 + it2
     J_eps = cmb.potential_energy(U_eps) - dot(f, U_eps, mask)
@@ -229,13 +231,14 @@ def omega_SSR_direct_continuation(lambda_ini, U_ini, eps, d_lambda, it_newt_max,
 
 
 def init_phase_SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol,
-                                       eps, r_min, K_elast, Q, f, cmb, solver, ws, log=None):
+                                       eps, r_min, K_elast, Q, f, cmb, solver, ws, log=None, driver="python"):
     """init_phase_SSR_direct_continuation.m:1-143."""
     U0 = DeviceArray(f.shape, zero=True)
     if log:
         log(f"Init step 1: lambda={lambda_init:.6g}")
     U1, omega1, flag, _ = omega_SSR_direct_continuation(lambda_init, U0, eps, 1000.0, it_newt_max, it_damp_max, tol,
-                                                        r_min, K_elast, Q, f, cmb, solver.copy(), ws, log)
+                                                        r_min, K_elast, Q, f, cmb, solver.copy(), ws, log,
+                                                        driver=driver)
     if flag == 1:
         raise ContinuationError("Initial choice of lambda seems to be too large.")
     d_lambda = d_lambda_init
@@ -245,7 +248,7 @@ def init_phase_SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min,
         solver.expand_deflation_basis(U1)
         U_it, omega_it, flag, _ = omega_SSR_direct_continuation(lambda_it, U1, eps, d_lambda, it_newt_max,
                                                                 it_damp_max, tol, r_min, K_elast, Q, f, cmb,
-                                                                solver.copy(), ws, log)
+                                                                solver.copy(), ws, log, driver=driver)
         if flag == 1:
             _rebuild_after_failure(solver)
             d_lambda = d_lambda / 2
@@ -265,7 +268,8 @@ def init_phase_SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min,
 
 
 def SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
-                            it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb, linear_system_solver, log=None):
+                            it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb, linear_system_solver, log=None,
+                            driver="python"):
     """SSR_direct_continuation.m:47-146: lambda steps, halved on failure or a
     decreasing omega, halved when omega grows by more than 1.5x; stops on the
     scaled lambda increase, the minimal increment or the step limit.
@@ -277,7 +281,7 @@ def SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_d
     eps = tol * 100
     U_old, U, omega_old, omega, lambda_old, lam = init_phase_SSR_direct_continuation(
         lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol, eps, r_min, K_elast, Q, f, cmb,
-        linear_system_solver, ws, log)
+        linear_system_solver, ws, log, driver=driver)
     linear_system_solver.expand_deflation_basis(U)
     lambda_hist, omega_hist = [lambda_old, lam], [omega_old, omega]
     Umax_hist = [_umax(U), _umax(U)]
@@ -291,7 +295,7 @@ def SSR_direct_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_d
             log(f"Step {step + 1}: lambda_target={lambda_it:.6g}")
         U_it, omega_it, flag, _ = omega_SSR_direct_continuation(lambda_it, U, eps, d_lambda, it_newt_max,
                                                                 it_damp_max, tol, r_min, K_elast, Q, f, cmb,
-                                                                linear_system_solver.copy(), ws, log)
+                                                                linear_system_solver.copy(), ws, log, driver=driver)
         d_omega_test = omega_it - omega_hist[-1]
         if flag == 1 or d_omega_test < 0:
             if log:

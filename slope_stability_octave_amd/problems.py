@@ -155,7 +155,8 @@ def slope_2d_family(seed, h=1.0):
     return slope_2d(h=h, x1=15.0, x2=x2, x3=15.0, y1=10.0, y2=y2, materials=mats), (beta, c0, phi)
 
 
-def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_options=None, precision=64):
+def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_options=None, precision=64,
+                   driver="native"):
     """Direct SSR continuation (SSR_direct_continuation.m) on the GPU: the
     second factor-of-safety estimate of the reference drivers."""
     from .continuation import SSR_direct_continuation
@@ -174,7 +175,8 @@ def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_option
     t1 = time.perf_counter()
     U, lam_h, om_h, umax_h, work_h = SSR_direct_continuation(
         p["lambda_init"], p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
-        p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], None, prob["Q"], f, cmb, lin, log=log)
+        p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], None, prob["Q"], f, cmb, lin, log=log,
+        driver=driver)
     t_run = time.perf_counter() - t1
     timings = dict(cmb.timings)
     timings.update(lin.timings)

@@ -51,3 +51,19 @@ def test_direct_continuation_recorded_omegas():
     for (l_ref, o_ref), l, o in zip(REF_DIRECT, lam, om):
         assert abs(l - l_ref) < 1e-12
         assert abs(o - o_ref) <= 1e-4 * o_ref + 5e-6
+
+
+def test_direct_continuation_native_driver_matches_python_driver():
+    """The direct continuation's Newton solves (newton.m at each lambda and at
+    lambda - eps) through ssr_newton against the Python loop: same omegas."""
+    prob = problems.slope_2d(h=1.0)
+    p = dict(problems.PARAMS_2D, linear_tol=1e-3, linear_maxit=300, step_max=5)
+    runs = {}
+    for driver in ("python", "native"):
+        print(f"direct driver={driver}", flush=True)
+        runs[driver] = problems.run_ssr_direct(prob, p, preconditioner="amg", driver=driver)
+    a, b = runs["python"], runs["native"]
+    print("lambda", list(a["lambda_hist"]), "omega", list(a["omega_hist"]), list(b["omega_hist"]), flush=True)
+    assert len(a["lambda_hist"]) == len(b["lambda_hist"]) >= 3
+    assert np.abs(np.asarray(a["lambda_hist"]) - b["lambda_hist"]).max() <= 1e-12
+    assert np.abs(np.asarray(a["omega_hist"]) - b["omega_hist"]).max() <= 1e-12 * np.abs(a["omega_hist"]).max()
