@@ -59,7 +59,9 @@ int reduction(const NewtonCtx &x, double lam)
                 break;
             }
     }
-    NCHECK(ssr_htod(C->lambda, &lam, sizeof(double)));
+    // lambda reaches the device by a one-element fill on the stream: no host
+    // synchronisation (a host copy would drain the queue)
+    NCHECK(ssr_dev_fill(1, lam, C->lambda));
     return ssr_dev_reduction(C->n_int, C->davis, C->c0, C->phi, C->psi, C->lambda, 0, c_all, C->c_bar,
                              C->sin_phi);
 }
