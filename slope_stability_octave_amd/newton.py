@@ -12,10 +12,12 @@ reductions (norms, f'V) return scalars to the host, which owns the control
 flow exactly as the reference's Octave loop does.  Restriction to the free
 DOFs Q is the Dirichlet mask (zero at fixed DOFs).
 
-``newton_ind_SSR(..., driver="native")`` runs the same loop inside
-libssr_hip.so (``ssr_newton_ind_SSR``): one library call per Newton solve
-instead of a few dozen per iteration, with the same kernels in the same
-order, so both drivers give the same iterates.
+``driver="native"`` runs the same loops inside libssr_hip.so
+(``ssr_newton_ind_SSR``, ``ssr_newton``, ``ssr_newton_ind_LL``): one library
+call per Newton solve instead of a few dozen per iteration, with the same
+kernels in the same order, so both drivers give the same iterates.  The
+functions here default to the Python loop; the continuations started by
+problems.run_ssr / run_ssr_direct / run_ll default to the native one.
 """
 from __future__ import annotations
 
