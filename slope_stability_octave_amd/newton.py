@@ -331,7 +331,7 @@ def _newton_native(U_ini, tol, it_newt_max, it_damp_max, r_min, f, cmb, linear_s
     hist = dict(residual=[], r=[], alpha=[], linear_iters=[])
     for i in range(n_it):
         row = rec[i]
-        if i == 0 or hist["alpha"] and hist["alpha"][-1] != 0:  # a fresh residual was computed
+        if i == 0 or (hist["alpha"] and hist["alpha"][-1] != 0):  # a fresh residual was computed
             hist["residual"].append(float(row[0]))
         if math.isnan(row[1]):  # stopped at the convergence test
             continue
