@@ -411,8 +411,6 @@ def main():
     ap.add_argument("--amg-per-step", action="store_true",
                     help="a new AMG hierarchy for every continuation step (default: one hierarchy shared by all "
                          "steps, rebuilt every --amg-rebuild-every setups)")
-    ap.add_argument("--no-amg-graph", action="store_true",
-                    help="launch the V-cycle kernel by kernel instead of replaying its captured graph")
     ap.add_argument("--autotune", action="store_true",
                     help="time the 3D kernel forms on this workload first and use the fastest")
     ap.add_argument("--hbm-probe", action="store_true",
@@ -426,9 +424,6 @@ def main():
     if args.variant is not None:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_CONSTITUTIVE_VARIANT, args.variant)
-    if args.no_amg_graph:
-        from slope_stability_octave_amd import _lib as L
-        L.set_tuning(L.TUNE_AMG_GRAPH, 0)
     if args.amg_block_smooth is not None:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_AMG_BLOCK_SMOOTH, args.amg_block_smooth)

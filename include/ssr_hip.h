@@ -112,7 +112,8 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *  key 1: lanes per block row of the node-block SpMV (0 = by dimension,
  *         4, 8, 16, 32; 64 = one wavefront per row);
  *  key 2: wavefronts per compute unit of persistent grids (0 = automatic);
- *  key 3: 1 = replay the AMG V-cycle as a captured HIP graph (0 = eager, default);
+ *  key 3: 0 only (eager V-cycle launches); replay from a captured HIP graph is
+ *         disabled and any other value is rejected;
  *  key 4: 1 = node-block Jacobi scaling in the finest-level Chebyshev smoother
  *         (default), 0 = scalar;
  *  key 5: assembly and internal-force form (0 = element contributions, then

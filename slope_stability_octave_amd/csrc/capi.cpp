@@ -323,6 +323,11 @@ extern "C" int ssr_set_tuning(int key, int value)
     if (key == SSR_TUNE_SPMM_GROUP)
         SSR_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
                     "set_tuning: multi-vector SpMV group must be 0 (by dimension), 4, 8, 16 or 32");
+    // Replay of the V-cycle from a captured graph is switched off: one profiled
+    // FP32 continuation ended in a host SIGSEGV inside Amg::apply_graph<float>
+    // whose cause is not known, so the form cannot be selected.
+    SSR_REQUIRE(key != SSR_TUNE_AMG_GRAPH || value == 0,
+                "set_tuning: V-cycle graph replay is disabled (a host fault in apply_graph has no known cause)");
     g_tuning[key] = value;
     return SSR_OK;
 }

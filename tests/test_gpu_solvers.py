@@ -306,6 +306,8 @@ def test_mixed_precision_refinement_meets_fp64_tolerance(precond, tol):
     assert d < max(1e3 * tol, 1e-4), d
 
 
+@pytest.mark.skip(reason="V-cycle graph replay is disabled: a profiled FP32 continuation ended in a host "
+                         "SIGSEGV inside Amg::apply_graph<float> whose cause is not known")
 @pytest.mark.parametrize("prec", [64, 32])
 def test_amg_graph_replays_match_eager(prec):
     """The V-cycle replayed from its captured HIP graph equals the eager
