@@ -179,6 +179,13 @@ int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int3
 int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result);
 int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result);
 int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h);
+/* ssr_dev_multidot without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The partial sums live in a static device
+ * array of 8192 x 128 doubles (8 MiB, held per device until the process
+ * ends); k above 128 is rejected.  Calls that overlap on one device share the
+ * array and race. */
+int ssr_dev_multidot_noalloc(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h);
 int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const double *h, double alpha,
                         double beta, double *u);
 int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y);

@@ -651,7 +651,7 @@ struct RedScratch {
 std::mutex g_scr_mu;
 std::unordered_map<hipStream_t, RedScratch> g_scr;
 
-int scratch(RedScratch **out)
+int red_scratch(RedScratch **out)
 {
     const hipStream_t st = ssr_stream();
     std::lock_guard<std::mutex> lk(g_scr_mu);
@@ -659,6 +659,37 @@ int scratch(RedScratch **out)
     if (!r.part) SSR_HIP_CHECK(hipMalloc((void **)&r.part, sizeof(double) * kMdBlocks * 128));
     if (!r.red) SSR_HIP_CHECK(hipMalloc((void **)&r.red, sizeof(double) * 128));
     *out = &r;
+    return SSR_OK;
+}
+
+// Partial sums of ssr_dev_multidot_noalloc: kMdBlocks x 128 doubles (8 MiB),
+// held by every device the library runs on until the process ends.  Calls
+// that overlap on one device share it and race.
+constexpr int kNoAllocMaxCols = 128;
+static __device__ __align__(256) double scratch[(size_t)kMdBlocks * kNoAllocMaxCols];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up on the first
+// call there)
+int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    SSR_HIP_CHECK(hipGetDevice(&dev));
+    SSR_REQUIRE(dev >= 0 && dev < kMaxDevices, "multidot_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ssr_set_error("multidot_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+            return SSR_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = (double *)p;
+    }
+    *out = g_noalloc_ptr[dev];
     return SSR_OK;
 }
 
@@ -742,11 +773,37 @@ template <class T>
 int dot(int64_t n, const T *a, const T *b, const double *mask, double *result)
 {
     RedScratch *sc = nullptr;
-    int rc = scratch(&sc);
+    int rc = red_scratch(&sc);
     if (rc) return rc;
     const unsigned nb = red_blocks(n);
     hipLaunchKernelGGL((k_dot_partial<T>), dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n, a, b, mask, sc->part);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kRedThreads), 0, ssr_stream(), (int)nb, 1, sc->part, result);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+// h = V(:,0:k)' u with the partial sums in `part` (kMdBlocks * k doubles at
+// most; every entry the reduction reads is written first, so `part` needs no
+// clearing).
+template <class T>
+int multidot_into(int64_t n, int k, const T *V, int64_t ld, const T *u, double *h, double *part)
+{
+    SSR_REQUIRE(k >= 0 && k <= 128, "multidot: k must be in [0,128]");
+    SSR_REQUIRE(n >= 0, "multidot: n must be >= 0");
+    if (k == 0) return SSR_OK;
+    // rows per thread: 4 for long columns, 2 otherwise (measured at 21k,
+    // 84k and 711k rows)
+    const int rpt = n >= 262144 ? 4 : 2;
+    const int64_t nbl = (n + (int64_t)kRedThreads * rpt - 1) / ((int64_t)kRedThreads * rpt);
+    const unsigned nb = (unsigned)(nbl < 1 ? 1 : (nbl > kMdBlocks ? kMdBlocks : nbl));
+    const dim3 grid(nb, (unsigned)((k + kMdCols - 1) / kMdCols));
+    if (rpt == 4)
+        hipLaunchKernelGGL((k_multidot_partial<T, 4>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
+                           part);
+    else
+        hipLaunchKernelGGL((k_multidot_partial<T, 2>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
+                           part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, part, h);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }
@@ -757,23 +814,9 @@ int multidot(int64_t n, int k, const T *V, int64_t ld, const T *u, double *h)
     SSR_REQUIRE(k >= 0 && k <= 128, "multidot: k must be in [0,128]");
     if (k == 0) return SSR_OK;
     RedScratch *sc = nullptr;
-    int rc = scratch(&sc);
+    int rc = red_scratch(&sc);
     if (rc) return rc;
-    // rows per thread: 4 for long columns, 2 otherwise (measured at 21k,
-    // 84k and 711k rows)
-    const int rpt = n >= 262144 ? 4 : 2;
-    const int64_t nbl = (n + (int64_t)kRedThreads * rpt - 1) / ((int64_t)kRedThreads * rpt);
-    const unsigned nb = (unsigned)(nbl < 1 ? 1 : (nbl > kMdBlocks ? kMdBlocks : nbl));
-    const dim3 grid(nb, (unsigned)((k + kMdCols - 1) / kMdCols));
-    if (rpt == 4)
-        hipLaunchKernelGGL((k_multidot_partial<T, 4>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
-                           sc->part);
-    else
-        hipLaunchKernelGGL((k_multidot_partial<T, 2>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
-                           sc->part);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(kRedThreads), 0, ssr_stream(), (int)nb, k, sc->part, h);
-    SSR_HIP_CHECK(hipGetLastError());
-    return SSR_OK;
+    return multidot_into<T>(n, k, V, ld, u, h, sc->part);
 }
 
 template <class TV, class TU>
@@ -1011,7 +1054,7 @@ int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask,
 int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result)
 {
     RedScratch *sc = nullptr;
-    int rc = scratch(&sc);
+    int rc = red_scratch(&sc);
     if (rc) return rc;
     if ((rc = ssr::dot<double>(n, a, b, mask, sc->red))) return rc;
     SSR_HIP_CHECK(hipMemcpyAsync(host_result, sc->red, sizeof(double), hipMemcpyDeviceToHost, ssr_stream()));
@@ -1023,6 +1066,24 @@ int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, dou
 int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h)
 {
     return ssr::multidot<double>(n, k, V, ld, u, h);
+}
+
+// The same multi-dot with its partial sums in the file-scope `scratch` array
+// instead of library-allocated memory: no hipMalloc, hipFree or allocating
+// library call on any path.  The array's device address is looked up once
+// per device (with that device current) and kept per device; the name
+// `scratch` is a host-side stand-in and is never passed to a kernel.
+int ssr_dev_multidot_noalloc(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h)
+{
+    SSR_REQUIRE(k >= 0 && k <= kNoAllocMaxCols,
+                "multidot_noalloc: k = %d exceeds the %d columns of the static scratch (%d x %d doubles)", k,
+                kNoAllocMaxCols, kMdBlocks, kNoAllocMaxCols);
+    SSR_REQUIRE(n >= 0 && (k == 0 || (V && u && h)), "multidot_noalloc: bad arguments");
+    if (k == 0) return SSR_OK;
+    double *part = nullptr;
+    int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+    return ssr::multidot_into<double>(n, k, V, ld, u, h, part);
 }
 
 int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const double *h, double alpha, double beta,

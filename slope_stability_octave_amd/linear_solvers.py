@@ -46,6 +46,7 @@ _lib.extend_signatures({
     "ssr_dev_dot": [_i64, _vp, _vp, _vp, _vp],
     "ssr_dot": [_i64, _vp, _vp, _vp, _vp],
     "ssr_dev_multidot": [_i64, _int, _vp, _i64, _vp, _vp],
+    "ssr_dev_multidot_noalloc": [_i64, _int, _vp, _i64, _vp, _vp],
     "ssr_dev_gemv_update": [_i64, _int, _vp, _i64, _vp, _dbl, _dbl, _vp],
     "ssr_dev_axpby": [_i64, _dbl, _vp, _dbl, _vp],
     "ssr_dev_lincomb": [_i64, _dbl, _vp, _dbl, _vp, _vp],
