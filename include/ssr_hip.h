@@ -118,7 +118,9 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         (default), 0 = scalar;
  *  key 5: assembly and internal-force form (0 = element contributions, then
  *         a deterministic per-block / per-node gather, default; 1 = FP64 atomics);
- *  key 6: lanes per block row of the multi-vector SpMV (0 = by dimension). */
+ *  key 6: lanes per block row of the multi-vector SpMV (0 = by dimension);
+ *  key 7: blocks per lane per pass of the node-block SpMV (0 = by dimension:
+ *         2 in 3D with 16 lanes per row, 1 in 2D; or 1, 2). */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 

@@ -198,6 +198,7 @@ TUNE_AMG_GRAPH = 3
 TUNE_AMG_BLOCK_SMOOTH = 4
 TUNE_ASSEMBLY_FORM = 5
 TUNE_SPMM_GROUP = 6
+TUNE_SPMV_UNROLL = 7
 
 
 def set_tuning(key: int, value: int):

@@ -323,6 +323,8 @@ extern "C" int ssr_set_tuning(int key, int value)
     if (key == SSR_TUNE_SPMM_GROUP)
         SSR_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
                     "set_tuning: multi-vector SpMV group must be 0 (by dimension), 4, 8, 16 or 32");
+    if (key == SSR_TUNE_SPMV_UNROLL)
+        SSR_REQUIRE(value >= 0 && value <= 2, "set_tuning: SpMV blocks per lane must be 0 (by dimension), 1 or 2");
     // Replay of the V-cycle from a captured graph is switched off: one profiled
     // FP32 continuation ended in a host SIGSEGV inside Amg::apply_graph<float>
     // whose cause is not known, so the form cannot be selected.

@@ -46,12 +46,18 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 // one block row: lanes stride over the row's blocks, each forming dim partial
 // outputs, then the group reduces with shuffles.  Block values row-major.
 // ---------------------------------------------------------------------------
-template <int DIM, int G, class T>
+// U blocks per lane per pass (U = 2: lane l takes blocks b and b + G, and
+// issues the loads of both before the products, which hides more of the
+// row_ptr -> col -> x latency chain; measured on the 3D slope operator at
+// 1.49M unknowns: 232 us against 253 us for U = 1, G = 32).  Each row's sum
+// is formed in the same order for a given (G, U).
+template <int DIM, int G, class T, int U = 1>
 __global__ __launch_bounds__(256) void k_spmv_bsr(int64_t n_rows, const int32_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
                                                   const T *__restrict__ x, T *y, double alpha, double beta,
                                                   const T *z)
 {
+    constexpr int DD = DIM * DIM;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t r = gid / G;
     const int l = (int)(gid % G);
@@ -60,16 +66,38 @@ __global__ __launch_bounds__(256) void k_spmv_bsr(int64_t n_rows, const int32_t 
     for (int i = 0; i < DIM; ++i) acc[i] = T(0);
     if (r < n_rows) {
         const int32_t b0 = row_ptr[r], b1 = row_ptr[r + 1];
-        for (int32_t b = b0 + l; b < b1; b += G) {
-            const int64_t c = col[b];
-            T xv[DIM];
+        if (U == 1) {
+            for (int32_t b = b0 + l; b < b1; b += G) {
+                const int64_t c = col[b];
+                T xv[DIM];
 #pragma unroll
-            for (int j = 0; j < DIM; ++j) xv[j] = x[c * DIM + j];
-            const T *v = val + (int64_t)b * DIM * DIM;
+                for (int j = 0; j < DIM; ++j) xv[j] = x[c * DIM + j];
+                const T *v = val + (int64_t)b * DD;
 #pragma unroll
-            for (int i = 0; i < DIM; ++i)
+                for (int i = 0; i < DIM; ++i)
 #pragma unroll
-                for (int j = 0; j < DIM; ++j) acc[i] += v[i * DIM + j] * xv[j];
+                    for (int j = 0; j < DIM; ++j) acc[i] += v[i * DIM + j] * xv[j];
+            }
+        } else {
+            for (int32_t b = b0 + l; b < b1; b += 2 * G) {
+                const int32_t bb = b + G;
+                const bool two = bb < b1;
+                const int64_t c0 = col[b];
+                const int64_t c1 = two ? (int64_t)col[bb] : c0;
+                T v0[DD], v1[DD], x0[DIM], x1[DIM];
+#pragma unroll
+                for (int e = 0; e < DD; ++e) v0[e] = val[(int64_t)b * DD + e];
+#pragma unroll
+                for (int e = 0; e < DD; ++e) v1[e] = two ? val[(int64_t)bb * DD + e] : T(0);
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) x0[j] = x[c0 * DIM + j];
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) x1[j] = x[c1 * DIM + j];
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) acc[i] += v0[i * DIM + j] * x0[j] + v1[i * DIM + j] * x1[j];
+            }
         }
     }
 #pragma unroll
@@ -718,11 +746,25 @@ int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
     if (n_rows == 0) return SSR_OK;
     if (!z) z = y;
     int G = ssr_tuning(SSR_TUNE_SPMV_GROUP);
-    if (G == 0) G = dim == 3 ? 32 : 4;  // measured best for P2 node-block rows (about 28 blocks in 3D, 11 in 2D)
+    int U = ssr_tuning(SSR_TUNE_SPMV_UNROLL);
+    if (G == 0) {
+        // measured best for P2 node-block rows (3D: 27 or 65 blocks, 28 on
+        // average; 2D: about 11)
+        G = dim == 3 ? 16 : 4;
+        if (U == 0) U = dim == 3 ? 2 : 1;
+    }
+    if (U == 0) U = 1;
     const int64_t threads = n_rows * G;
     const dim3 grid(ssr_blocks(threads, 256)), blk(256);
-#define SPMV(D_, G_) hipLaunchKernelGGL((k_spmv_bsr<D_, G_, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, \
-                                        val, x, y, alpha, beta, z)
+#define SPMV(D_, G_)                                                                                          \
+    do {                                                                                                      \
+        if (U == 2)                                                                                           \
+            hipLaunchKernelGGL((k_spmv_bsr<D_, G_, T, 2>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, x, \
+                               y, alpha, beta, z);                                                            \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_spmv_bsr<D_, G_, T, 1>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, x, \
+                               y, alpha, beta, z);                                                            \
+    } while (0)
     if (G == 64) {
         const dim3 gw(ssr_blocks(n_rows * 64, 256));
         if (dim == 3)

@@ -263,7 +263,6 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
         tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
         tm[2] += lap.take();
-        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 0));
         NCHECK(ssr_solver_A_orthogonalize(solver));
         tm[1] += lap.take();
         if (h) {
@@ -384,7 +383,6 @@ extern "C" int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int met
         tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
         tm[2] += lap.take();
-        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 0));
         NCHECK(ssr_solver_A_orthogonalize(solver));
         tm[1] += lap.take();
         if (h) {
@@ -490,7 +488,6 @@ extern "C" int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, 
         tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
         tm[2] += lap.take();
-        NCHECK(ssr_solver_set_matrix(solver, C->K_r, 0));
         NCHECK(ssr_solver_A_orthogonalize(solver));
         tm[1] += lap.take();
         if (h) {

@@ -57,14 +57,16 @@ template <class T>
 struct DevBuf {
     T *p = nullptr;
     int64_t n = 0;
-    ~DevBuf() { (void)hipFree(p); }
+    ~DevBuf() { ssr_pool_free(p); }
     int alloc(int64_t len)
     {
         if (p && n >= len) return SSR_OK;
-        (void)hipFree(p);
+        ssr_pool_free(p);
         p = nullptr;
+        n = 0;
+        const int rc = ssr_pool_malloc((void **)&p, sizeof(T) * (len > 0 ? len : 1));
+        if (rc) return rc;
         n = len;
-        SSR_HIP_CHECK(hipMalloc((void **)&p, sizeof(T) * (len > 0 ? len : 1)));
         return SSR_OK;
     }
 };
@@ -524,10 +526,11 @@ int ssr_solver_expand_deflation(ssr_solver *s, const double *v)
     if (s->kW == s->capW) {
         const int cap = s->capW ? 2 * s->capW : 16;
         double *p = nullptr;
-        SSR_HIP_CHECK(hipMalloc((void **)&p, sizeof(double) * s->n * cap));
+        int rc = ssr_pool_malloc((void **)&p, sizeof(double) * s->n * cap);
+        if (rc) return rc;
         if (s->kW) SSR_HIP_CHECK(hipMemcpyAsync(p, s->W.p, sizeof(double) * s->n * s->kW, hipMemcpyDeviceToDevice, ssr_stream()));
         SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
-        (void)hipFree(s->W.p);
+        ssr_pool_free(s->W.p);
         s->W.p = p;
         s->W.n = s->n * cap;
         s->capW = cap;
@@ -549,10 +552,18 @@ int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src)
     dst->kW = 0;
     dst->kW32 = -1;
     dst->kAW = -1;
-    for (int c = 0; c < src->kW; ++c) {
-        int rc = ssr_solver_expand_deflation(dst, src->W.p + (int64_t)c * src->n);
+    if (src->kW == 0) return SSR_OK;
+    // one allocation at the source's capacity and one copy (the source's
+    // columns are already zero at the Dirichlet DOFs, as expand leaves them)
+    if (dst->capW < src->kW) {
+        const int cap = src->capW > 16 ? src->capW : 16;
+        int rc = dst->W.alloc(dst->n * (int64_t)cap);
         if (rc) return rc;
+        dst->capW = cap;
     }
+    SSR_HIP_CHECK(hipMemcpyAsync(dst->W.p, src->W.p, sizeof(double) * src->n * src->kW, hipMemcpyDeviceToDevice,
+                                 ssr_stream()));
+    dst->kW = src->kW;
     return SSR_OK;
 }
 

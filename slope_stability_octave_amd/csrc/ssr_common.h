@@ -39,6 +39,11 @@ static inline unsigned ssr_blocks(int64_t n, int threads)
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Cached device buffers (devpool.cpp): blocks freed here are kept for later
+// requests of a similar size instead of being returned to the runtime.
+int ssr_pool_malloc(void **out, size_t bytes);
+void ssr_pool_free(void *p);
+
 hipStream_t ssr_stream(void);
 // divert ssr_stream() of the calling thread to `s` while capturing a graph
 void ssr_capture_stream_push(hipStream_t s);
@@ -69,5 +74,6 @@ struct ssr_precond {
                                            deterministic gather per block (per node);
                                            1: FP64 atomic adds of the element contributions */
 #define SSR_TUNE_SPMM_GROUP 6           /* lanes per block row of the multi-vector SpMV (0: by dimension) */
+#define SSR_TUNE_SPMV_UNROLL 7          /* blocks per lane per pass of the node-block SpMV: 1 or 2 (0: by dimension) */
 #define SSR_TUNE_COUNT 8
 int ssr_tuning(int key);

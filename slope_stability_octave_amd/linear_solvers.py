@@ -148,6 +148,7 @@ class DFGMRES:
     def setup_preconditioner(self, K):
         t0 = time.perf_counter()
         self._K = K
+        self._fresh = True  # the operator was just handed over (A_orthogonalize need not again)
         call("ssr_solver_set_matrix", self.handle, ptr(K), 1)
         self._t("DFGMRES.update_preconditioner", t0)
 
@@ -163,6 +164,7 @@ class DFGMRES:
 
     def set_operator(self, K):
         self._K = K
+        self._fresh = False
         call("ssr_solver_set_matrix", self.handle, ptr(K), 0)
 
     # -- deflation ------------------------------------------------------------
@@ -177,7 +179,9 @@ class DFGMRES:
 
     def A_orthogonalize(self, K):
         t0 = time.perf_counter()
-        self.set_operator(K)
+        if not (K is self._K and getattr(self, "_fresh", False)):
+            self.set_operator(K)
+        self._fresh = False
         call("ssr_solver_A_orthogonalize", self.handle)
         self._t("DFGMRES.A_orthogonalize", t0)
         if self.counters is not None:
@@ -217,6 +221,7 @@ class DFGMRES:
         t0 = time.perf_counter()
         if K is not self._K:
             self.set_operator(K)
+        self._fresh = False
         if x is None:
             x = DeviceArray(b.shape)
         it = ctypes.c_int(0)
