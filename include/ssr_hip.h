@@ -120,7 +120,9 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         a deterministic per-block / per-node gather, default; 1 = FP64 atomics);
  *  key 6: lanes per block row of the multi-vector SpMV (0 = by dimension);
  *  key 7: blocks per lane per pass of the node-block SpMV (0 = by dimension:
- *         2 in 3D with 16 lanes per row, 1 in 2D; or 1, 2). */
+ *         2 in 3D with 16 lanes per row, 1 in 2D; or 1, 2);
+ *  key 8: element tangent kernel (0 = per-lane products from LDS-staged point
+ *         data, default; 1 = D*B shared through LDS per point). */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 
@@ -169,6 +171,14 @@ int ssr_dev_force(const ssr_pattern *P, int n_q, const double *DPhi1, const doub
                   const double *DPhi3, const double *S, int s_rows, const double *WEIGHT, double *F);
 /* K_r = r K_el + (1-r) K_t, (K_r + K_r')/2, Dirichlet masked (CONSTITUTIVE.m:515-532). */
 int ssr_dev_regularize(const ssr_pattern *P, double r, const double *Kel, const double *Kt, double *Kr);
+/* K_r straight from DS: the tangent assembly and the regularisation of
+ * CONSTITUTIVE.build_K_r_QQ_vals (CONSTITUTIVE.m:487-532) in one call, K_t never
+ * stored; equal bit for bit to ssr_dev_assemble_bsr + ssr_dev_regularize.
+ * K_elast is the regularised elastic operator (as ssr_dev_regularize with r = 1
+ * leaves it), K_r must not alias it. */
+int ssr_dev_assemble_K_r(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                         const double *DPhi3, const double *DS, int ds_layout, const double *WEIGHT, double r,
+                         const double *K_elast, double *K_r);
 /* block values -> V in the reference's (ref_I, ref_J) order (CONSTITUTIVE.m:445). */
 int ssr_dev_extract_ref(const ssr_pattern *P, const double *Kvals, double *Vref);
 /* DS_elast = 2 DEV shear + VOL bulk per point (CONSTITUTIVE.m:413-422), MEX layout. */
@@ -275,7 +285,7 @@ typedef struct ssr_constitutive {
     double *c_bar, *sin_phi;                  /* n_int, reduced material */
     double *E, *S, *DS;                       /* n_strain, s_rows, n_strain^2 per point */
     const double *K_elast;                    /* block values of the elastic operator */
-    double *K_tang, *K_r;                     /* block values, overwritten */
+    double *K_tang, *K_r;                     /* block values, overwritten (K_tang: unused by the loops) */
     const double *mask;                       /* dim*n_n: 1 at free DOFs, 0 at Dirichlet DOFs */
     const double *f;                          /* dim*n_n load vector */
 } ssr_constitutive;

@@ -199,6 +199,7 @@ TUNE_AMG_BLOCK_SMOOTH = 4
 TUNE_ASSEMBLY_FORM = 5
 TUNE_SPMM_GROUP = 6
 TUNE_SPMV_UNROLL = 7
+TUNE_ELEMENT_KERNEL = 8
 
 
 def set_tuning(key: int, value: int):

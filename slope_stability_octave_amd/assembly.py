@@ -35,6 +35,7 @@ _lib.extend_signatures({
     "ssr_dev_strain": [_vp, _int, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_force": [_vp, _int, _vp, _vp, _vp, _vp, _int, _vp, _vp],
     "ssr_dev_regularize": [_vp, _dbl, _vp, _vp, _vp],
+    "ssr_dev_assemble_K_r": [_vp, _int, _vp, _vp, _vp, _vp, _int, _vp, _dbl, _vp, _vp],
     "ssr_dev_extract_ref": [_vp, _vp, _vp],
     "ssr_dev_ds_elastic": [_int, _i64, _vp, _vp, _vp],
 })
@@ -166,6 +167,12 @@ class Pattern:
 
     def regularize(self, r, Kel, Kt, Kr):
         call("ssr_dev_regularize", self.handle, float(r), ptr(Kel), ptr(Kt), ptr(Kr))
+
+    def assemble_K_r(self, n_q, DPhi, DS, WEIGHT, r, Kel, Kr, ds_layout=0):
+        """Kr = regularised tangent straight from DS (assemble + regularize in one call)."""
+        call("ssr_dev_assemble_K_r", self.handle, int(n_q), ptr(DPhi[0]), ptr(DPhi[1]),
+             ptr(DPhi[2]) if self.dim == 3 else _vp(0), ptr(DS), int(ds_layout), ptr(WEIGHT), float(r), ptr(Kel),
+             ptr(Kr))
 
     def extract_ref(self, Kvals, Vref):
         call("ssr_dev_extract_ref", self.handle, ptr(Kvals), ptr(Vref))

@@ -164,9 +164,12 @@ class CONSTITUTIVE:
 
     def build_K_r_QQ_vals(self, r):
         """K_r = r K_elast + (1-r) K_tang, symmetrised, Dirichlet-masked; the
-        device block values (the pattern is fixed)."""
-        self.build_K_tangent_QQ_vals()
-        self.pattern.regularize(r, self.K_elast, self.K_tang, self.K_r)
+        device block values (the pattern is fixed).  One call: the tangent is
+        not stored on the way."""
+        t0 = time.perf_counter()
+        P = self.pattern_or_raise()
+        P.assemble_K_r(self.n_q, self.DPhi, self.DS, self.WEIGHT, r, self.K_elast, self.K_r)
+        self._t("CONSTITUTIVE.build_K_r_QQ_vals", t0)
         return self.K_r
 
     def build_K_r_QQ_vals_ref(self, r):

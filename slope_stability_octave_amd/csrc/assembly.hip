@@ -26,6 +26,8 @@ const int32_t *ssr_pattern_d_row_ptr(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_col(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_emap(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_ublk(const ssr_pattern *p);
+int64_t ssr_pattern_n_ublk(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_elem(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_brow(const ssr_pattern *p);
@@ -220,6 +222,161 @@ __global__ __launch_bounds__(64) void k_assemble(
     }
 }
 
+// Element pair blocks without per-point barriers: the element's gradients and
+// w*D of all its points are staged in LDS once (4.5 kB for a P2
+// tetrahedron), then every lane forms its own D*B_b (54 FMA per point in 3D)
+// and B_a'(D B_b) (27 FMA) straight from LDS; the w*D reads are the same
+// address for every lane of the element (broadcast).  The k_assemble form
+// shares D*B_b between the lanes of one node through LDS instead, at the cost
+// of two barriers and its LDS round trip per point.  The products are the
+// same expressions in the same order.
+template <int DIM, int NP, int NQ, int DSL>
+__global__ __launch_bounds__(64) void k_assemble_pairs(int64_t n_e, const double *__restrict__ D1,
+                                                       const double *__restrict__ D2, const double *__restrict__ D3,
+                                                       const double *__restrict__ DS, const double *__restrict__ W,
+                                                       double *__restrict__ out)
+{
+    using V = Voigt<DIM>;
+    constexpr int NS = V::NS, NSYM = NS * (NS + 1) / 2, NPAIR = NP * (NP + 1) / 2;
+    constexpr int EPB = 64 / NPAIR;
+    __shared__ double sG[EPB][NQ][NP][DIM];
+    __shared__ double sD[EPB][NQ][NSYM];
+    const int lane = threadIdx.x;
+    const int le = lane / NPAIR;
+    const int pr = lane - le * NPAIR;
+    int pa = 0, pb = 0;
+    {
+        int k = pr, a = 0;
+        while (k >= NP - a) { k -= NP - a; ++a; }
+        pa = a; pb = a + k;
+    }
+    const bool active = le < EPB;
+    for (int64_t e0 = (int64_t)blockIdx.x * EPB; e0 < n_e; e0 += (int64_t)gridDim.x * EPB) {
+        for (int t = lane; t < EPB * NQ * NP; t += 64) {
+            const int s = t / (NQ * NP), r = t - s * NQ * NP;
+            const int64_t e = e0 + s;
+            if (e < n_e) {
+                const int64_t g = e * NQ * NP + r;
+                const int q = r / NP, a = r - q * NP;
+                sG[s][q][a][0] = D1[g];
+                sG[s][q][a][1] = D2[g];
+                if (DIM == 3) sG[s][q][a][DIM - 1] = D3[g];
+            }
+        }
+        for (int t = lane; t < EPB * NQ * NSYM; t += 64) {
+            const int s = t / (NQ * NSYM), r = t - s * NQ * NSYM;
+            const int64_t e = e0 + s;
+            if (e < n_e) {
+                const int q = r / NSYM, k = r - q * NSYM;
+                int i = 0, kk = k;
+                while (kk >= NS - i) { kk -= NS - i; ++i; }
+                const int j = i + kk;
+                const int64_t g = e * NQ + q;
+                const double w = W[g];
+                const double d = DSL == 0 ? DS[g * NS * NS + i + NS * j] : DS[g * NSYM + k];
+                sD[s][q][k] = w * d;
+            }
+        }
+        __syncthreads();
+        const int64_t e = e0 + le;
+        if (active && e < n_e) {
+            double acc[DIM][DIM];
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) acc[i][j] = 0.0;
+            for (int q = 0; q < NQ; ++q) {
+                double ga[DIM], gb[DIM];
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    ga[c] = sG[le][q][pa][c];
+                    gb[c] = sG[le][q][pb][c];
+                }
+                // t = (wD) B_b, NS x DIM
+                double t[NS][DIM];
+#pragma unroll
+                for (int row = 0; row < NS; ++row)
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < V::NNZ; ++k) v += sD[le][q][symidx(row, V::row(c, k), NS)] * gb[V::grad(c, k)];
+                        t[row][c] = v;
+                    }
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < V::NNZ; ++k) v += ga[V::grad(i, k)] * t[V::row(i, k)][j];
+                        acc[i][j] += v;
+                    }
+            }
+            double *o = out + ((int64_t)e * NPAIR + pr) * DIM * DIM;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) o[i * DIM + j] = acc[i][j];
+        }
+        __syncthreads();
+    }
+}
+
+// K_r = r K_el + (1-r) sum_e Ke, symmetrised and Dirichlet-masked, in one pass
+// over the blocks on and above the block diagonal (CONSTITUTIVE.m:515-532):
+// each such block b = (v, w) sums its element contributions once and writes
+// both b and, transposed, its mirror tb = (w, v).  The mirror's sum is the
+// same sum bit for bit (the contribution lists of b and tb hold the same
+// element pair blocks in the same element order, b's as is where tb's are
+// transposed), so the result equals the two-pass path (gather, then
+// k_regularize) exactly, and neither K_t nor the transposed reads of it exist.
+// Diagonal blocks (tb == b) also form the transposed entry's sum.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_gather_regularize(int64_t n_ub, const int32_t *__restrict__ ublk,
+                                                           const int32_t *__restrict__ brow,
+                                                           const int32_t *__restrict__ col,
+                                                           const int32_t *__restrict__ tblk,
+                                                           const int32_t *__restrict__ cptr,
+                                                           const int32_t *__restrict__ code,
+                                                           const double *__restrict__ ke,
+                                                           const double *__restrict__ mask, double r,
+                                                           const double *__restrict__ Kel, double *__restrict__ Kr)
+{
+    constexpr int DD = DIM * DIM;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t u = t / DD;
+    if (u >= n_ub) return;
+    const int q = (int)(t - u * DD), i = q / DIM, j = q - i * DIM, qt = j * DIM + i;
+    const int64_t b = ublk[u];
+    const int64_t tb = tblk[b];
+    const int32_t c0 = cptr[b], c1 = cptr[b + 1];
+    double s = 0.0;
+    for (int32_t c = c0; c < c1; ++c) {
+        const int32_t cd = code[c];
+        s += ke[(int64_t)(cd >> 1) * DD + ((cd & 1) ? qt : q)];
+    }
+    double st = s;
+    if (tb == b) {
+        st = 0.0;
+        for (int32_t c = c0; c < c1; ++c) {
+            const int32_t cd = code[c];
+            st += ke[(int64_t)(cd >> 1) * DD + ((cd & 1) ? q : qt)];
+        }
+    }
+    double v;
+    {
+#pragma clang fp contract(off)
+        const double v1 = r * Kel[b * DD + q] + (1.0 - r) * s;
+        const double v2 = r * Kel[tb * DD + qt] + (1.0 - r) * st;
+        v = 0.5 * (v1 + v2);
+    }
+    const int64_t ro = (int64_t)brow[b] * DIM + i, co = (int64_t)col[b] * DIM + j;
+    if (mask[ro] == 0.0 || mask[co] == 0.0) v = (ro == co) ? 1.0 : 0.0;
+    Kr[b * DD + q] = v;
+    if (tb != b) Kr[tb * DD + qt] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Strain gather: one thread per integration point.  U is dim x n_n
 // (node-major), E is point-major (ns per point).
@@ -337,9 +494,15 @@ __global__ void k_regularize(int64_t nnzb, const int32_t *__restrict__ brow,
     const int ij = (int)(t - b * BS), i = ij / DIM, j = ij - (ij / DIM) * DIM;
     const int64_t bt = tblk[b];
     const int64_t tt = bt * BS + j * DIM + i;
-    const double v1 = r * Kel[t] + (1.0 - r) * Kt[t];
-    const double v2 = r * Kel[tt] + (1.0 - r) * Kt[tt];
-    double v = 0.5 * (v1 + v2);
+    double v;
+    {
+        // no FMA contraction: the same rounding as the reference's
+        // r*V_elast + (1-r)*V_tang (CONSTITUTIVE.m:515-532) and as k_gather_regularize
+#pragma clang fp contract(off)
+        const double v1 = r * Kel[t] + (1.0 - r) * Kt[t];
+        const double v2 = r * Kel[tt] + (1.0 - r) * Kt[tt];
+        v = 0.5 * (v1 + v2);
+    }
     const int64_t ro = (int64_t)brow[b] * DIM + i, co = (int64_t)col[b] * DIM + j;
     if (mask[ro] == 0.0 || mask[co] == 0.0) v = (ro == co) ? 1.0 : 0.0;
     Kr[t] = v;
@@ -409,7 +572,12 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         const unsigned grid = (unsigned)std::min<int64_t>(n_e, want * 16);
 #define L3(DSL_, SC_) hipLaunchKernelGGL((k_assemble<3, 10, 11, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
-        if (scat == 2) {
+#define P3(DSL_) hipLaunchKernelGGL((k_assemble_pairs<3, 10, 11, DSL_>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, \
+                                    D1, D2, D3, DS, W, out)
+        if (scat == 2 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0) {
+            if (dsl == 0) P3(0);
+            else P3(1);
+        } else if (scat == 2) {
             if (dsl == 0) L3(0, 2);
             else L3(1, 2);
         } else if (dsl == 0 && scat == 0) L3(0, 0);
@@ -417,6 +585,7 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         else if (scat == 0) L3(1, 0);
         else L3(1, 1);
 #undef L3
+#undef P3
     } else {
         SSR_REQUIRE(n_p == 6 && n_q == 7, "tangent assembly (2D): P2 triangles with 7 points supported (got n_p=%lld, n_q=%d)",
                     (long long)n_p, n_q);
@@ -424,7 +593,12 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         const unsigned grid = (unsigned)std::min<int64_t>(groups, want * 16);
 #define L2(DSL_, SC_) hipLaunchKernelGGL((k_assemble<2, 6, 7, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
-        if (scat == 2) {
+#define P2(DSL_) hipLaunchKernelGGL((k_assemble_pairs<2, 6, 7, DSL_>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, \
+                                    D1, D2, D3, DS, W, out)
+        if (scat == 2 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0) {
+            if (dsl == 0) P2(0);
+            else P2(1);
+        } else if (scat == 2) {
             if (dsl == 0) L2(0, 2);
             else L2(1, 2);
         } else if (dsl == 0 && scat == 0) L2(0, 0);
@@ -432,6 +606,7 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         else if (scat == 0) L2(1, 0);
         else L2(1, 1);
 #undef L2
+#undef P2
     }
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
@@ -569,6 +744,49 @@ int ssr_dev_regularize(const ssr_pattern *P, double r, const double *Kel, const 
     else
         hipLaunchKernelGGL(k_regularize<2>, dim3(ssr_blocks(n, 256)), dim3(256), 0, ssr_stream(), nnzb, brow,
                            ssr_pattern_d_col(P), ssr_pattern_d_tblk(P), ssr_pattern_d_mask(P), r, Kel, Kt, Kr);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+// K_r = r K_elast + (1-r) sum_e B'(w DS)B, symmetrised and Dirichlet-masked,
+// written once (CONSTITUTIVE.build_K_r_QQ_vals, CONSTITUTIVE.m:487-532):
+// element pair blocks, then one gather-and-regularise pass over the upper
+// blocks.  Equal bit for bit to ssr_dev_assemble_bsr followed by
+// ssr_dev_regularize in the gather form; the atomic form (tuning key 5) and
+// patterns without contribution lists take that two-pass path.
+int ssr_dev_assemble_K_r(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
+                         const double *DPhi3, const double *DS, int ds_layout, const double *WEIGHT, double r,
+                         const double *K_elast, double *K_r)
+{
+    SSR_REQUIRE(P && K_elast && K_r && K_elast != K_r, "assemble_K_r: bad arguments");
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    SSR_REQUIRE(ssr_pattern_d_emap(P), "assemble_K_r: pattern has no device tables");
+    int rc;
+    if (ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) != 0 || !ssr_pattern_d_cptr(P) || !ssr_pattern_d_ublk(P)) {
+        double *Kt = nullptr;
+        if ((rc = ssr_pool_malloc((void **)&Kt, sizeof(double) * nnzb * dim * dim))) return rc;
+        rc = ssr_dev_assemble_bsr(P, n_q, DPhi1, DPhi2, DPhi3, DS, ds_layout, WEIGHT, Kt);
+        if (!rc) rc = ssr_dev_regularize(P, r, K_elast, Kt, K_r);
+        ssr_pool_free(Kt);
+        return rc;
+    }
+    double *ke = nullptr;
+    if ((rc = ssr_pattern_keblk(P, &ke))) return rc;
+    if ((rc = assemble_launch(dim, n_p, n_e, n_q, ds_layout, 2, DPhi1, DPhi2, DPhi3, DS, WEIGHT, nullptr, nullptr,
+                              ke)))
+        return rc;
+    const int64_t n_ub = ssr_pattern_n_ublk(P);
+    const int64_t threads = n_ub * dim * dim;
+    if (threads == 0) return SSR_OK;
+#define GR(D_)                                                                                                    \
+    hipLaunchKernelGGL(k_gather_regularize<D_>, dim3(ssr_blocks(threads, 256)), dim3(256), 0, ssr_stream(), n_ub, \
+                       ssr_pattern_d_ublk(P), ssr_pattern_d_brow(P), ssr_pattern_d_col(P), ssr_pattern_d_tblk(P),   \
+                       ssr_pattern_d_cptr(P), ssr_pattern_d_ccode(P), ke, ssr_pattern_d_mask(P), r, K_elast, K_r)
+    if (dim == 3) GR(3);
+    else GR(2);
+#undef GR
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

@@ -253,9 +253,8 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
         }
         if (h) h[1] = r;
         // K_r (CONSTITUTIVE.m:487-532)
-        NCHECK(ssr_dev_assemble_bsr(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT,
-                                    C->K_tang));
-        NCHECK(ssr_dev_regularize(C->pattern, r, C->K_elast, C->K_tang, C->K_r));
+        NCHECK(ssr_dev_assemble_K_r(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT, r,
+                                    C->K_elast, C->K_r));
         // G = (F(lambda + eps) - F) / eps   (:94-96)
         NCHECK(build_F(x, lambda_it + eps, U_it, x.F_eps, 0));
         NCHECK(ssr_dev_lincomb(n_dof, 1.0 / eps, x.F_eps, -1.0 / eps, x.F, x.G));
@@ -377,9 +376,8 @@ extern "C" int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int met
             h[0] = rel_resid;
         }
         if (h) h[1] = r;
-        NCHECK(ssr_dev_assemble_bsr(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT,
-                                    C->K_tang));
-        NCHECK(ssr_dev_regularize(C->pattern, r, C->K_elast, C->K_tang, C->K_r));
+        NCHECK(ssr_dev_assemble_K_r(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT, r,
+                                    C->K_elast, C->K_r));
         tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
         tm[2] += lap.take();
@@ -482,9 +480,8 @@ extern "C" int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, 
             break;
         }
         if (h) h[1] = r;
-        NCHECK(ssr_dev_assemble_bsr(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT,
-                                    C->K_tang));
-        NCHECK(ssr_dev_regularize(C->pattern, r, C->K_elast, C->K_tang, C->K_r));
+        NCHECK(ssr_dev_assemble_K_r(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT, r,
+                                    C->K_elast, C->K_r));
         tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
         tm[2] += lap.take();

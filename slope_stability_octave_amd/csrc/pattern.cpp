@@ -32,11 +32,12 @@ struct ssr_pattern {
     int64_t nnz_free = 0;  // entries of K(Q,Q)
     // host copies
     std::vector<int32_t> row_ptr, col, emap, tblk, diag, brow;
+    std::vector<int32_t> ublk;  // blocks on or above the block diagonal (row <= column), row-major order
     std::vector<int64_t> ref_pos;
     std::vector<uint8_t> mask;  // dim*n_n, 1 = free
     // device copies
     int32_t *d_row_ptr = nullptr, *d_col = nullptr, *d_emap = nullptr, *d_tblk = nullptr,
-            *d_diag = nullptr, *d_elem = nullptr, *d_brow = nullptr;
+            *d_diag = nullptr, *d_elem = nullptr, *d_brow = nullptr, *d_ublk = nullptr;
     int64_t *d_ref_pos = nullptr;
     double *d_mask = nullptr;  // dim*n_n, 1.0 free / 0.0 Dirichlet
     // gather-form assembly: the element blocks that sum into each block, in
@@ -84,6 +85,7 @@ int ssr_pattern_destroy(ssr_pattern *p)
     (void)hipFree(p->d_ncode);
     (void)hipFree(p->d_fe);
     (void)hipFree(p->d_tblk);
+    (void)hipFree(p->d_ublk);
     (void)hipFree(p->d_diag);
     (void)hipFree(p->d_elem);
     (void)hipFree(p->d_brow);
@@ -159,6 +161,10 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
     p->tblk.resize(p->nnzb);
     for (int64_t r = 0; r < n_n; ++r)
         for (int32_t k = p->row_ptr[r]; k < p->row_ptr[r + 1]; ++k) p->tblk[k] = find_blk(p->col[k], (int32_t)r);
+    p->ublk.clear();
+    for (int64_t r = 0; r < n_n; ++r)
+        for (int32_t k = p->row_ptr[r]; k < p->row_ptr[r + 1]; ++k)
+            if (p->col[k] >= r) p->ublk.push_back(k);
     p->emap.resize(n_e * n_p * n_p);
     for (int64_t e = 0; e < n_e; ++e)
         for (int64_t a = 0; a < n_p; ++a)
@@ -226,6 +232,7 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
         std::vector<double> md(p->mask.begin(), p->mask.end());
         if ((rc = upload(&p->d_row_ptr, p->row_ptr)) || (rc = upload(&p->d_col, p->col)) ||
             (rc = upload(&p->d_emap, p->emap)) || (rc = upload(&p->d_tblk, p->tblk)) ||
+            (rc = upload(&p->d_ublk, p->ublk)) ||
             (rc = upload(&p->d_diag, p->diag)) || (rc = upload(&p->d_elem, el32)) ||
             (rc = upload(&p->d_ref_pos, p->ref_pos)) || (rc = upload(&p->d_mask, md)) ||
             (rc = upload(&p->d_brow, p->brow)) ||
@@ -279,6 +286,8 @@ const int32_t *ssr_pattern_d_row_ptr(const ssr_pattern *p) { return p->d_row_ptr
 const int32_t *ssr_pattern_d_col(const ssr_pattern *p) { return p->d_col; }
 const int32_t *ssr_pattern_d_emap(const ssr_pattern *p) { return p->d_emap; }
 const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p) { return p->d_tblk; }
+const int32_t *ssr_pattern_d_ublk(const ssr_pattern *p) { return p->d_ublk; }
+int64_t ssr_pattern_n_ublk(const ssr_pattern *p) { return (int64_t)p->ublk.size(); }
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p) { return p->d_diag; }
 const int32_t *ssr_pattern_d_elem(const ssr_pattern *p) { return p->d_elem; }
 const int32_t *ssr_pattern_d_brow(const ssr_pattern *p) { return p->d_brow; }

@@ -171,3 +171,50 @@ def test_strain_and_force(dim):
     for F in (out[0][0], out[1][0]):
         assert np.abs(F - F0).max() <= 1e-12 * np.abs(F0).max()
     assert np.array_equal(out[0][0], out[0][1])
+
+
+@pytest.mark.parametrize("dim,size", [(3, "small"), (2, "small"), (3, "medium"), (2, "medium")])
+def test_fused_K_r_equals_two_pass_path(dim, size):
+    """ssr_dev_assemble_K_r (element pair blocks, then one gather-and-
+    regularise pass over the upper blocks writing each block and its mirror)
+    against assemble + regularize: the same bits, for both element-kernel
+    forms; the two element kernels agree to rounding, and K_r matches the
+    oracle's regularised tangent at 1e-11."""
+    from slope_stability_octave_amd import _lib
+    p = problem(dim, size, seed=5)
+    P = A.Pattern(p["elem"], p["Q"], dim)
+    dD = [DeviceArray.from_host(x.ravel(order="F")) for x in p["DPhi"]]
+    dW = DeviceArray.from_host(p["WEIGHT"])
+    dDS = DeviceArray.from_host(np.asfortranarray(p["DS"]).T.copy())
+    n_int = p["WEIGHT"].size
+    ns = 6 if dim == 3 else 3
+    dDSe = DeviceArray((n_int, ns * ns))
+    A.ds_elastic_dev(dim, n_int, DeviceArray.from_host(p["shear"]), DeviceArray.from_host(p["bulk"]), dDSe)
+    Kraw = DeviceArray(P.nnzb * P.bs)
+    P.assemble(p["n_q"], dD, dDSe, dW, Kraw)
+    Kel = DeviceArray(P.nnzb * P.bs)
+    P.regularize(1.0, Kraw, Kraw, Kel)
+    r = 0.21
+    old = _lib.get_tuning(_lib.TUNE_ELEMENT_KERNEL)
+    res = {}
+    try:
+        for form in (0, 1):
+            _lib.set_tuning(_lib.TUNE_ELEMENT_KERNEL, form)
+            Kt = DeviceArray(P.nnzb * P.bs)
+            P.assemble(p["n_q"], dD, dDS, dW, Kt)
+            K2 = DeviceArray(P.nnzb * P.bs)
+            P.regularize(r, Kel, Kt, K2)
+            K1 = DeviceArray(P.nnzb * P.bs)
+            K1.set(np.full(P.nnzb * P.bs, np.nan))  # every entry must be written
+            P.assemble_K_r(p["n_q"], dD, dDS, dW, r, Kel, K1)
+            a, b = K1.get(), K2.get()
+            assert np.array_equal(a, b), (form, np.abs(a - b).max())
+            res[form] = (Kt.get(), a)
+    finally:
+        _lib.set_tuning(_lib.TUNE_ELEMENT_KERNEL, old)
+    t0, t1 = res[0][0], res[1][0]
+    assert np.abs(t0 - t1).max() <= 1e-14 * np.abs(t1).max()
+    Vr = DeviceArray(P.nnz_free)
+    P.extract_ref(DeviceArray.from_host(res[0][1]), Vr)
+    Vr0 = p["R"].build_K_r(r, p["R"].global_tangent(p["DS"], p["WEIGHT"]))
+    assert relK(Vr.get(), Vr0) <= TOL_K
