@@ -18,6 +18,13 @@ Roofline: 424 algorithmic bytes per point (E 48 + 5 x 8 material + S 48 +
 DS 288), HBM peak 8 TB/s.  CPU baseline: the reference's own per-point kernel
 compiled from its sources (oracle/_ref) with OpenMP, timed on a bounded
 sample on this host.
+
+The metric's time-to-FoS half is the extra field `time_to_fos`: the ~1.5M-DOF
+3D homogeneous slope solved to its factor of safety in the plain GPU form
+(FP64) and in the chosen form (FP32 Krylov/AMG with FP64 refinement), with
+the FoS of each, |dFoS|, the time cut and the gate flag (cut >= 30 % at
+|dFoS| <= 1e-4), and the 2D plain-method configuration end to end on the
+GPU and on this host's CPU cores (`config1_end_to_end`).
 """
 from __future__ import annotations
 
@@ -66,6 +73,97 @@ def cpu_baseline(d, cb, sp, budget_s=12.0, sample=1_320_000):
         "sample": f"{reps} x {n} points (first {n} of the {d['E'].shape[0]}-point workload), "
                   f"S+DS, OpenMP static schedule, {el:.1f} s",
     }
+
+
+CONFIG3_CPM = 1.3  # cells per metre: 1,492,582 free unknowns (the ~1.5M-DOF configuration)
+GEOM_3D = (20.0, 20.0, 20.0, 10.0, 20.0, 20.0)
+
+
+def _fos_summary(res):
+    its = int(res["newton_iterations"])
+    krylov = int(res["counts"].get("DFGMRES.solve.n_iters", 0))
+    return {"time_to_fos_s": res["time_to_fos_s"], "FoS": res["FoS"], "steps": len(res["lambda_hist"]),
+            "newton_iterations": its, "krylov_iterations": krylov,
+            "ms_per_newton_iteration": 1e3 * res["time_to_fos_s"] / max(its, 1),
+            "lambda_hist": [float(x) for x in res["lambda_hist"]],
+            "omega_hist": [float(x) for x in res["omega_hist"]]}
+
+
+def time_to_fos_leg(log, cpm=CONFIG3_CPM, lambda_init=0.7, linear_tol=None):
+    """The metric's time-to-FoS half: the 3D homogeneous slope (60x30x20 m
+    box, 20 m high 1:1 slope, structured P2 tetrahedra, ~1.5M free DOF, the
+    material of the 2D slope: c = 6 kPa, phi = 45 deg, E = 40 MPa, nu = 0.3,
+    gamma = 20 kN/m^3, Davis B) solved to its factor of safety by the indirect
+    continuation with DFGMRES + AMG, in the plain GPU form (FP64 throughout,
+    secant predictor) and in the chosen form (FP32 Krylov and AMG with FP64
+    outer-residual refinement; constitutive branches, Newton residual and the
+    continuation in FP64).  The continuation, Newton and Krylov parameters
+    are the reference driver's (slope_stability_3D_homo_SSR.m:118-140: DFGMRES
+    tolerance 1e-1), with lambda_init 0.7 below this mesh's factor of safety.
+    Continuation time only (mesh and pattern setup excluded, reported
+    separately)."""
+    from slope_stability_octave_amd import problems
+
+    counts = tuple(int(round(c * cpm)) for c in (20, 20, 20, 10, 20, 20))
+    t0 = time.perf_counter()
+    prob = problems.slope_3d(counts, *GEOM_3D, problems.HOMO_2D)
+    mesh_s = time.perf_counter() - t0
+    p = dict(problems.PARAMS_3D, lambda_init=lambda_init)
+    if linear_tol is not None:
+        p["linear_tol"] = linear_tol
+    log(f"[bench] time-to-FoS: 3D slope cells {counts}, {prob['n_dof_free']} free DOFs (mesh {mesh_s:.1f} s)")
+    runs = {}
+    for name, prec in (("plain", 64), ("chosen", 32)):
+        res = problems.run_ssr(prob, p, preconditioner="amg", precision=prec)
+        runs[name] = (res, _fos_summary(res))
+        log(f"[bench] {name}: FoS {res['FoS']:.7f}, {runs[name][1]['steps']} steps, "
+            f"{runs[name][1]['newton_iterations']} Newton iterations, {res['time_to_fos_s']:.2f} s")
+    (rp, sp_), (rc, sc) = runs["plain"], runs["chosen"]
+    cut = 1.0 - rc["time_to_fos_s"] / rp["time_to_fos_s"]
+    dfos = abs(rc["FoS"] - rp["FoS"])
+    per_it_cut = 1.0 - sc["ms_per_newton_iteration"] / sp_["ms_per_newton_iteration"]
+    return {
+        "workload": f"3D homogeneous slope SSR, 60x30x20 m box with a 20 m high 1:1 slope, structured P2 "
+                    f"tetrahedra (cells {counts}), {prob['n_dof_free']} free DOFs, {prob['n_int']} points, "
+                    f"lambda_init {lambda_init}, DFGMRES tol {p['linear_tol']} + SA-AMG",
+        "plain": sp_, "chosen": sc,
+        "chosen_form": "FP32 Krylov/AMG with FP64 refinement, secant predictor",
+        "time_cut": cut, "time_cut_per_newton_iteration": per_it_cut,
+        "dFoS": dfos, "curve_max_dlambda": _curve_gap(rp, rc),
+        "gate_pass": bool(cut >= 0.30 and dfos <= 1e-4),
+        "setup_s": {"mesh_and_fe": mesh_s, "pattern_plain": rp["setup_s"], "pattern_chosen": rc["setup_s"]},
+    }
+
+
+def config1_leg(log):
+    """The plain method's CPU-runnable configuration (2D homogeneous slope,
+    P2 h = 0.5, 20,840 free DOF, lambda_init 1) end to end: the GPU path
+    (FP64, DFGMRES tol 1e-2 + SA-AMG) and the CPU restatement of the same
+    Newton/continuation path (oracle/newton_path.py: FP64, OpenMP constitutive
+    and assembly kernels, sparse direct solves) timed on this host."""
+    from oracle import newton_path as NP
+    from oracle import oracle as O
+    from slope_stability_octave_amd import problems
+
+    prob = problems.slope_2d(h=0.5)
+    p = dict(problems.PARAMS_2D, lambda_init=1.0, linear_tol=1e-2)
+    res = problems.run_ssr(prob, p, preconditioner="amg", precision=64)
+    log(f"[bench] config 1 GPU: FoS {res['FoS']:.7f} in {res['time_to_fos_s']:.2f} s")
+    O.build()
+    cpu = NP.ConstitutiveCPU(prob["coord"], prob["elem"], prob["Q"], 2, prob["DPhi"], prob["WEIGHT"], prob["mats"])
+    t0 = time.perf_counter()
+    _, lam, om = NP.SSR_indirect_continuation(
+        p["lambda_init"], p["d_lambda_init"], p["d_lambda_min"], p["d_lambda_diff_scaled_min"], p["step_max"],
+        p["omega_max_stop"], p["it_newt_max"], p["it_damp_max"], p["tol"], p["r_min"], prob["f"], cpu)
+    t_cpu = time.perf_counter() - t0
+    log(f"[bench] config 1 CPU: FoS {lam[-1]:.7f} in {t_cpu:.1f} s")
+    return {"workload": "2D homogeneous slope SSR, P2 h=0.5 (20,840 free DOF), lambda_init 1.0",
+            "gpu": _fos_summary(res),
+            "cpu": {"time_to_fos_s": t_cpu, "FoS": float(lam[-1]), "steps": int(len(lam)),
+                    "cores": O.num_threads(), "kind": "port",
+                    "note": "oracle/newton_path.py: OpenMP C constitutive and assembly kernels on `cores` "
+                            "threads, single-threaded sparse LU per Newton system"},
+            "dFoS": abs(res["FoS"] - float(lam[-1])), "speedup": t_cpu / res["time_to_fos_s"]}
 
 
 def run_fos(args):
@@ -374,6 +472,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-time-to-fos", action="store_true",
+                    help="skip the time-to-FoS leg (plain vs chosen form on the ~1.5M-DOF 3D slope)")
     ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
                              "learned3d", "batch", "ll2d"])
     ap.add_argument("--h", type=float, default=0.5)
@@ -550,6 +650,16 @@ def main():
         print(f"[bench] oracle check skipped: {exc}", file=sys.stderr)
         cpu = None
 
+    ttf = None
+    if not args.no_time_to_fos:
+        del dE, dc0, dphi, dpsi, dlam, dG, dK, dL, dcb, dsp, dS, dD  # free the 13.2M-point arrays
+        ttf = time_to_fos_leg(lambda s: print(s, file=sys.stderr, flush=True))
+        if not args.no_cpu_baseline:
+            try:
+                ttf["config1_end_to_end"] = config1_leg(lambda s: print(s, file=sys.stderr, flush=True))
+            except Exception as exc:  # the checker must not hide the measurement
+                print(f"[bench] config-1 CPU leg skipped: {exc}", file=sys.stderr)
+
     gpts = n / (kern_ms * 1e-3) / 1e9
     achieved = BYTES_PER_POINT * n / (kern_ms * 1e-3) / 1e9
     traffic = None
@@ -596,6 +706,7 @@ def main():
             "hbm_probe": probe,
             "kernel_variant": _lib.get_tuning(_lib.TUNE_CONSTITUTIVE_VARIANT),
             "autotune": tuned,
+            "time_to_fos": ttf,
         },
     }
     print(json.dumps(out))

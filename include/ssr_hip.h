@@ -265,6 +265,34 @@ int ssr_solver_precond_apply(ssr_solver *s, const double *x, double *y);
 /* out[4] = total iterations, solves, preconditioner applications, SpMVs */
 int ssr_solver_stats(const ssr_solver *s, int64_t *out);
 
+/* ---- AMG instances from COO triplets (the BoomerAMG binding's commands) ---- */
+/* info = hypre_boomeramg_mex('setup_ijv', I, J, V, n, Z, opts, instance_id)
+ * replaces +LINEAR_SOLVERS/mex/hypre_boomeramg_mex.cpp:1013-1149 (called by
+ * hypre_boomeramg_setup_ijv.m:23) with this library's smoothed-aggregation AMG.
+ * I, J: 1-based indices as doubles, V values, nnz triplets of an n x n
+ * symmetric matrix (host); Z: n x n_null near-null space, column-major
+ * (NULL with n_null = 0: one constant per function); dofs_per_node: 2 or 3,
+ * consecutive unknowns of one node (num_functions; an n that is not a
+ * multiple is padded with decoupled unknowns); opts: NULL or double[8]
+ * as ssr_solver_set_amg.  An existing id with the same n and nnz takes the
+ * values-only path (:1054-1104) unless force_rebuild.  info (NULL or 5
+ * doubles): n, nnz, num_functions, near-null columns, setup seconds. */
+int ssr_amg_setup_ijv(const char *id, const double *I, const double *J, const double *V, int64_t nnz, int64_t n,
+                      const double *Z, int n_null, int dofs_per_node, const double *opts, int force_rebuild,
+                      double *info);
+/* t = hypre_boomeramg_mex('update_values', V, id): values in the triplet order
+ * of setup_ijv, hierarchy rebuilt; replaces hypre_boomeramg_mex.cpp:1151-1211. */
+int ssr_amg_update_values(const char *id, const double *V, int64_t nnz, double *setup_seconds);
+/* y = hypre_boomeramg_mex('apply', r, id): one V-cycle from x = 0 on host
+ * vectors of length n; replaces hypre_boomeramg_mex.cpp:751-796.  The _dev
+ * form takes device vectors. */
+int ssr_amg_apply(const char *id, const double *r, int64_t n, double *y);
+int ssr_amg_apply_dev(const char *id, const double *r, double *y);
+/* hypre_boomeramg_mex('clear'[, id]): id NULL clears every instance. */
+int ssr_amg_clear(const char *id);
+/* hypre_boomeramg_mex('info'): number of instances and their total size. */
+int ssr_amg_info(int64_t *count, int64_t *n_total);
+
 /* ---- Newton step of the indirect SSR continuation ---------------------------- */
 /* The device state of the CONSTITUTIVE handle class (CONSTITUTIVE.m) that the
  * Newton loop touches.  All pointers are device memory owned by the caller

@@ -438,9 +438,15 @@ __global__ void k_sum_chunks(int nc, int k, const double *__restrict__ part, dou
     if (e >= len) return;
     const int i = (int)(e % k), j = (int)(e / k);
     const int64_t src = (upper_only && i / kTile > j / kTile) ? (int64_t)i * k + j : e;
-    double s = 0.0;
-    for (int c = 0; c < nc; ++c) s += part[(int64_t)c * len + src];
-    out[e] = s;
+    // eight interleaved partial sums keep eight loads in flight (a single
+    // running sum waits on every load); combined in a fixed order
+    double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int c = 0;
+    for (; c + 8 <= nc; c += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] += part[(int64_t)(c + u) * len + src];
+    for (int u = 0; c < nc; ++c, ++u) s[u] += part[(int64_t)c * len + src];
+    out[e] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, const double *__restrict__ X,

@@ -285,6 +285,54 @@ int ssr_pattern_get(const ssr_pattern *p, int which, void *dst)
 const int32_t *ssr_pattern_d_row_ptr(const ssr_pattern *p) { return p->d_row_ptr; }
 const int32_t *ssr_pattern_d_col(const ssr_pattern *p) { return p->d_col; }
 const int32_t *ssr_pattern_d_emap(const ssr_pattern *p) { return p->d_emap; }
+// Pattern of a given node-block CSR (no elements): rows sorted, every
+// diagonal block present, all DOFs free.  Used by the AMG instances built
+// from COO triplets (amg_ijv.cpp); the assembly tables stay empty.
+int ssr_pattern_create_blocks(int dim, int64_t n_n, std::vector<int32_t> row_ptr, std::vector<int32_t> col,
+                              ssr_pattern **out)
+{
+    SSR_REQUIRE(out && (dim == 2 || dim == 3) && n_n > 0 && (int64_t)row_ptr.size() == n_n + 1,
+                "pattern_create_blocks: bad arguments");
+    ssr_pattern *p = new ssr_pattern();
+    p->dim = dim;
+    p->n_n = n_n;
+    p->row_ptr.swap(row_ptr);
+    p->col.swap(col);
+    p->nnzb = (int64_t)p->col.size();
+    auto find_blk = [&](int64_t r, int32_t c) -> int32_t {
+        const int32_t *b = p->col.data() + p->row_ptr[r], *e = p->col.data() + p->row_ptr[r + 1];
+        const int32_t *it = std::lower_bound(b, e, c);
+        return (it != e && *it == c) ? (int32_t)(it - p->col.data()) : -1;
+    };
+    p->diag.resize(n_n);
+    p->brow.resize(p->nnzb);
+    p->tblk.resize(p->nnzb);
+    for (int64_t r = 0; r < n_n; ++r) {
+        p->diag[r] = find_blk(r, (int32_t)r);
+        if (p->diag[r] < 0) {
+            delete p;
+            ssr_set_error("pattern_create_blocks: block row %lld has no diagonal block", (long long)r);
+            return SSR_ERR_ARG;
+        }
+        for (int32_t k = p->row_ptr[r]; k < p->row_ptr[r + 1]; ++k) {
+            p->brow[k] = (int32_t)r;
+            p->tblk[k] = find_blk(p->col[k], (int32_t)r);
+        }
+    }
+    p->mask.assign((size_t)dim * n_n, 1);
+    p->n_free = dim * n_n;
+    std::vector<double> md(p->mask.begin(), p->mask.end());
+    int rc;
+    if ((rc = upload(&p->d_row_ptr, p->row_ptr)) || (rc = upload(&p->d_col, p->col)) ||
+        (rc = upload(&p->d_diag, p->diag)) || (rc = upload(&p->d_brow, p->brow)) ||
+        (rc = upload(&p->d_tblk, p->tblk)) || (rc = upload(&p->d_mask, md))) {
+        ssr_pattern_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return SSR_OK;
+}
+
 const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p) { return p->d_tblk; }
 const int32_t *ssr_pattern_d_ublk(const ssr_pattern *p) { return p->d_ublk; }
 int64_t ssr_pattern_n_ublk(const ssr_pattern *p) { return (int64_t)p->ublk.size(); }
