@@ -166,6 +166,119 @@ def config1_leg(log):
             "dFoS": abs(res["FoS"] - float(lam[-1])), "speedup": t_cpu / res["time_to_fos_s"]}
 
 
+def run_assembly(args):
+    """Tangent assembly on the constitutive + assembly microbenchmark box
+    (40x20x20 m, 100x50x40 hexes x 6 P2 tetrahedra = 1.2M elements, 11
+    points each, 13.2M points) with DS from the return mapping of the
+    benchmark's trial strains (Davis B at lambda ~ U[1,2]).  Times, with HIP
+    events on the library stream: the tangent K_t into the node-block values
+    (ssr_dev_assemble_bsr: element pair blocks, then the per-block gather)
+    and the regularised K_r written once (ssr_dev_assemble_K_r).
+    Algorithmic bytes per element: DS 36x11x8 + WEIGHT 11x8 + DPhi 3x10x11x8
+    read, the block values written once (8 bytes x 9 x blocks / elements);
+    K_r also reads the elastic operator once.  CPU baseline: the oracle's
+    OpenMP element assembly (the reference's assemble_K_tangent_vals.c
+    restated) on a 12,000-element box of the same element size."""
+    from oracle import fe_oracle as FO
+    from oracle import oracle as O
+    from slope_stability_octave_amd import _lib, assembly as A, fem, mesh
+    from slope_stability_octave_amd import constitutive_mex as cm
+    from slope_stability_octave_amd import workloads as W
+    from slope_stability_octave_amd._lib import DeviceArray, EventTimer
+
+    _lib.require_gpu()
+    log = (lambda s: print(s, file=sys.stderr, flush=True))
+    nx, ny, nz = N_HEX
+    t0 = time.perf_counter()
+    coord, elem, Q = mesh.mesh_P2_box(nx, ny, nz, 40.0, 20.0, 20.0)
+    DPhi, WEIGHT, _, WF = fem.gradients(coord, elem, 3)
+    n_e, n_q, n_int = elem.shape[1], WF.size, WEIGHT.size
+    P = A.Pattern(elem, Q, 3)
+    log(f"[bench] box mesh, gradients and pattern: {n_e} elements, {P.nnzb} blocks, "
+        f"{time.perf_counter() - t0:.1f} s")
+    dD = [DeviceArray.from_host(x.ravel(order="F")) for x in DPhi]
+    del DPhi
+    dW = DeviceArray.from_host(WEIGHT)
+    d = W.config2_points(n_int)
+    dE = DeviceArray.from_host(d["E"])
+    dc0, dphi, dpsi, dlam = (DeviceArray.from_host(d[k]) for k in ("c0", "phi", "psi", "lam"))
+    dG, dK, dL = (DeviceArray.from_host(d[k]) for k in ("shear", "bulk", "lame"))
+    dcb, dsp = DeviceArray(n_int), DeviceArray(n_int)
+    cm.reduction_dev(n_int, "B", dc0, dphi, dpsi, dlam, True, dcb, dsp)
+    dS, dDS = DeviceArray((n_int, 6)), DeviceArray((n_int, 36))
+    cm.constitutive_dev(3, n_int, dE, dcb, dsp, dG, dK, dL, dS, dDS)
+    del dE, dc0, dphi, dpsi, dlam, dS, dcb, dsp
+    dDSe = DeviceArray((n_int, 36))
+    A.ds_elastic_dev(3, n_int, dG, dK, dDSe)
+    bsz = P.nnzb * P.bs
+    Kraw, Kel = DeviceArray(bsz), DeviceArray(bsz)
+    P.assemble(n_q, dD, dDSe, dW, Kraw)
+    P.regularize(1.0, Kraw, Kraw, Kel)
+    del dDSe
+    Kt, Kr = Kraw, DeviceArray(bsz)
+    if args.copy_calibration:
+        from slope_stability_octave_amd import linear_solvers as LS
+        for _ in range(2):
+            LS.axpby(1.0, Kel, 0.0, Kr)  # plain copy of 8*bsz bytes: the counter calibration
+        _lib.synchronize()
+    tim = EventTimer()
+    res = {}
+    for name, fn in (("K_t", lambda: P.assemble(n_q, dD, dDS, dW, Kt)),
+                     ("K_r", lambda: P.assemble_K_r(n_q, dD, dDS, dW, 1e-4, Kel, Kr))):
+        fn()
+        _lib.synchronize()
+        tim.start()
+        for _ in range(args.steps):
+            fn()
+        tim.stop()
+        res[name] = tim.elapsed_ms() / args.steps
+    blk_bytes = 8.0 * bsz / n_e
+    per_elem = {"K_t": 36 * n_q * 8 + n_q * 8 + 3 * 10 * n_q * 8 + blk_bytes}
+    per_elem["K_r"] = per_elem["K_t"] + blk_bytes
+    # CPU baseline on a 12,000-element box of the same element size
+    cpu = None
+    if not args.no_cpu_baseline:
+        c2, e2, Q2 = mesh.mesh_P2_box(20, 10, 10, 8.0, 4.0, 5.0)
+        D2, W2, _, _ = fem.gradients(c2, e2, 3)
+        n2 = W2.size
+        d2 = W.config2_points(n2)
+        cb2, sp2 = O.reduction(d2["c0"], d2["phi"], d2["psi"], d2["lam"], "B")
+        _, DS2, _ = O.mc3d(np.ascontiguousarray(d2["E"].T), cb2, sp2, d2["shear"], d2["bulk"], d2["lame"])
+        R2 = FO.RefPattern(e2, Q2, 3, D2, W2, d2["shear"], d2["bulk"])
+        smap = R2.scatter_map(e2, c2.shape[1])
+        O.assemble(3, D2, DS2, W2, smap, n_q, R2.nnz)
+        reps, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < 10.0:
+            O.assemble(3, D2, DS2, W2, smap, n_q, R2.nnz)
+            reps += 1
+        el = time.perf_counter() - t1
+        cpu = {"value": reps * e2.shape[1] / el / 1e6, "unit": "M elements/s", "cores": O.num_threads(),
+               "kind": "port", "sample": f"{reps} x {e2.shape[1]} elements (20x10x10 hexes, same element size), "
+                                         f"oracle element assembly with the int64 scatter map, {el:.1f} s"}
+    out = {
+        "metric": "tangent assembly throughput (config-2 box)",
+        "value": n_e / (res["K_t"] * 1e-3) / 1e6,
+        "unit": "M elements/s",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "ms_per_step": res["K_t"],
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D box 40x20x20 m, 100x50x40 hexes x 6 P2 tetrahedra, 11 points, DS of the "
+                               "microbenchmark strains", "elements": n_e, "blocks": P.nnzb},
+        "roofline": {"bound": "hbm", "achieved": per_elem["K_t"] * n_e / (res["K_t"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": per_elem["K_t"] * n_e / (res["K_t"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": {"K_r_ms": res["K_r"], "K_r_GBs": per_elem["K_r"] * n_e / (res["K_r"] * 1e-3) / 1e9,
+                  "K_r_frac": per_elem["K_r"] * n_e / (res["K_r"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "bytes_per_element": per_elem,
+                  "element_kernel": _lib.get_tuning(_lib.TUNE_ELEMENT_KERNEL)},
+    }
+    print(json.dumps(out))
+
+
 def run_fos(args):
     """Time-to-FoS of the indirect SSR continuation on the GPU (plain FP64
     form): the 2D homogeneous slope of slope_stability_2D_homo_SSR.m or the 3D
@@ -474,8 +587,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-time-to-fos", action="store_true",
                     help="skip the time-to-FoS leg (plain vs chosen form on the ~1.5M-DOF 3D slope)")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "assembly", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
                              "learned3d", "batch", "ll2d"])
+    ap.add_argument("--element-kernel", type=int, default=None, choices=[0, 1],
+                    help="element tangent kernel form (tuning key 8)")
     ap.add_argument("--h", type=float, default=0.5)
     ap.add_argument("--lambda-init", type=float, default=None,
                     help="first lambda of the continuation (default: 1.0 for fos2d, the parameter set's otherwise)")
@@ -537,6 +652,11 @@ def main():
         return run_batch(args)
     if args.workload == "ll2d":
         return run_ll(args)
+    if args.element_kernel is not None:
+        from slope_stability_octave_amd import _lib as L
+        L.set_tuning(L.TUNE_ELEMENT_KERNEL, args.element_kernel)
+    if args.workload == "assembly":
+        return run_assembly(args)
 
     from slope_stability_octave_amd import _lib
     from slope_stability_octave_amd import constitutive_mex as cm

@@ -27,6 +27,8 @@ const int32_t *ssr_pattern_d_col(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_emap(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_ublk(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_uptr(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_pslot(const ssr_pattern *p);
 int64_t ssr_pattern_n_ublk(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_elem(const ssr_pattern *p);
@@ -230,11 +232,15 @@ __global__ __launch_bounds__(64) void k_assemble(
 // shares D*B_b between the lanes of one node through LDS instead, at the cost
 // of two barriers and its LDS round trip per point.  The products are the
 // same expressions in the same order.
-template <int DIM, int NP, int NQ, int DSL>
+// With SLOT, each pair block goes to its slot in the upper block's
+// contribution list (pslot), transposed when its first node has the larger
+// global id, so that the slot holds the block as the upper block takes it.
+template <int DIM, int NP, int NQ, int DSL, bool SLOT = false>
 __global__ __launch_bounds__(64) void k_assemble_pairs(int64_t n_e, const double *__restrict__ D1,
                                                        const double *__restrict__ D2, const double *__restrict__ D3,
                                                        const double *__restrict__ DS, const double *__restrict__ W,
-                                                       double *__restrict__ out)
+                                                       double *__restrict__ out, const int32_t *__restrict__ elem,
+                                                       const int32_t *__restrict__ pslot)
 {
     using V = Voigt<DIM>;
     constexpr int NS = V::NS, NSYM = NS * (NS + 1) / 2, NPAIR = NP * (NP + 1) / 2;
@@ -313,14 +319,78 @@ __global__ __launch_bounds__(64) void k_assemble_pairs(int64_t n_e, const double
                         acc[i][j] += v;
                     }
             }
-            double *o = out + ((int64_t)e * NPAIR + pr) * DIM * DIM;
+            if (SLOT) {
+                double *o = out + (int64_t)pslot[(int64_t)e * NPAIR + pr] * DIM * DIM;
+                if (elem[e * NP + pa] > elem[e * NP + pb]) {
 #pragma unroll
-            for (int i = 0; i < DIM; ++i)
+                    for (int i = 0; i < DIM; ++i)
 #pragma unroll
-                for (int j = 0; j < DIM; ++j) o[i * DIM + j] = acc[i][j];
+                        for (int j = 0; j < DIM; ++j) o[j * DIM + i] = acc[i][j];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                        for (int j = 0; j < DIM; ++j) o[i * DIM + j] = acc[i][j];
+                }
+            } else {
+                double *o = out + ((int64_t)e * NPAIR + pr) * DIM * DIM;
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) o[i * DIM + j] = acc[i][j];
+            }
         }
         __syncthreads();
     }
+}
+
+// Upper blocks from their slot lists (contiguous, in element order): entry q
+// of upper block b is the plain sum of entry q of its slots, the same terms in
+// the same order as the per-block gather of k_gather_blocks, so the values
+// are the same bits.  REG = false writes K_t into b and, transposed, its
+// mirror tb; REG = true writes the regularised, symmetrised, masked K_r as
+// k_gather_regularize does.
+template <int DIM, bool REG>
+__global__ __launch_bounds__(256) void k_gather_slots(int64_t n_ub, const int32_t *__restrict__ ublk,
+                                                      const int32_t *__restrict__ uptr,
+                                                      const double *__restrict__ slots,
+                                                      const int32_t *__restrict__ brow,
+                                                      const int32_t *__restrict__ col,
+                                                      const int32_t *__restrict__ tblk,
+                                                      const double *__restrict__ mask, double r,
+                                                      const double *__restrict__ Kel, double *__restrict__ out)
+{
+    constexpr int DD = DIM * DIM;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t u = t / DD;
+    if (u >= n_ub) return;
+    const int q = (int)(t - u * DD), i = q / DIM, j = q - i * DIM, qt = j * DIM + i;
+    const int64_t b = ublk[u];
+    const int64_t tb = tblk[b];
+    const int32_t c0 = uptr[u], c1 = uptr[u + 1];
+    double s = 0.0;
+    for (int32_t c = c0; c < c1; ++c) s += slots[(int64_t)c * DD + q];
+    double st = s;
+    if (tb == b) {
+        st = 0.0;
+        for (int32_t c = c0; c < c1; ++c) st += slots[(int64_t)c * DD + qt];
+    }
+    if (!REG) {
+        out[b * DD + q] = s;
+        if (tb != b) out[tb * DD + qt] = s;
+        return;
+    }
+    double v;
+    {
+#pragma clang fp contract(off)
+        const double v1 = r * Kel[b * DD + q] + (1.0 - r) * s;
+        const double v2 = r * Kel[tb * DD + qt] + (1.0 - r) * st;
+        v = 0.5 * (v1 + v2);
+    }
+    const int64_t ro = (int64_t)brow[b] * DIM + i, co = (int64_t)col[b] * DIM + j;
+    if (mask[ro] == 0.0 || mask[co] == 0.0) v = (ro == co) ? 1.0 : 0.0;
+    out[b * DD + q] = v;
+    if (tb != b) out[tb * DD + qt] = v;
 }
 
 // K_r = r K_el + (1-r) sum_e Ke, symmetrised and Dirichlet-masked, in one pass
@@ -573,7 +643,7 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
 #define L3(DSL_, SC_) hipLaunchKernelGGL((k_assemble<3, 10, 11, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
 #define P3(DSL_) hipLaunchKernelGGL((k_assemble_pairs<3, 10, 11, DSL_>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, \
-                                    D1, D2, D3, DS, W, out)
+                                    D1, D2, D3, DS, W, out, nullptr, nullptr)
         if (scat == 2 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0) {
             if (dsl == 0) P3(0);
             else P3(1);
@@ -594,7 +664,7 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
 #define L2(DSL_, SC_) hipLaunchKernelGGL((k_assemble<2, 6, 7, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
 #define P2(DSL_) hipLaunchKernelGGL((k_assemble_pairs<2, 6, 7, DSL_>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, \
-                                    D1, D2, D3, DS, W, out)
+                                    D1, D2, D3, DS, W, out, nullptr, nullptr)
         if (scat == 2 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0) {
             if (dsl == 0) P2(0);
             else P2(1);
@@ -610,6 +680,66 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
     }
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
+}
+
+// Element pair blocks into the slot lists, then the upper-block gather
+// (REG: K_r from K_elast and r; else K_t).  Needs the slot tables.
+static int slot_assembly(const ssr_pattern *P, int n_q, const double *D1, const double *D2, const double *D3,
+                         const double *DS, int dsl, const double *W, bool reg, double r, const double *Kel,
+                         double *out)
+{
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    if (n_e == 0) return SSR_OK;
+    double *slots = nullptr;
+    int rc = ssr_pattern_keblk(P, &slots);
+    if (rc) return rc;
+    const int32_t *elem = ssr_pattern_d_elem(P), *pslot = ssr_pattern_d_pslot(P);
+    const int64_t want = 4096;
+    if (dim == 3) {
+        SSR_REQUIRE(n_p == 10 && n_q == 11, "tangent assembly (3D): P2 tetrahedra with 11 points supported");
+        const unsigned grid = (unsigned)std::min<int64_t>(n_e, want * 16);
+        if (dsl == 0)
+            hipLaunchKernelGGL((k_assemble_pairs<3, 10, 11, 0, true>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, D1,
+                               D2, D3, DS, W, slots, elem, pslot);
+        else
+            hipLaunchKernelGGL((k_assemble_pairs<3, 10, 11, 1, true>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, D1,
+                               D2, D3, DS, W, slots, elem, pslot);
+    } else {
+        SSR_REQUIRE(n_p == 6 && n_q == 7, "tangent assembly (2D): P2 triangles with 7 points supported");
+        const unsigned grid = (unsigned)std::min<int64_t>((n_e + 2) / 3, want * 16);
+        if (dsl == 0)
+            hipLaunchKernelGGL((k_assemble_pairs<2, 6, 7, 0, true>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, D1,
+                               D2, D3, DS, W, slots, elem, pslot);
+        else
+            hipLaunchKernelGGL((k_assemble_pairs<2, 6, 7, 1, true>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, D1,
+                               D2, D3, DS, W, slots, elem, pslot);
+    }
+    SSR_HIP_CHECK(hipGetLastError());
+    const int64_t n_ub = ssr_pattern_n_ublk(P);
+    const int64_t threads = n_ub * dim * dim;
+    if (threads == 0) return SSR_OK;
+#define GS(D_, R_)                                                                                                  \
+    hipLaunchKernelGGL((k_gather_slots<D_, R_>), dim3(ssr_blocks(threads, 256)), dim3(256), 0, ssr_stream(), n_ub,  \
+                       ssr_pattern_d_ublk(P), ssr_pattern_d_uptr(P), slots, ssr_pattern_d_brow(P),                  \
+                       ssr_pattern_d_col(P), ssr_pattern_d_tblk(P), ssr_pattern_d_mask(P), r, Kel, out)
+    if (dim == 3) {
+        if (reg) GS(3, true);
+        else GS(3, false);
+    } else {
+        if (reg) GS(2, true);
+        else GS(2, false);
+    }
+#undef GS
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+static bool slot_form(const ssr_pattern *P)
+{
+    return ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) == 0 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0 &&
+           ssr_pattern_d_pslot(P) && ssr_pattern_d_uptr(P) && ssr_pattern_d_ublk(P);
 }
 
 extern "C" {
@@ -640,6 +770,8 @@ int ssr_dev_assemble_bsr(const ssr_pattern *P, int n_q, const double *DPhi1, con
     int64_t n_n, n_p, n_e, nnzb, nnzf;
     ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
     SSR_REQUIRE(ssr_pattern_d_emap(P), "assemble_bsr: pattern has no device tables");
+    if (slot_form(P)) return slot_assembly(P, n_q, DPhi1, DPhi2, DPhi3, DS, ds_layout, WEIGHT, false, 0.0, nullptr,
+                                           Kvals);
     if (ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) == 0 && ssr_pattern_d_cptr(P)) {
         double *ke = nullptr;
         int rc = ssr_pattern_keblk(P, &ke);
@@ -764,6 +896,7 @@ int ssr_dev_assemble_K_r(const ssr_pattern *P, int n_q, const double *DPhi1, con
     ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
     SSR_REQUIRE(ssr_pattern_d_emap(P), "assemble_K_r: pattern has no device tables");
     int rc;
+    if (slot_form(P)) return slot_assembly(P, n_q, DPhi1, DPhi2, DPhi3, DS, ds_layout, WEIGHT, true, r, K_elast, K_r);
     if (ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) != 0 || !ssr_pattern_d_cptr(P) || !ssr_pattern_d_ublk(P)) {
         double *Kt = nullptr;
         if ((rc = ssr_pool_malloc((void **)&Kt, sizeof(double) * nnzb * dim * dim))) return rc;

@@ -44,6 +44,12 @@ struct ssr_pattern {
     // element order, as code = (e * n_pair + pair) * 2 + transposed
     std::vector<int32_t> cptr, ccode;
     int32_t *d_cptr = nullptr, *d_ccode = nullptr;
+    // slot form: every element pair block has one slot in the contribution
+    // list of its upper block (row <= column); lists in ublk order, elements
+    // ascending inside a list.  pslot: n_e * n_pair slot indices, uptr: list
+    // bounds per upper block
+    std::vector<int32_t> uptr, pslot;
+    int32_t *d_uptr = nullptr, *d_pslot = nullptr;
     mutable double *d_keblk = nullptr;  // n_e * n_pair * dim * dim element blocks (allocated on first use)
     // gather-form internal force: the element nodes (code = e * n_p + a) that
     // sum into each node, in element order
@@ -80,6 +86,8 @@ int ssr_pattern_destroy(ssr_pattern *p)
     (void)hipFree(p->d_emap);
     (void)hipFree(p->d_cptr);
     (void)hipFree(p->d_ccode);
+    (void)hipFree(p->d_uptr);
+    (void)hipFree(p->d_pslot);
     (void)hipFree(p->d_keblk);
     (void)hipFree(p->d_nptr);
     (void)hipFree(p->d_ncode);
@@ -193,6 +201,32 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
         }
     }
 
+    // slot lists of the upper blocks
+    {
+        const int64_t npair = n_p * (n_p + 1) / 2;
+        if (n_e * npair < ((int64_t)1 << 31)) {
+            std::vector<int32_t> uidx(p->nnzb, -1);
+            for (size_t u = 0; u < p->ublk.size(); ++u) uidx[p->ublk[u]] = (int32_t)u;
+            const int64_t nu = (int64_t)p->ublk.size();
+            p->uptr.assign(nu + 1, 0);
+            p->pslot.resize(n_e * npair);
+            auto upper_of = [&](int64_t e, int64_t a, int64_t b) -> int32_t {
+                const int64_t v = elem[e * n_p + a], w = elem[e * n_p + b];
+                const int64_t lo = v <= w ? a : b, hi = v <= w ? b : a;
+                return uidx[p->emap[(e * n_p + lo) * n_p + hi]];
+            };
+            for (int64_t e = 0; e < n_e; ++e)
+                for (int64_t a = 0; a < n_p; ++a)
+                    for (int64_t b = a; b < n_p; ++b) p->uptr[upper_of(e, a, b) + 1]++;
+            for (int64_t u = 0; u < nu; ++u) p->uptr[u + 1] += p->uptr[u];
+            std::vector<int32_t> fill(p->uptr.begin(), p->uptr.end() - 1);
+            int64_t k = 0;
+            for (int64_t e = 0; e < n_e; ++e)
+                for (int64_t a = 0; a < n_p; ++a)
+                    for (int64_t b = a; b < n_p; ++b) p->pslot[k++] = fill[upper_of(e, a, b)]++;
+        }
+    }
+
     // node incidences for the gather-form internal force
     if (n_e * n_p < ((int64_t)1 << 31)) {
         p->nptr.assign(n_n + 1, 0);
@@ -237,6 +271,7 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
             (rc = upload(&p->d_ref_pos, p->ref_pos)) || (rc = upload(&p->d_mask, md)) ||
             (rc = upload(&p->d_brow, p->brow)) ||
             (!p->cptr.empty() && ((rc = upload(&p->d_cptr, p->cptr)) || (rc = upload(&p->d_ccode, p->ccode)))) ||
+            (!p->pslot.empty() && ((rc = upload(&p->d_uptr, p->uptr)) || (rc = upload(&p->d_pslot, p->pslot)))) ||
             (!p->nptr.empty() && ((rc = upload(&p->d_nptr, p->nptr)) || (rc = upload(&p->d_ncode, p->ncode))))) {
             ssr_pattern_destroy(p);
             return rc;
@@ -335,6 +370,8 @@ int ssr_pattern_create_blocks(int dim, int64_t n_n, std::vector<int32_t> row_ptr
 
 const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p) { return p->d_tblk; }
 const int32_t *ssr_pattern_d_ublk(const ssr_pattern *p) { return p->d_ublk; }
+const int32_t *ssr_pattern_d_uptr(const ssr_pattern *p) { return p->d_uptr; }
+const int32_t *ssr_pattern_d_pslot(const ssr_pattern *p) { return p->d_pslot; }
 int64_t ssr_pattern_n_ublk(const ssr_pattern *p) { return (int64_t)p->ublk.size(); }
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p) { return p->d_diag; }
 const int32_t *ssr_pattern_d_elem(const ssr_pattern *p) { return p->d_elem; }

@@ -289,6 +289,7 @@ class LearnedPredictor:
         self.used = 0
         self.rejected = 0
         self.F = None
+        self.start_residuals = []  # (network start, secant start) per judged step
 
     def _residual(self, lam, U):
         from .newton import norm_Q
@@ -330,6 +331,7 @@ class LearnedPredictor:
         lincomb(c[0], Uh[0], c[1], Uh[1], U_sec)
         r_pred = self._residual(lam_pred, U)
         r_sec = self._residual(lh[-1], U_sec)
+        self.start_residuals.append((float(r_pred), float(r_sec)))
         if not (r_pred < r_sec):
             self.rejected += 1
             return None
