@@ -200,6 +200,7 @@ TUNE_ASSEMBLY_FORM = 5
 TUNE_SPMM_GROUP = 6
 TUNE_SPMV_UNROLL = 7
 TUNE_ELEMENT_KERNEL = 8
+TUNE_AMG_COARSE_BLOCKS = 9
 
 
 def set_tuning(key: int, value: int):

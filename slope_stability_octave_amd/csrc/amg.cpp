@@ -193,32 +193,103 @@ int dupload(T **d, const std::vector<T> &h)
     return SSR_OK;
 }
 
+// Device copy of a level matrix: scalar CSR, or block CSR with dense rb x cb
+// blocks when its rows come in whole nodes of rb unknowns and its columns in
+// whole nodes of cb (the coarse unknowns of SA-AMG are aggregate x mode, and
+// every aggregate of a level usually keeps all its modes).  The blocks hold
+// the CSR entries at their places and zeros elsewhere; each row's sum runs
+// over the same columns in the same order.
 template <class T>
 struct DCsr {
     int64_t nr = 0, nnz = 0;
     int64_t *rp = nullptr;
     int32_t *ci = nullptr;
     T *v = nullptr;
-    ~DCsr()
+    int rb = 0, cb = 0;  // block shape (0: scalar CSR)
+    int64_t nbr = 0, nblk = 0;
+    int32_t *brp = nullptr, *bcol = nullptr;
+    T *bval = nullptr;
+    ~DCsr() { release(); }
+    void release()
     {
         (void)hipFree(rp);
         (void)hipFree(ci);
         (void)hipFree(v);
+        (void)hipFree(brp);
+        (void)hipFree(bcol);
+        (void)hipFree(bval);
+        rp = nullptr;
+        ci = nullptr;
+        v = nullptr;
+        brp = bcol = nullptr;
+        bval = nullptr;
+        rb = cb = 0;
     }
-    int upload(const HCsr &A)
+    int upload(const HCsr &A, int row_block = 0, int col_block = 0)
     {
+        release();
         nr = A.nr;
         nnz = (int64_t)A.ci.size();
+        if (row_block > 0 && col_block > 0 && ssr::brc_supported(row_block, col_block) && A.nr % row_block == 0 &&
+            A.nc % col_block == 0 && A.nr / row_block < ((int64_t)1 << 31))
+            return upload_blocks(A, row_block, col_block);
         int rc;
         if ((rc = dupload(&rp, A.rp)) || (rc = dupload(&ci, A.ci))) return rc;
         std::vector<T> vt(A.v.begin(), A.v.end());
         return dupload(&v, vt);
     }
+    int upload_blocks(const HCsr &A, int r_b, int c_b)
+    {
+        const int64_t n_br = A.nr / r_b;
+        std::vector<int32_t> hrp(n_br + 1, 0), hcol;
+        std::vector<std::vector<int32_t>> cols(n_br);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t I = 0; I < n_br; ++I) {
+            auto &c = cols[I];
+            for (int64_t r = I * r_b; r < (I + 1) * r_b; ++r)
+                for (int64_t t = A.rp[r]; t < A.rp[r + 1]; ++t) c.push_back(A.ci[t] / c_b);
+            std::sort(c.begin(), c.end());
+            c.erase(std::unique(c.begin(), c.end()), c.end());
+        }
+        for (int64_t I = 0; I < n_br; ++I) hrp[I + 1] = hrp[I] + (int32_t)cols[I].size();
+        const int64_t nb = hrp[n_br];
+        hcol.resize(nb);
+        std::vector<T> hv((size_t)nb * r_b * c_b, T(0));
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t I = 0; I < n_br; ++I) {
+            const auto &c = cols[I];
+            std::copy(c.begin(), c.end(), hcol.begin() + hrp[I]);
+            for (int64_t r = I * r_b; r < (I + 1) * r_b; ++r)
+                for (int64_t t = A.rp[r]; t < A.rp[r + 1]; ++t) {
+                    const int32_t J = A.ci[t] / c_b;
+                    const int64_t k = std::lower_bound(c.begin(), c.end(), J) - c.begin() + hrp[I];
+                    hv[(size_t)k * r_b * c_b + (r - I * r_b) * c_b + (A.ci[t] - (int64_t)J * c_b)] = (T)A.v[t];
+                }
+        }
+        int rc;
+        if ((rc = dupload(&brp, hrp)) || (rc = dupload(&bcol, hcol)) || (rc = dupload(&bval, hv))) return rc;
+        rb = r_b;
+        cb = c_b;
+        nbr = n_br;
+        nblk = nb;
+        return SSR_OK;
+    }
     int spmv(const T *x, T *y, double a, double b, const T *z = nullptr) const
     {
+        if (rb) return ssr::spmv_brc<T>(rb, cb, nbr, brp, bcol, bval, x, y, a, b, z, nblk);
         return ssr::spmv_csr<T>(nr, rp, ci, v, x, y, a, b, z, nnz);
     }
 };
+
+// unknowns per node when every node of the level has the same number, else 0
+inline int uniform_node_size(const std::vector<int64_t> &node_ptr)
+{
+    if (node_ptr.size() < 2) return 0;
+    const int64_t d = node_ptr[1] - node_ptr[0];
+    for (size_t i = 1; i + 1 < node_ptr.size(); ++i)
+        if (node_ptr[i + 1] - node_ptr[i] != d) return 0;
+    return d > 0 && d <= 8 ? (int)d : 0;
+}
 
 // Device copy of one level in precision T.
 template <class T>
@@ -625,9 +696,11 @@ struct Amg : ssr_precond {
         Level &lv = *L[l];
         DevLevel<T> &dl = lv.dev<T>();
         int rc;
-        if (l > 0 && (rc = dl.A.upload(lv.A))) return rc;
+        const int d_l = ssr_tuning(SSR_TUNE_AMG_COARSE_BLOCKS) ? uniform_node_size(lv.node_ptr) : 0;
+        if (l > 0 && (rc = dl.A.upload(lv.A, d_l, d_l))) return rc;
         if (l + 1 < L.size()) {
-            if ((rc = dl.P.upload(lv.P)) || (rc = dl.R.upload(lv.R))) return rc;
+            const int d_c = d_l ? uniform_node_size(L[l + 1]->node_ptr) : 0;
+            if ((rc = dl.P.upload(lv.P, d_l, d_c)) || (rc = dl.R.upload(lv.R, d_c, d_l))) return rc;
         }
         std::vector<T> di(lv.dinv.begin(), lv.dinv.end());
         if ((rc = dupload(&dl.dinv, di))) return rc;

@@ -32,6 +32,12 @@ int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double 
 template <class T>
 int spmv_csr(int64_t n_rows, const int64_t *rp, const int32_t *ci, const T *v, const T *x, T *y, double alpha,
              double beta, const T *z = nullptr, int64_t nnz = -1);
+// y = alpha*A*x + beta*z on block CSR with dense rb x cb blocks (row-major;
+// supported shapes: brc_supported); nblk blocks in all picks the lanes per row
+template <class T>
+int spmv_brc(int rb, int cb, int64_t n_rows, const int32_t *rp, const int32_t *col, const T *val, const T *x, T *y,
+             double alpha, double beta, const T *z, int64_t nblk);
+bool brc_supported(int rb, int cb);
 // result_dev[0] = sum a_i b_i (m_i)
 template <class T>
 int dot(int64_t n, const T *a, const T *b, const double *mask, double *result_dev);

@@ -77,5 +77,7 @@ struct ssr_precond {
 #define SSR_TUNE_SPMV_UNROLL 7          /* blocks per lane per pass of the node-block SpMV: 1 or 2 (0: by dimension) */
 #define SSR_TUNE_ELEMENT_KERNEL 8       /* element pair blocks: 0 = per-lane products from LDS-staged point data
                                            (k_assemble_pairs), 1 = D*B shared through LDS per point (k_assemble) */
-#define SSR_TUNE_COUNT 9
+#define SSR_TUNE_AMG_COARSE_BLOCKS 9     /* 1: coarse AMG operators and transfers as block CSR over whole nodes
+                                           (default), 0: scalar CSR */
+#define SSR_TUNE_COUNT 10
 int ssr_tuning(int key);

@@ -359,3 +359,44 @@ def test_amg_graph_replays_match_eager(prec):
             s.setup_preconditioner(K)
     finally:
         _lib.set_tuning(_lib.TUNE_AMG_GRAPH, old)
+
+
+@pytest.mark.parametrize("prec,dim", [(64, 3), (32, 3), (64, 2)])
+def test_amg_coarse_block_csr_matches_scalar_csr(prec, dim):
+    """The V-cycle with its coarse operators and transfers in block CSR over
+    whole aggregates (the default) against the same hierarchy in scalar CSR:
+    the same entries summed in another lane order, so the results agree to
+    rounding (1e-12 in double, 1e-5 in float)."""
+    from slope_stability_octave_amd import _lib
+    if dim == 3:
+        prob = problems.slope_3d((3, 3, 3, 3, 3, 3), 3.0, 3.0, 3.0, 3.0, 3.0, 3.0, problems.HOMO_2D)
+    else:
+        prob = problems.slope_2d(h=0.5)
+    cmb = problems.make_constitutive(prob)
+    cmb.reduction(1.0)
+    U = DeviceArray((cmb.n_n, cmb.dim), zero=True)
+    cmb.build_F_and_DS_all(1.0, U)
+    K = cmb.build_K_r_QQ_vals(1e-4)
+    rng = np.random.default_rng(11)
+    x = DeviceArray.from_host(rng.normal(size=(cmb.n_n, cmb.dim)) * cmb.Q.T)
+    old = _lib.get_tuning(_lib.TUNE_AMG_COARSE_BLOCKS)
+    out = {}
+    try:
+        for mode in (1, 0):
+            _lib.set_tuning(_lib.TUNE_AMG_COARSE_BLOCKS, mode)
+            s = DFGMRES(cmb.pattern, 1e-8, 200, 1e-3, "amg", amg_options=dict(coord=prob["coord"], max_coarse=50),
+                        precision=prec)
+            s.setup_preconditioner(K)
+            y = DeviceArray((cmb.n_n, cmb.dim))
+            if prec == 64:
+                _lib.call("ssr_solver_precond_apply", s.handle, _lib.ptr(x), _lib.ptr(y))
+                out[mode] = y.get()
+            else:
+                xs, it = s.solve(K, x)
+                out[mode] = xs.get()
+    finally:
+        _lib.set_tuning(_lib.TUNE_AMG_COARSE_BLOCKS, old)
+    a, b = out[1], out[0]
+    rel = np.abs(a - b).max() / np.abs(b).max()
+    print(f"prec {prec} dim {dim}: block vs scalar CSR, max rel {rel:.2e}", flush=True)
+    assert rel <= (1e-12 if prec == 64 else 1e-5)
