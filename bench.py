@@ -589,7 +589,7 @@ def main():
                     help="skip the time-to-FoS leg (plain vs chosen form on the ~1.5M-DOF 3D slope)")
     ap.add_argument("--workload", default="constitutive", choices=["constitutive", "assembly", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
                              "learned3d", "batch", "ll2d"])
-    ap.add_argument("--element-kernel", type=int, default=None, choices=[0, 1],
+    ap.add_argument("--element-kernel", type=int, default=None, choices=[0, 1, 2],
                     help="element tangent kernel form (tuning key 8)")
     ap.add_argument("--h", type=float, default=0.5)
     ap.add_argument("--lambda-init", type=float, default=None,

@@ -121,8 +121,10 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *  key 6: lanes per block row of the multi-vector SpMV (0 = by dimension);
  *  key 7: blocks per lane per pass of the node-block SpMV (0 = by dimension:
  *         2 in 3D with 16 lanes per row, 1 in 2D; or 1, 2);
- *  key 8: element tangent kernel (0 = per-lane products from LDS-staged point
- *         data, default; 1 = D*B shared through LDS per point);
+ *  key 8: element tangent kernel (0 = two pairs of one column node per lane,
+ *         sharing D*B_b, default (2D as 2);
+ *         1 = D*B shared through LDS per point; 2 = per-lane products from
+ *         LDS-staged point data);
  *  key 9: coarse AMG operators, prolongators and restrictions as block CSR
  *         over whole nodes / aggregates (1, default) or scalar CSR (0);
  *         applies to hierarchies built after the call. */

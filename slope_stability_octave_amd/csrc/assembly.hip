@@ -344,6 +344,391 @@ __global__ __launch_bounds__(64) void k_assemble_pairs(int64_t n_e, const double
     }
 }
 
+// 3D pair blocks, two pairs of one column node per lane: the lane of node b
+// forms t = (wD) B_b once per point and uses it for the pairs (a1, b) and
+// (a2, b), a1 < a2 <= b, so the 54 FMA of D*B_b and the 21 broadcast LDS
+// reads of w*D serve two blocks (30 lanes per element, ceil((b+1)/2) for
+// node b; a column of odd length leaves one lane with a single pair, which
+// repeats its first pair and stores it once).  Two elements per wavefront,
+// one per 32-lane half.  The block arithmetic is k_assemble_pairs' term for
+// term, so the two kernels give the same bits.
+template <int DSL, bool SLOT>
+__global__ __launch_bounds__(64) void k_assemble_pairs2(int64_t n_e, const double *__restrict__ D1,
+                                                        const double *__restrict__ D2,
+                                                        const double *__restrict__ D3,
+                                                        const double *__restrict__ DS,
+                                                        const double *__restrict__ W, double *__restrict__ out,
+                                                        const int32_t *__restrict__ elem,
+                                                        const int32_t *__restrict__ pslot)
+{
+    using V = Voigt<3>;
+    constexpr int NP = 10, NQ = 11, NS = 6, NSYM = 21, NPAIR = 55, LPE = 30;
+    __shared__ double sG[2][NQ][NP][3];
+    __shared__ double sD[2][NQ][NSYM];
+    const int lane = threadIdx.x, le = lane >> 5, li = lane & 31;
+    int b = 0, a1 = 0, a2 = -1;
+    {
+        int k = li < LPE ? li : LPE - 1;
+        for (int bb = 0; bb < NP; ++bb) {
+            const int ng = (bb + 2) / 2;
+            if (k < ng) {
+                b = bb;
+                a1 = 2 * k;
+                a2 = 2 * k + 1 <= bb ? 2 * k + 1 : -1;
+                break;
+            }
+            k -= ng;
+        }
+    }
+    const int a2c = a2 >= 0 ? a2 : a1;
+    const int pr1 = a1 * NP - (a1 * (a1 - 1)) / 2 + (b - a1);
+    const int pr2 = a2c * NP - (a2c * (a2c - 1)) / 2 + (b - a2c);
+    for (int64_t e0 = (int64_t)blockIdx.x * 2; e0 < n_e; e0 += (int64_t)gridDim.x * 2) {
+        for (int t = lane; t < 2 * NQ * NP; t += 64) {
+            const int s = t / (NQ * NP), r = t - s * NQ * NP;
+            const int64_t e = e0 + s;
+            if (e < n_e) {
+                const int64_t g = e * NQ * NP + r;
+                const int q = r / NP, a = r - q * NP;
+                sG[s][q][a][0] = D1[g];
+                sG[s][q][a][1] = D2[g];
+                sG[s][q][a][2] = D3[g];
+            }
+        }
+        for (int t = lane; t < 2 * NQ * NSYM; t += 64) {
+            const int s = t / (NQ * NSYM), r = t - s * NQ * NSYM;
+            const int64_t e = e0 + s;
+            if (e < n_e) {
+                const int q = r / NSYM, k = r - q * NSYM;
+                int i = 0, kk = k;
+                while (kk >= NS - i) { kk -= NS - i; ++i; }
+                const int j = i + kk;
+                const int64_t g = e * NQ + q;
+                const double w = W[g];
+                const double d = DSL == 0 ? DS[g * NS * NS + i + NS * j] : DS[g * NSYM + k];
+                sD[s][q][k] = w * d;
+            }
+        }
+        __syncthreads();
+        const int64_t e = e0 + le;
+        if (li < LPE && e < n_e) {
+            double acc1[3][3], acc2[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc1[i][j] = acc2[i][j] = 0.0;
+            for (int q = 0; q < NQ; ++q) {
+                double g1[3], g2[3], gb[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    g1[c] = sG[le][q][a1][c];
+                    g2[c] = sG[le][q][a2c][c];
+                    gb[c] = sG[le][q][b][c];
+                }
+                double t[NS][3];
+#pragma unroll
+                for (int row = 0; row < NS; ++row)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < V::NNZ; ++k) v += sD[le][q][symidx(row, V::row(c, k), NS)] * gb[V::grad(c, k)];
+                        t[row][c] = v;
+                    }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        double v1 = 0.0, v2 = 0.0;
+#pragma unroll
+                        for (int k = 0; k < V::NNZ; ++k) {
+                            v1 += g1[V::grad(i, k)] * t[V::row(i, k)][j];
+                            v2 += g2[V::grad(i, k)] * t[V::row(i, k)][j];
+                        }
+                        acc1[i][j] += v1;
+                        acc2[i][j] += v2;
+                    }
+            }
+            const int32_t nb = elem[e * NP + b];
+            if (SLOT) {
+                double *o = out + (int64_t)pslot[e * NPAIR + pr1] * 9;
+                if (elem[e * NP + a1] > nb) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) o[j * 3 + i] = acc1[i][j];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) o[i * 3 + j] = acc1[i][j];
+                }
+                if (a2 >= 0) {
+                    double *o2 = out + (int64_t)pslot[e * NPAIR + pr2] * 9;
+                    if (elem[e * NP + a2] > nb) {
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int j = 0; j < 3; ++j) o2[j * 3 + i] = acc2[i][j];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int j = 0; j < 3; ++j) o2[i * 3 + j] = acc2[i][j];
+                    }
+                }
+            } else {
+                double *o = out + (e * NPAIR + pr1) * 9;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) o[i * 3 + j] = acc1[i][j];
+                if (a2 >= 0) {
+                    double *o2 = out + (e * NPAIR + pr2) * 9;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) o2[i * 3 + j] = acc2[i][j];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Loads of one element pair of k_assemble_pairs2p into registers.  Chunk
+// indices past the end of the pair (the last pair may hold one element) are
+// clamped to its last chunk, so every lane loads and stores without a branch:
+// duplicates load and store the same values.  (Branches here make the
+// compiler wait on each load in turn.)
+// A native two-double vector: its loads, stores and copies stay in registers.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+template <int NGI, int NDI, bool SLOT>
+__device__ __forceinline__ void pairs2p_prefetch(int64_t n_e, int64_t e0, int lane, const double *__restrict__ D1,
+                                                 const double *__restrict__ D2, const double *__restrict__ D3,
+                                                 const double *__restrict__ DS, const double *__restrict__ W,
+                                                 const int32_t *__restrict__ elem, const int32_t *__restrict__ pslot,
+                                                 int b, int a1, int a2c, int pr1, int pr2, dvec2 (&pg0)[NGI],
+                                                 dvec2 (&pg1)[NGI], dvec2 (&pg2)[NGI], dvec2 (&pd)[NDI],
+                                                 double (&pw)[NDI], int32_t (&pm)[5])
+{
+    constexpr int NP = 10, NQ = 11, GE = 110, DE = 396, NPAIR = 55;
+    const int ne = n_e - e0 >= 2 ? 2 : 1;
+    // the lane's element (its half of the pair, or the first when the pair
+    // holds one) and where its two blocks go
+    const int64_t ec = e0 + ((lane >> 5) < ne ? (lane >> 5) : 0);
+    if (SLOT) {
+        pm[0] = elem[ec * NP + b];
+        pm[1] = elem[ec * NP + a1];
+        pm[2] = elem[ec * NP + a2c];
+        pm[3] = pslot[ec * NPAIR + pr1];
+        pm[4] = pslot[ec * NPAIR + pr2];
+    }
+    const int gmax = ne * (GE / 2) - 1, dmax = ne * (DE / 2) - 1;
+#pragma unroll
+    for (int m = 0; m < NGI; ++m) {
+        const int ch = min(lane + 64 * m, gmax);
+        pg0[m] = *reinterpret_cast<const dvec2 *>(D1 + e0 * GE + 2 * ch);
+        pg1[m] = *reinterpret_cast<const dvec2 *>(D2 + e0 * GE + 2 * ch);
+        pg2[m] = *reinterpret_cast<const dvec2 *>(D3 + e0 * GE + 2 * ch);
+    }
+#pragma unroll
+    for (int m = 0; m < NDI; ++m) {
+        const int ch = min(lane + 64 * m, dmax);
+        pd[m] = *reinterpret_cast<const dvec2 *>(DS + e0 * DE + 2 * ch);
+        const int o = 2 * ch, se = o / DE, q = (o - se * DE) / 36;
+        pw[m] = W[(e0 + se) * NQ + q];
+    }
+}
+
+// Software-pipelined k_assemble_pairs2 (MEX-layout DS, 16-byte aligned
+// inputs): a persistent grid, and while a wavefront forms the pair blocks of
+// one element pair from LDS, the gradients, tangents and weights of its next
+// element pair are in flight into registers (16-byte loads), moved to LDS
+// once the current blocks are stored.  The memory latency of each element pair
+// is hidden behind the previous pair's compute instead of exposed before its
+// own.  The staged values and the block arithmetic are k_assemble_pairs2's,
+// so the results are the same bits.
+template <bool SLOT>
+__global__ __launch_bounds__(64) void k_assemble_pairs2p(int64_t n_e, const double *__restrict__ D1,
+                                                         const double *__restrict__ D2,
+                                                         const double *__restrict__ D3,
+                                                         const double *__restrict__ DS,
+                                                         const double *__restrict__ W, double *__restrict__ out,
+                                                         const int32_t *__restrict__ elem,
+                                                         const int32_t *__restrict__ pslot)
+{
+    using V = Voigt<3>;
+    constexpr int NP = 10, NQ = 11, NS = 6, NSYM = 21, NPAIR = 55, LPE = 30;
+    constexpr int GE = NQ * NP;           // gradients per element and component
+    constexpr int DE = NQ * NS * NS;      // raw tangent entries per element
+    constexpr int NGI = (GE + 63) / 64;   // 16-byte gradient chunks per lane and component (2 elements)
+    constexpr int NDI = (DE + 63) / 64;   // 16-byte tangent chunks per lane (2 elements)
+    __shared__ __align__(16) double sG[3][2 * GE];
+    __shared__ double sD[2][NQ][NSYM + 1];
+    __shared__ double sO[SLOT ? 2 : 1][NPAIR][9];    // the pair's blocks on their way to the slots
+    __shared__ int64_t sSlot[SLOT ? 2 : 1][NPAIR];
+    const int lane = threadIdx.x, le = lane >> 5, li = lane & 31;
+    int b = 0, a1 = 0, a2 = -1;
+    {
+        int k = li < LPE ? li : LPE - 1;
+        for (int bb = 0; bb < NP; ++bb) {
+            const int ng = (bb + 2) / 2;
+            if (k < ng) {
+                b = bb;
+                a1 = 2 * k;
+                a2 = 2 * k + 1 <= bb ? 2 * k + 1 : -1;
+                break;
+            }
+            k -= ng;
+        }
+    }
+    const int a2c = a2 >= 0 ? a2 : a1;
+    const int pr1 = a1 * NP - (a1 * (a1 - 1)) / 2 + (b - a1);
+    const int pr2 = a2c * NP - (a2c * (a2c - 1)) / 2 + (b - a2c);
+    dvec2 pg0[NGI], pg1[NGI], pg2[NGI], pd[NDI];
+    double pw[NDI];
+    int32_t pm[5] = {0, 0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * 2;
+    int64_t e0 = (int64_t)blockIdx.x * 2;
+    if (e0 < n_e)
+        pairs2p_prefetch<NGI, NDI, SLOT>(n_e, e0, lane, D1, D2, D3, DS, W, elem, pslot, b, a1, a2c, pr1, pr2, pg0, pg1,
+                                         pg2, pd, pw, pm);
+    for (; e0 < n_e; e0 += stride) {
+        {
+            const int ne = n_e - e0 >= 2 ? 2 : 1;
+            const int gmax = ne * (GE / 2) - 1, dmax = ne * (DE / 2) - 1;
+#pragma unroll
+            for (int m = 0; m < NGI; ++m) {
+                const int ch = min(lane + 64 * m, gmax);
+                *reinterpret_cast<dvec2 *>(&sG[0][2 * ch]) = pg0[m];
+                *reinterpret_cast<dvec2 *>(&sG[1][2 * ch]) = pg1[m];
+                *reinterpret_cast<dvec2 *>(&sG[2][2 * ch]) = pg2[m];
+            }
+#pragma unroll
+            for (int m = 0; m < NDI; ++m) {
+                const int ch = min(lane + 64 * m, dmax);
+                const int o = 2 * ch, se = o / DE, r = o - se * DE, q = r / (NS * NS), idx = r - q * NS * NS;
+                const int i = idx % NS, j = idx / NS;  // idx even: entries (i, j), (i + 1, j)
+                // entries below the diagonal go to a spare slot (index NSYM) instead of a branch
+                const int k0 = i <= j ? i * NS - (i * (i - 1)) / 2 + (j - i) : NSYM;
+                const int k1 = i + 1 <= j ? (i + 1) * NS - ((i + 1) * i) / 2 + (j - i - 1) : NSYM;
+                sD[se][q][k0] = pw[m] * pd[m].x;
+                sD[se][q][k1] = pw[m] * pd[m].y;
+            }
+        }
+        // the store targets of this pair came with its prefetch: loaded before
+        // the next pair's prefetch, so (the load counter being in order)
+        // waiting for them at the stores does not wait for that prefetch
+        const int64_t e = e0 + le;
+        const bool act = li < LPE && e < n_e;
+        const bool tr1 = pm[1] > pm[0], tr2 = pm[2] > pm[0];
+        const int64_t s1 = SLOT ? (int64_t)pm[3] : e * NPAIR + pr1;
+        const int64_t s2 = SLOT ? (int64_t)pm[4] : e * NPAIR + pr2;
+        __syncthreads();
+        // unconditional (the last iteration reloads its own pair), so that no
+        // branch makes the stores below wait for these loads
+        pairs2p_prefetch<NGI, NDI, SLOT>(n_e, e0 + stride < n_e ? e0 + stride : e0, lane, D1, D2, D3, DS, W, elem, pslot,
+                                         b, a1, a2c, pr1, pr2, pg0, pg1, pg2, pd, pw, pm);
+        if (act) {
+            double acc1[3][3], acc2[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc1[i][j] = acc2[i][j] = 0.0;
+            for (int q = 0; q < NQ; ++q) {
+                double g1[3], g2[3], gb[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    g1[c] = sG[c][le * GE + q * NP + a1];
+                    g2[c] = sG[c][le * GE + q * NP + a2c];
+                    gb[c] = sG[c][le * GE + q * NP + b];
+                }
+                double t[NS][3];
+#pragma unroll
+                for (int row = 0; row < NS; ++row)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        double v = 0.0;
+#pragma unroll
+                        for (int k = 0; k < V::NNZ; ++k) v += sD[le][q][symidx(row, V::row(c, k), NS)] * gb[V::grad(c, k)];
+                        t[row][c] = v;
+                    }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        double v1 = 0.0, v2 = 0.0;
+#pragma unroll
+                        for (int k = 0; k < V::NNZ; ++k) {
+                            v1 += g1[V::grad(i, k)] * t[V::row(i, k)][j];
+                            v2 += g2[V::grad(i, k)] * t[V::row(i, k)][j];
+                        }
+                        acc1[i][j] += v1;
+                        acc2[i][j] += v2;
+                    }
+            }
+            if (SLOT) {
+                // through LDS, so that the stores below write each slot with
+                // consecutive lanes
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) sO[le][pr1][tr1 ? j * 3 + i : i * 3 + j] = acc1[i][j];
+                sSlot[le][pr1] = s1;
+                if (a2 >= 0) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) sO[le][pr2][tr2 ? j * 3 + i : i * 3 + j] = acc2[i][j];
+                    sSlot[le][pr2] = s2;
+                }
+            } else {
+                double *o1 = out + s1 * 9;
+                double *o2 = out + s2 * 9;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) o1[i * 3 + j] = acc1[i][j];
+                if (a2 >= 0) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) o2[i * 3 + j] = acc2[i][j];
+                }
+            }
+        }
+        if (SLOT) {
+            // Each 72-byte slot as four 16-byte pieces and one 8-byte piece,
+            // five consecutive lanes per slot (a slot at an even index starts
+            // 16-byte aligned and ends with the 8-byte piece, one at an odd
+            // index starts with it): about two memory requests per slot
+            // instead of nine scattered 8-byte stores.
+            __syncthreads();
+            const int ne = n_e - e0 >= 2 ? 2 : 1;
+            for (int p = lane; p < ne * NPAIR * 5; p += 64) {
+                const int sl = p / 5, piece = p - sl * 5;
+                const int se = sl / NPAIR, pr = sl - se * NPAIR;
+                const int64_t dst = sSlot[se][pr];
+                const double *v = sO[se][pr];
+                double *o = out + dst * 9;
+                const int first = (dst & 1) == 0 ? 2 * piece : (piece == 0 ? 0 : 2 * piece - 1);
+                const bool wide = (dst & 1) == 0 ? piece < 4 : piece > 0;
+                if (wide) {
+                    dvec2 w2;
+                    w2.x = v[first];
+                    w2.y = v[first + 1];
+                    *reinterpret_cast<dvec2 *>(o + first) = w2;
+                } else {
+                    o[first] = v[first];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Upper blocks from their slot lists (contiguous, in element order): entry q
 // of upper block b is the plain sum of entry q of its slots, the same terms in
 // the same order as the per-block gather of k_gather_blocks, so the values
@@ -368,12 +753,43 @@ __global__ __launch_bounds__(256) void k_gather_slots(int64_t n_ub, const int32_
     const int64_t b = ublk[u];
     const int64_t tb = tblk[b];
     const int32_t c0 = uptr[u], c1 = uptr[u + 1];
+    // the operands of the epilogue are loaded before the slot stream
+    double kb = 0.0, ktb = 0.0, mro = 1.0, mco = 1.0;
+    int64_t ro = 0, co = 0;
+    if (REG) {
+        kb = Kel[b * DD + q];
+        ktb = Kel[tb * DD + qt];
+        ro = (int64_t)brow[b] * DIM + i;
+        co = (int64_t)col[b] * DIM + j;
+        mro = mask[ro];
+        mco = mask[co];
+    }
+    // four slot loads in flight per step, added in list order (the same sum)
+    const double *sp = slots + (int64_t)c0 * DD;
+    const int32_t n = c1 - c0;
     double s = 0.0;
-    for (int32_t c = c0; c < c1; ++c) s += slots[(int64_t)c * DD + q];
+    int32_t c = 0;
+    for (; c + 4 <= n; c += 4) {
+        const double x0 = sp[(int64_t)c * DD + q], x1 = sp[(int64_t)(c + 1) * DD + q];
+        const double x2 = sp[(int64_t)(c + 2) * DD + q], x3 = sp[(int64_t)(c + 3) * DD + q];
+        s += x0;
+        s += x1;
+        s += x2;
+        s += x3;
+    }
+    for (; c < n; ++c) s += sp[(int64_t)c * DD + q];
     double st = s;
     if (tb == b) {
         st = 0.0;
-        for (int32_t c = c0; c < c1; ++c) st += slots[(int64_t)c * DD + qt];
+        for (c = 0; c + 4 <= n; c += 4) {
+            const double x0 = sp[(int64_t)c * DD + qt], x1 = sp[(int64_t)(c + 1) * DD + qt];
+            const double x2 = sp[(int64_t)(c + 2) * DD + qt], x3 = sp[(int64_t)(c + 3) * DD + qt];
+            st += x0;
+            st += x1;
+            st += x2;
+            st += x3;
+        }
+        for (; c < n; ++c) st += sp[(int64_t)c * DD + qt];
     }
     if (!REG) {
         out[b * DD + q] = s;
@@ -383,12 +799,11 @@ __global__ __launch_bounds__(256) void k_gather_slots(int64_t n_ub, const int32_
     double v;
     {
 #pragma clang fp contract(off)
-        const double v1 = r * Kel[b * DD + q] + (1.0 - r) * s;
-        const double v2 = r * Kel[tb * DD + qt] + (1.0 - r) * st;
+        const double v1 = r * kb + (1.0 - r) * s;
+        const double v2 = r * ktb + (1.0 - r) * st;
         v = 0.5 * (v1 + v2);
     }
-    const int64_t ro = (int64_t)brow[b] * DIM + i, co = (int64_t)col[b] * DIM + j;
-    if (mask[ro] == 0.0 || mask[co] == 0.0) v = (ro == co) ? 1.0 : 0.0;
+    if (mro == 0.0 || mco == 0.0) v = (ro == co) ? 1.0 : 0.0;
     out[b * DD + q] = v;
     if (tb != b) out[tb * DD + qt] = v;
 }
@@ -630,6 +1045,19 @@ __global__ __launch_bounds__(256) void k_gather_blocks(int64_t nnzb, const int32
 // ---------------------------------------------------------------------------
 // Entry points
 // ---------------------------------------------------------------------------
+static bool aligned16(const void *a, const void *b, const void *c, const void *d)
+{
+    return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(c) |
+             reinterpret_cast<uintptr_t>(d)) & 15u) == 0;
+}
+
+// Persistent grid of the pipelined element kernel: two resident wavefronts
+// per SIMD (its register use), each looping over element pairs.
+static unsigned pipelined_grid(int64_t n_e)
+{
+    return (unsigned)std::min<int64_t>((n_e + 1) / 2, (int64_t)ssr_cu_count() * 8);
+}
+
 static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, int scat,
                            const double *D1, const double *D2, const double *D3, const double *DS,
                            const double *W, const int32_t *emap, const int64_t *smap, double *out)
@@ -640,11 +1068,21 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         SSR_REQUIRE(n_p == 10 && n_q == 11, "tangent assembly (3D): P2 tetrahedra with 11 points supported (got n_p=%lld, n_q=%d)",
                     (long long)n_p, n_q);
         const unsigned grid = (unsigned)std::min<int64_t>(n_e, want * 16);
+        const unsigned grid2 = (unsigned)std::min<int64_t>((n_e + 1) / 2, INT32_MAX);  // two elements per group
 #define L3(DSL_, SC_) hipLaunchKernelGGL((k_assemble<3, 10, 11, DSL_, SC_>), dim3(grid), dim3(64), 0, ssr_stream(), \
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
 #define P3(DSL_) hipLaunchKernelGGL((k_assemble_pairs<3, 10, 11, DSL_>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, \
                                     D1, D2, D3, DS, W, out, nullptr, nullptr)
-        if (scat == 2 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0) {
+#define U3(DSL_) hipLaunchKernelGGL((k_assemble_pairs2<DSL_, false>), dim3(grid2), dim3(64), 0, ssr_stream(), \
+                                    n_e, D1, D2, D3, DS, W, out, nullptr, nullptr)
+        const int ek = ssr_tuning(SSR_TUNE_ELEMENT_KERNEL);
+        if (scat == 2 && ek == 0 && dsl == 0 && aligned16(D1, D2, D3, DS)) {
+            hipLaunchKernelGGL((k_assemble_pairs2p<false>), dim3(pipelined_grid(n_e)), dim3(64), 0, ssr_stream(), n_e,
+                               D1, D2, D3, DS, W, out, nullptr, nullptr);
+        } else if (scat == 2 && ek == 0) {
+            if (dsl == 0) U3(0);
+            else U3(1);
+        } else if (scat == 2 && ek == 2) {
             if (dsl == 0) P3(0);
             else P3(1);
         } else if (scat == 2) {
@@ -656,6 +1094,7 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
         else L3(1, 1);
 #undef L3
 #undef P3
+#undef U3
     } else {
         SSR_REQUIRE(n_p == 6 && n_q == 7, "tangent assembly (2D): P2 triangles with 7 points supported (got n_p=%lld, n_q=%d)",
                     (long long)n_p, n_q);
@@ -665,7 +1104,7 @@ static int assemble_launch(int dim, int64_t n_p, int64_t n_e, int n_q, int dsl, 
                                          n_e, D1, D2, D3, DS, W, emap, smap, out)
 #define P2(DSL_) hipLaunchKernelGGL((k_assemble_pairs<2, 6, 7, DSL_>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, \
                                     D1, D2, D3, DS, W, out, nullptr, nullptr)
-        if (scat == 2 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0) {
+        if (scat == 2 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) != 1) {
             if (dsl == 0) P2(0);
             else P2(1);
         } else if (scat == 2) {
@@ -700,7 +1139,20 @@ static int slot_assembly(const ssr_pattern *P, int n_q, const double *D1, const 
     if (dim == 3) {
         SSR_REQUIRE(n_p == 10 && n_q == 11, "tangent assembly (3D): P2 tetrahedra with 11 points supported");
         const unsigned grid = (unsigned)std::min<int64_t>(n_e, want * 16);
-        if (dsl == 0)
+        // two elements per group, one group per element pair: the elements in
+        // flight stay a contiguous range, so their slot writes share lines
+        const unsigned grid2 = (unsigned)std::min<int64_t>((n_e + 1) / 2, INT32_MAX);
+        const bool two = ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0;
+        if (two && dsl == 0 && aligned16(D1, D2, D3, DS))
+            hipLaunchKernelGGL((k_assemble_pairs2p<true>), dim3(pipelined_grid(n_e)), dim3(64), 0, ssr_stream(), n_e,
+                               D1, D2, D3, DS, W, slots, elem, pslot);
+        else if (two && dsl == 0)
+            hipLaunchKernelGGL((k_assemble_pairs2<0, true>), dim3(grid2), dim3(64), 0, ssr_stream(), n_e, D1,
+                               D2, D3, DS, W, slots, elem, pslot);
+        else if (two)
+            hipLaunchKernelGGL((k_assemble_pairs2<1, true>), dim3(grid2), dim3(64), 0, ssr_stream(), n_e, D1,
+                               D2, D3, DS, W, slots, elem, pslot);
+        else if (dsl == 0)
             hipLaunchKernelGGL((k_assemble_pairs<3, 10, 11, 0, true>), dim3(grid), dim3(64), 0, ssr_stream(), n_e, D1,
                                D2, D3, DS, W, slots, elem, pslot);
         else
@@ -738,7 +1190,7 @@ static int slot_assembly(const ssr_pattern *P, int n_q, const double *D1, const 
 
 static bool slot_form(const ssr_pattern *P)
 {
-    return ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) == 0 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) == 0 &&
+    return ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) == 0 && ssr_tuning(SSR_TUNE_ELEMENT_KERNEL) != 1 &&
            ssr_pattern_d_pslot(P) && ssr_pattern_d_uptr(P) && ssr_pattern_d_ublk(P);
 }
 

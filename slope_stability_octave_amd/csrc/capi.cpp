@@ -325,6 +325,8 @@ extern "C" int ssr_set_tuning(int key, int value)
                     "set_tuning: multi-vector SpMV group must be 0 (by dimension), 4, 8, 16 or 32");
     if (key == SSR_TUNE_SPMV_UNROLL)
         SSR_REQUIRE(value >= 0 && value <= 2, "set_tuning: SpMV blocks per lane must be 0 (by dimension), 1 or 2");
+    if (key == SSR_TUNE_ELEMENT_KERNEL)
+        SSR_REQUIRE(value >= 0 && value <= 2, "set_tuning: element kernel must be 0, 1 or 2");
     // Replay of the V-cycle from a captured graph is switched off: one profiled
     // FP32 continuation ended in a host SIGSEGV inside Amg::apply_graph<float>
     // whose cause is not known, so the form cannot be selected.

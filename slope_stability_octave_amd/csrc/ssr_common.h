@@ -75,8 +75,10 @@ struct ssr_precond {
                                            1: FP64 atomic adds of the element contributions */
 #define SSR_TUNE_SPMM_GROUP 6           /* lanes per block row of the multi-vector SpMV (0: by dimension) */
 #define SSR_TUNE_SPMV_UNROLL 7          /* blocks per lane per pass of the node-block SpMV: 1 or 2 (0: by dimension) */
-#define SSR_TUNE_ELEMENT_KERNEL 8       /* element pair blocks: 0 = per-lane products from LDS-staged point data
-                                           (k_assemble_pairs), 1 = D*B shared through LDS per point (k_assemble) */
+#define SSR_TUNE_ELEMENT_KERNEL 8       /* element pair blocks: 0 = two pairs of one column node per lane
+                                           (k_assemble_pairs2; 2D as 2),
+                                           1 = D*B shared through LDS per point (k_assemble),
+                                           2 = per-lane products from LDS-staged point data (k_assemble_pairs) */
 #define SSR_TUNE_AMG_COARSE_BLOCKS 9     /* 1: coarse AMG operators and transfers as block CSR over whole nodes
                                            (default), 0: scalar CSR */
 #define SSR_TUNE_COUNT 10

@@ -177,8 +177,8 @@ def test_strain_and_force(dim):
 def test_fused_K_r_equals_two_pass_path(dim, size):
     """ssr_dev_assemble_K_r (element pair blocks, then one gather-and-
     regularise pass over the upper blocks writing each block and its mirror)
-    against assemble + regularize: the same bits, for both element-kernel
-    forms; the two element kernels agree to rounding, and K_r matches the
+    against assemble + regularize: the same bits, for every element-kernel
+    form; the element kernels agree to rounding, and K_r matches the
     oracle's regularised tangent at 1e-11."""
     from slope_stability_octave_amd import _lib
     p = problem(dim, size, seed=5)
@@ -198,7 +198,7 @@ def test_fused_K_r_equals_two_pass_path(dim, size):
     old = _lib.get_tuning(_lib.TUNE_ELEMENT_KERNEL)
     res = {}
     try:
-        for form in (0, 1):
+        for form in (0, 1, 2):
             _lib.set_tuning(_lib.TUNE_ELEMENT_KERNEL, form)
             Kt = DeviceArray(P.nnzb * P.bs)
             P.assemble(p["n_q"], dD, dDS, dW, Kt)
@@ -218,3 +218,72 @@ def test_fused_K_r_equals_two_pass_path(dim, size):
     P.extract_ref(DeviceArray.from_host(res[0][1]), Vr)
     Vr0 = p["R"].build_K_r(r, p["R"].global_tangent(p["DS"], p["WEIGHT"]))
     assert relK(Vr.get(), Vr0) <= TOL_K
+
+
+class _Offset:
+    """A device address a few bytes into a DeviceArray (to present an input
+    that is not 16-byte aligned)."""
+
+    def __init__(self, base, nbytes):
+        self.base, self.ptr = base, base.ptr + nbytes
+
+
+@pytest.mark.parametrize("n_drop", [0, 1])
+def test_element_kernel_forms_same_bits(n_drop):
+    """The element kernels that give each lane two pairs of one column node
+    (the persistent, prefetching form for aligned MEX-layout DS; the plain
+    form otherwise) against the one-pair-per-lane kernel: the same terms in
+    the same order, so K_t and K_r agree bit for bit, with an even and an odd
+    element count (the last element pair then holds one element), 16-byte
+    aligned and unaligned DS, MEX-layout and packed DS."""
+    from slope_stability_octave_amd import _lib
+    p = problem(3, "medium", seed=7)
+    n_q = p["n_q"]
+    keep = np.arange(p["elem"].shape[1])
+    if n_drop:  # drop one element whose nodes all lie in other elements too
+        cnt = np.bincount(p["elem"].ravel())
+        k = next(k for k in range(keep.size) if (cnt[p["elem"][:, k]] > 1).all())
+        keep = np.delete(keep, k)
+    elem = p["elem"][:, keep]
+    n_e = elem.shape[1]
+    assert n_e % 2 == (1 if n_drop else 0)
+    assert np.unique(elem).size == p["coord"].shape[1]  # every node still in an element
+    pts = (keep[:, None] * n_q + np.arange(n_q)).ravel()
+    n_int = pts.size
+    DPhi = [x[:, pts] for x in p["DPhi"]]
+    WEIGHT, DS = p["WEIGHT"][pts], p["DS"][:, pts]
+    P = A.Pattern(elem, p["Q"], 3)
+    dD = [DeviceArray.from_host(x.ravel(order="F")) for x in DPhi]
+    dW = DeviceArray.from_host(WEIGHT)
+    DSm = np.asfortranarray(DS).T.copy()  # (n_int, 36)
+    dDS = DeviceArray.from_host(DSm)
+    # the same DS one double into a larger buffer: 8-byte but not 16-byte aligned
+    buf = DeviceArray.from_host(np.concatenate([[0.0], DSm.ravel()]))
+    dDSu = _Offset(buf, 8)
+    iu = np.triu_indices(6)
+    DSp = np.ascontiguousarray(DSm.reshape(n_int, 6, 6).transpose(0, 2, 1)[:, iu[0], iu[1]])  # row-major upper
+    dDSp = DeviceArray.from_host(DSp)
+    Kel = DeviceArray(P.nnzb * P.bs)
+    Kel.set(np.abs(np.random.default_rng(1).normal(size=P.nnzb * P.bs)))
+    old = _lib.get_tuning(_lib.TUNE_ELEMENT_KERNEL)
+    res = {}
+    try:
+        for form in (2, 0):
+            _lib.set_tuning(_lib.TUNE_ELEMENT_KERNEL, form)
+            for name, ds, lay in (("mex", dDS, 0), ("unaligned", dDSu, 0), ("packed", dDSp, 1)):
+                Kt = DeviceArray(P.nnzb * P.bs)
+                P.assemble(n_q, dD, ds, dW, Kt, ds_layout=lay)
+                Kr = DeviceArray(P.nnzb * P.bs)
+                P.assemble_K_r(n_q, dD, ds, dW, 0.3, Kel, Kr, ds_layout=lay)
+                res[(form, name)] = (Kt.get(), Kr.get())
+    finally:
+        _lib.set_tuning(_lib.TUNE_ELEMENT_KERNEL, old)
+    ref_t, ref_r = res[(2, "mex")]
+    for key, (kt, kr) in res.items():
+        assert np.array_equal(kt, ref_t), key
+        assert np.array_equal(kr, ref_r), key
+    # and the values themselves against the oracle's element assembly
+    R = FO.RefPattern(elem, p["Q"], 3, DPhi, WEIGHT, p["shear"][pts], p["bulk"][pts])
+    Vt = DeviceArray(P.nnz_free)
+    P.extract_ref(DeviceArray.from_host(ref_t), Vt)
+    assert relK(Vt.get(), R.global_tangent(DS, WEIGHT)) <= TOL_K
