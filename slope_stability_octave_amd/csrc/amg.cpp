@@ -74,23 +74,54 @@ struct HCsr {
     std::vector<double> v;
 };
 
+// A' with the rows of each column in ascending order: per-thread column
+// counts over fixed row ranges give each thread its place in every column,
+// so the result does not depend on the thread count.
 HCsr transpose(const HCsr &A)
 {
     HCsr T;
     T.nr = A.nc;
     T.nc = A.nr;
     T.rp.assign(T.nr + 1, 0);
-    for (int32_t c : A.ci) T.rp[c + 1]++;
-    for (int64_t i = 0; i < T.nr; ++i) T.rp[i + 1] += T.rp[i];
     T.ci.resize(A.ci.size());
     T.v.resize(A.v.size());
-    std::vector<int64_t> fill(T.rp.begin(), T.rp.end() - 1);
-    for (int64_t r = 0; r < A.nr; ++r)
-        for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) {
-            const int64_t d = fill[A.ci[k]]++;
-            T.ci[d] = (int32_t)r;
-            T.v[d] = A.v[k];
+    const int nt = std::max(1, std::min<int>(omp_get_max_threads(), (int)(A.nr / 4096) + 1));
+    std::vector<int64_t> cnt((size_t)nt * T.nr, 0);
+    auto range = [&](int t, int64_t &r0, int64_t &r1) {
+        r0 = A.nr * t / nt;
+        r1 = A.nr * (t + 1) / nt;
+    };
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+        int64_t r0, r1;
+        range(t, r0, r1);
+        int64_t *c = cnt.data() + (size_t)t * T.nr;
+        for (int64_t k = A.rp[r0]; k < A.rp[r1]; ++k) c[A.ci[k]]++;
+    }
+    // column c of thread t starts after column c of threads < t
+    for (int64_t c = 0; c < T.nr; ++c) {
+        int64_t o = 0;
+        for (int t = 0; t < nt; ++t) {
+            const int64_t x = cnt[(size_t)t * T.nr + c];
+            cnt[(size_t)t * T.nr + c] = o;
+            o += x;
         }
+        T.rp[c + 1] = o;
+    }
+    for (int64_t c = 0; c < T.nr; ++c) T.rp[c + 1] += T.rp[c];
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+        int64_t r0, r1;
+        range(t, r0, r1);
+        int64_t *f = cnt.data() + (size_t)t * T.nr;
+        for (int64_t r = r0; r < r1; ++r)
+            for (int64_t k = A.rp[r]; k < A.rp[r + 1]; ++k) {
+                const int32_t c = A.ci[k];
+                const int64_t d = T.rp[c] + f[c]++;
+                T.ci[d] = (int32_t)r;
+                T.v[d] = A.v[k];
+            }
+    }
     return T;
 }
 
@@ -410,7 +441,31 @@ struct Amg : ssr_precond {
     double rho_up_block = 1.0;  // Gershgorin bound of rho(D_B^-1 A) on the finest level
     double *binv64 = nullptr;
     float *binv32 = nullptr;
-    std::vector<double> Kh;
+    // page-locked host copy of the fine operator's block values (downloaded
+    // at each setup; kept for the next one)
+    struct PinnedVals {
+        double *p = nullptr;
+        size_t n = 0;
+        int reserve(size_t m)
+        {
+            if (m <= n) return SSR_OK;
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            n = 0;
+            SSR_HIP_CHECK(hipHostMalloc((void **)&p, sizeof(double) * m, hipHostMallocDefault));
+            n = m;
+            return SSR_OK;
+        }
+        double &operator[](size_t i) { return p[i]; }
+        const double &operator[](size_t i) const { return p[i]; }
+        double *data() { return p; }
+        const double *data() const { return p; }
+        size_t size() const { return n; }
+        ~PinnedVals()
+        {
+            if (p) (void)hipHostFree(p);
+        }
+    } Kh;
     int64_t n_setup = 0;
     int64_t generation = 0;  // bumped whenever device level data is (re)uploaded
     bool built = false;
@@ -536,19 +591,23 @@ struct Amg : ssr_precond {
         std::vector<int32_t> node_of(A.nr);
         for (int64_t i = 0; i < nn; ++i)
             for (int64_t d = lv.node_ptr[i]; d < lv.node_ptr[i + 1]; ++d) node_of[d] = (int32_t)i;
-        // node diagonal block norms
+        // node diagonal block norms (each node sums its own rows, in order)
         std::vector<double> dn(nn, 0.0);
-        for (int64_t r = 0; r < A.nr; ++r)
-            for (int64_t t = A.rp[r]; t < A.rp[r + 1]; ++t)
-                if (node_of[A.ci[t]] == node_of[r]) dn[node_of[r]] += A.v[t] * A.v[t];
-        // strong node graph
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < nn; ++i) {
+            double s = 0.0;
+            for (int64_t r = lv.node_ptr[i]; r < lv.node_ptr[i + 1]; ++r)
+                for (int64_t t = A.rp[r]; t < A.rp[r + 1]; ++t)
+                    if (node_of[A.ci[t]] == i) s += A.v[t] * A.v[t];
+            dn[i] = s;
+        }
+        // strong node graph: nodes in parallel, counted first, then filled
         std::vector<int64_t> sp(nn + 1, 0);
         std::vector<int32_t> sc;
-        std::vector<double> acc(nn, 0.0);
-        std::vector<int64_t> mk(nn, -1);
-        std::vector<int32_t> nb;
-        for (int64_t i = 0; i < nn; ++i) {
+        auto strong = [&](int64_t i, std::vector<double> &acc, std::vector<int64_t> &mk, std::vector<int32_t> &nb,
+                          std::vector<int32_t> &out) {
             nb.clear();
+            out.clear();
             for (int64_t d = lv.node_ptr[i]; d < lv.node_ptr[i + 1]; ++d)
                 for (int64_t t = A.rp[d]; t < A.rp[d + 1]; ++t) {
                     const int32_t j = node_of[A.ci[t]];
@@ -561,8 +620,25 @@ struct Amg : ssr_precond {
                     acc[j] += A.v[t] * A.v[t];
                 }
             for (int32_t j : nb)
-                if (acc[j] > 0.0 && acc[j] >= opt.theta * opt.theta * sqrt(dn[i] * dn[j])) sc.push_back(j);
-            sp[i + 1] = (int64_t)sc.size();
+                if (acc[j] > 0.0 && acc[j] >= opt.theta * opt.theta * sqrt(dn[i] * dn[j])) out.push_back(j);
+        };
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1) {
+                for (int64_t i = 0; i < nn; ++i) sp[i + 1] += sp[i];
+                sc.resize(sp[nn]);
+            }
+#pragma omp parallel
+            {
+                std::vector<double> acc(nn, 0.0);
+                std::vector<int64_t> mk(nn, -1);
+                std::vector<int32_t> nb, out;
+#pragma omp for schedule(dynamic, 1024)
+                for (int64_t i = 0; i < nn; ++i) {
+                    strong(i, acc, mk, nb, out);
+                    if (pass == 0) sp[i + 1] = (int64_t)out.size();
+                    else std::copy(out.begin(), out.end(), sc.begin() + sp[i]);
+                }
+            }
         }
         std::vector<int32_t> agg(nn, -1);
         n_agg = 0;
@@ -604,24 +680,31 @@ struct Amg : ssr_precond {
     {
         const int64_t nn = (int64_t)lv.node_ptr.size() - 1;
         const int64_t nf = lv.A.nr;
-        std::vector<std::vector<int64_t>> dofs(n_agg);
+        // the fine unknowns of each aggregate, in node order
+        std::vector<int64_t> dptr(n_agg + 1, 0), dofs;
         for (int64_t i = 0; i < nn; ++i)
-            if (agg[i] >= 0)
-                for (int64_t d = lv.node_ptr[i]; d < lv.node_ptr[i + 1]; ++d) dofs[agg[i]].push_back(d);
-        // per-row entries of T (each fine dof belongs to at most one aggregate)
-        std::vector<std::vector<std::pair<int32_t, double>>> rows(nf);
-        std::vector<double> bcv;
-        cnode_ptr.assign(1, 0);
-        int64_t nc = 0;
-        std::vector<std::vector<double>> Rcols;  // per aggregate, kept R rows (k each)
+            if (agg[i] >= 0) dptr[agg[i] + 1] += lv.node_ptr[i + 1] - lv.node_ptr[i];
+        for (int32_t a = 0; a < n_agg; ++a) dptr[a + 1] += dptr[a];
+        dofs.resize(dptr[n_agg]);
+        {
+            std::vector<int64_t> f(dptr.begin(), dptr.end() - 1);
+            for (int64_t i = 0; i < nn; ++i)
+                if (agg[i] >= 0)
+                    for (int64_t d = lv.node_ptr[i]; d < lv.node_ptr[i + 1]; ++d) dofs[f[agg[i]]++] = d;
+        }
+        // per aggregate (independent, in parallel): the near-null space
+        // restricted to it, orthonormalised column by column (CGS2) with the
+        // dependent columns dropped; Q (m x kept) and the kept rows of R
+        std::vector<std::vector<double>> Qs(n_agg), Rs(n_agg);
+        std::vector<int> kept_of(n_agg, 0);
+#pragma omp parallel for schedule(dynamic, 64)
         for (int32_t a = 0; a < n_agg; ++a) {
-            const auto &D = dofs[a];
-            const int64_t m = (int64_t)D.size();
-            std::vector<double> Qm((size_t)m * kl, 0.0);
+            const int64_t *D = dofs.data() + dptr[a];
+            const int64_t m = dptr[a + 1] - dptr[a];
+            std::vector<double> Qm((size_t)m * kl, 0.0), v(m);
             std::vector<std::vector<double>> R;  // rows of R for kept columns
             int kept = 0;
             for (int c = 0; c < kl; ++c) {
-                std::vector<double> v(m);
                 double n0 = 0.0;
                 for (int64_t i = 0; i < m; ++i) {
                     v[i] = Bl[(size_t)c * nf + D[i]];
@@ -648,24 +731,43 @@ struct Amg : ssr_precond {
                     kept++;
                 }
             }
-            for (int q = 0; q < kept; ++q) {
-                for (int64_t i = 0; i < m; ++i) rows[D[i]].push_back({(int32_t)(nc + q), Qm[(size_t)q * m + i]});
-                bcv.insert(bcv.end(), R[q].begin(), R[q].end());  // row (nc+q) of Bc, k entries
-            }
-            nc += kept;
-            cnode_ptr.push_back(nc);
+            Qm.resize((size_t)kept * m);
+            Qs[a].swap(Qm);
+            std::vector<double> rr;
+            rr.reserve((size_t)kept * kl);
+            for (int q = 0; q < kept; ++q) rr.insert(rr.end(), R[q].begin(), R[q].end());
+            Rs[a].swap(rr);
+            kept_of[a] = kept;
         }
+        // coarse numbering in aggregate order
+        std::vector<int64_t> cofs(n_agg + 1, 0);
+        for (int32_t a = 0; a < n_agg; ++a) cofs[a + 1] = cofs[a] + kept_of[a];
+        const int64_t nc = cofs[n_agg];
+        cnode_ptr.assign(cofs.begin(), cofs.end());
+        std::vector<double> bcv((size_t)nc * kl);
+        for (int32_t a = 0; a < n_agg; ++a) std::copy(Rs[a].begin(), Rs[a].end(), bcv.begin() + cofs[a] * kl);
+        // T: fine unknown D[i] of aggregate a has entries Q(i, q) at columns cofs[a] + q
+        std::vector<int32_t> agg_of(nf, -1), pos_of(nf, 0);
+        for (int32_t a = 0; a < n_agg; ++a)
+            for (int64_t i = dptr[a]; i < dptr[a + 1]; ++i) {
+                agg_of[dofs[i]] = a;
+                pos_of[dofs[i]] = (int32_t)(i - dptr[a]);
+            }
         T.nr = nf;
         T.nc = nc;
         T.rp.assign(nf + 1, 0);
-        T.ci.clear();
-        T.v.clear();
-        for (int64_t i = 0; i < nf; ++i) {
-            for (auto &e : rows[i]) {
-                T.ci.push_back(e.first);
-                T.v.push_back(e.second);
+        for (int64_t d = 0; d < nf; ++d) T.rp[d + 1] = T.rp[d] + (agg_of[d] >= 0 ? kept_of[agg_of[d]] : 0);
+        T.ci.resize(T.rp[nf]);
+        T.v.resize(T.rp[nf]);
+#pragma omp parallel for schedule(static)
+        for (int64_t d = 0; d < nf; ++d) {
+            const int32_t a = agg_of[d];
+            if (a < 0) continue;
+            const int64_t m = dptr[a + 1] - dptr[a];
+            for (int q = 0; q < kept_of[a]; ++q) {
+                T.ci[T.rp[d] + q] = (int32_t)(cofs[a] + q);
+                T.v[T.rp[d] + q] = Qs[a][(size_t)q * m + pos_of[d]];
             }
-            T.rp[i + 1] = (int64_t)T.ci.size();
         }
         // Bc column-major nc x kl from row-major bcv
         Bc.assign((size_t)nc * kl, 0.0);
@@ -746,15 +848,13 @@ struct Amg : ssr_precond {
             const double om = 4.0 / (3.0 * lv.rho);
             HCsr P;
             {
-                // P = T - om * Dinv * AT (union of patterns; T's pattern is contained in AT's
-                // for rows touching aggregates, merged generally)
+                // P = T - om * Dinv * AT (union of the two patterns, columns
+                // sorted per row); rows in parallel, counted first
                 P.nr = T.nr;
                 P.nc = T.nc;
                 P.rp.assign(P.nr + 1, 0);
-                std::vector<double> accv(T.nc, 0.0);
-                std::vector<int64_t> mk(T.nc, -1);
-                std::vector<int32_t> cs;
-                for (int64_t r = 0; r < P.nr; ++r) {
+                auto row = [&](int64_t r, std::vector<double> &accv, std::vector<int64_t> &mk,
+                               std::vector<int32_t> &cs) {
                     cs.clear();
                     for (int64_t t = T.rp[r]; t < T.rp[r + 1]; ++t) {
                         if (mk[T.ci[t]] != r) { mk[T.ci[t]] = r; accv[T.ci[t]] = 0.0; cs.push_back(T.ci[t]); }
@@ -765,8 +865,35 @@ struct Amg : ssr_precond {
                         accv[AT.ci[t]] -= om * lv.dinv[r] * AT.v[t];
                     }
                     std::sort(cs.begin(), cs.end());
-                    for (int32_t c : cs) { P.ci.push_back(c); P.v.push_back(accv[c]); }
-                    P.rp[r + 1] = (int64_t)P.ci.size();
+                };
+#pragma omp parallel
+                {
+                    std::vector<double> accv(T.nc, 0.0);
+                    std::vector<int64_t> mk(T.nc, -1);
+                    std::vector<int32_t> cs;
+#pragma omp for schedule(dynamic, 512)
+                    for (int64_t r = 0; r < P.nr; ++r) {
+                        row(r, accv, mk, cs);
+                        P.rp[r + 1] = (int64_t)cs.size();
+                    }
+                }
+                for (int64_t r = 0; r < P.nr; ++r) P.rp[r + 1] += P.rp[r];
+                P.ci.resize(P.rp[P.nr]);
+                P.v.resize(P.rp[P.nr]);
+#pragma omp parallel
+                {
+                    std::vector<double> accv(T.nc, 0.0);
+                    std::vector<int64_t> mk(T.nc, -1);
+                    std::vector<int32_t> cs;
+#pragma omp for schedule(dynamic, 512)
+                    for (int64_t r = 0; r < P.nr; ++r) {
+                        row(r, accv, mk, cs);
+                        int64_t o = P.rp[r];
+                        for (int32_t c : cs) {
+                            P.ci[o] = c;
+                            P.v[o++] = accv[c];
+                        }
+                    }
                 }
             }
             clk.lap(6);
@@ -901,9 +1028,9 @@ struct Amg : ssr_precond {
         int rc = SSR_OK;
         if (rebuild || refresh_now) {
             clk.lap(11);
-            Kh.resize((size_t)nnzb * dim * dim);
-            SSR_HIP_CHECK(hipMemcpyAsync(Kh.data(), Kvals, sizeof(double) * Kh.size(), hipMemcpyDeviceToHost,
-                                         ssr_stream()));
+            if ((rc = Kh.reserve((size_t)nnzb * dim * dim))) return rc;
+            SSR_HIP_CHECK(hipMemcpyAsync(Kh.data(), Kvals, sizeof(double) * (size_t)nnzb * dim * dim,
+                                         hipMemcpyDeviceToHost, ssr_stream()));
             SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
             clk.lap(0);
             rc = rebuild ? build() : refresh();
