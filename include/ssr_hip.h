@@ -159,8 +159,32 @@ int ssr_pattern_info(const ssr_pattern *p, int64_t *info);
 /* host tables: 0 row_ptr, 1 col, 2 element block map, 3 transpose block, 4 diagonal
  * block, 5 reference-order positions (int64), 6 free mask (uint8) */
 int ssr_pattern_get(const ssr_pattern *p, int which, void *dst);
-/* device tables: 0 row_ptr, 1 col, 2 emap, 3 tblk, 4 diag, 5 elem, 6 ref_pos, 7 mask, 8 brow */
+/* device tables: 0 row_ptr, 1 col, 2 emap, 3 tblk, 4 diag, 5 elem, 6 ref_pos, 7 mask, 8 brow,
+ * 9 upper blocks, 10/11 slot-list bounds and element pair slots, 12/13 per-block contribution
+ * lists, 14/15 per-node element incidences (NULL when the table was not built) */
 int ssr_pattern_dev_ptr(const ssr_pattern *p, int which, void **out);
+/* The same pattern built on the device (CONSTITUTIVE.init_K_r_pattern, CONSTITUTIVE.m:393-464,
+ * and build_scatter_map, :582-635): every table equal to ssr_pattern_create's.  DEVICE
+ * pointers: elem int32 n_p x n_e (0-based), Q uint8 dim x n_n (nonzero = free). */
+int ssr_pattern_create_dev(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int32_t *elem,
+                           const uint8_t *Q, ssr_pattern **out);
+
+/* ---- FE setup on the device ---------------------------------------------- */
+/* DPhi1..3 (n_p x n_int point-major) and WEIGHT (1 x n_int) of P2 elements from
+ * coord (dim x n_n node-major) and elem (int32 n_p x n_e): elastic_stiffness_matrix_3D.m:60-96,148
+ * (2D: elastic_stiffness_matrix_2D.m:50-80,118) with the P2 basis and quadrature of
+ * local_basis_volume_{3D,2D}.m and quadrature_volume_{3D,2D}.m.  DPhi3 unused in 2D. */
+int ssr_dev_fe_gradients(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const double *coord,
+                         const int32_t *elem, double *DPhi1, double *DPhi2, double *DPhi3, double *WEIGHT);
+/* f_V (dim x n_n node-major) = vector_volume_{3D,2D}.m: f_int is f_rows x n_int point-major with
+ * f_rows = dim, or f_rows = 1 for a density acting along -y (f_V_int = [0; -gamma(; 0)]). */
+int ssr_dev_volume_force(const ssr_pattern *P, int n_q, const double *WEIGHT, const double *f_int,
+                         int f_rows, double *f);
+/* Per-point fields from a per-element material id (heterogenous_materials.m:35-87):
+ * out[k][p] = table[mat_id[p / n_q] * n_fields + k]; out is a HOST array of n_fields DEVICE
+ * pointers (NULL entries skipped). */
+int ssr_dev_materials(int64_t n_e, int n_q, int n_mat, int n_fields, const double *table,
+                      const int32_t *mat_id, double *const *out);
 
 /* sum_e sum_q B'(w D)B into the block values (zeroed first); ds_layout 0 = MEX
  * layout (ns^2 per point), 1 = packed symmetric (ns(ns+1)/2 per point).
@@ -301,7 +325,9 @@ int ssr_amg_info(int64_t *count, int64_t *n_total);
 /* ---- Newton step of the indirect SSR continuation ---------------------------- */
 /* The device state of the CONSTITUTIVE handle class (CONSTITUTIVE.m) that the
  * Newton loop touches.  All pointers are device memory owned by the caller
- * except phi_host/psi_host (host, n_int each, read only for Davis C). */
+ * except phi_host/psi_host (host, read only for Davis C): n_friction entries
+ * each -- the (phi, psi) of the distinct materials -- or n_int entries when
+ * n_friction is 0. */
 typedef struct ssr_constitutive {
     int dim;     /* 2 or 3 */
     int n_q;     /* integration points per element; n_int = n_e * n_q */
@@ -321,6 +347,7 @@ typedef struct ssr_constitutive {
     double *K_tang, *K_r;                     /* block values, overwritten (K_tang: unused by the loops) */
     const double *mask;                       /* dim*n_n: 1 at free DOFs, 0 at Dirichlet DOFs */
     const double *f;                          /* dim*n_n load vector */
+    int64_t n_friction;                       /* entries of phi_host, psi_host (0: n_int) */
 } ssr_constitutive;
 
 #define SSR_NEWTON_UNKNOWN 0

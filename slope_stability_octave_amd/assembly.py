@@ -94,6 +94,16 @@ class Pattern:
             _lib.require_gpu()
         h = _vp(0)
         call("ssr_pattern_create", dim, n_n, n_p, n_e, ptr(elem), ptr(Qm), 1 if device else 0, ctypes.byref(h))
+        self._adopt(h)
+
+    @classmethod
+    def from_handle(cls, h):
+        """Wrap a pattern the library created (e.g. ssr_pattern_create_dev)."""
+        self = object.__new__(cls)
+        self._adopt(h)
+        return self
+
+    def _adopt(self, h):
         self.handle = h
         info = np.zeros(8, dtype=np.int64)
         call("ssr_pattern_info", h, ptr(info))

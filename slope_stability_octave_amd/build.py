@@ -33,20 +33,28 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
 
+def _compile(src: str, verbose: bool) -> str:
+    obj = os.path.join(CSRC, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", src, "-o", obj, "-I", CSRC]
+    if src.endswith(".cpp"):
+        # host-side drivers (AMG setup uses OpenMP threads)
+        cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-fopenmp", "-x", "hip", "-c", src, "-o", obj,
+                                                               "-I", CSRC]
+    if verbose:
+        print(" ".join(cmd), file=sys.stderr)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every source into libssr_hip.so (always recompiles)."""
-    objs = []
-    for src in sources():
-        obj = os.path.join(CSRC, os.path.basename(src) + ".o")
-        cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", src, "-o", obj, "-I", CSRC]
-        if src.endswith(".cpp"):
-            # host-side drivers (AMG setup uses OpenMP threads)
-            cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-fopenmp", "-x", "hip", "-c", src, "-o", obj,
-                                                                   "-I", CSRC]
-        if verbose:
-            print(" ".join(cmd), file=sys.stderr)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    """Compile every source into libssr_hip.so (always recompiles; the
+    translation units are compiled concurrently)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    srcs = sources()
+    jobs = max(1, min(len(srcs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs
     subprocess.run(cmd, check=True)

@@ -22,7 +22,12 @@ from ._lib import DeviceArray
 
 
 class CONSTITUTIVE:
-    def __init__(self, elem, DPhi, c0, phi, psi, Davis_type, shear, bulk, lame, WEIGHT, n_strain, n_int, dim):
+    def __init__(self, elem, DPhi, c0, phi, psi, Davis_type, shear, bulk, lame, WEIGHT, n_strain, n_int, dim,
+                 friction_pairs=None):
+        """Host arrays are uploaded; DeviceArrays (the device setup path) are
+        used in place -- DPhi then in the device layout (n_p values of a
+        point contiguous), and ``friction_pairs`` = (phi, psi) of the
+        distinct materials, which the Davis C test needs on the host."""
         self.dim = int(dim)
         self.n_strain = int(n_strain)
         self.n_int = int(n_int)
@@ -31,13 +36,26 @@ class CONSTITUTIVE:
         self.n_p, self.n_e = self.elem.shape
         self.n_q = self.n_int // self.n_e
         self.n_n = int(self.elem.max()) + 1
-        self._c0, self._phi, self._psi = (np.asarray(x, dtype=np.float64) for x in (c0, phi, psi))
-        self.c0, self.phi, self.psi = (DeviceArray.from_host(x) for x in (self._c0, self._phi, self._psi))
-        self.shear, self.bulk, self.lame = (DeviceArray.from_host(np.asarray(x, dtype=np.float64))
-                                            for x in (shear, bulk, lame))
-        self.WEIGHT = DeviceArray.from_host(np.asarray(WEIGHT, dtype=np.float64).ravel())
+
+        def dev(x):
+            return x if isinstance(x, DeviceArray) else DeviceArray.from_host(np.asarray(x, dtype=np.float64).ravel())
+
+        if friction_pairs is None:
+            if isinstance(phi, DeviceArray) or isinstance(psi, DeviceArray):
+                raise ValueError("device material fields need friction_pairs=(phi, psi) of the materials")
+            # the distinct (phi, psi) pairs: Davis C's all-points test
+            # (reduction.m:47) gives the same answer on them
+            pairs = np.unique(np.stack([np.asarray(phi, dtype=np.float64).ravel(),
+                                        np.asarray(psi, dtype=np.float64).ravel()]), axis=1)
+            friction_pairs = (pairs[0], pairs[1])
+        self._phi_u, self._psi_u = (np.asarray(x, dtype=np.float64) for x in friction_pairs)
+        self._c_all = {}
+        self.c0, self.phi, self.psi = (dev(x) for x in (c0, phi, psi))
+        self.shear, self.bulk, self.lame = (dev(x) for x in (shear, bulk, lame))
+        self.WEIGHT = dev(WEIGHT)
         # n_p x n_int, Octave column-major: the n_p values of a point contiguous
-        self.DPhi = [DeviceArray.from_host(np.asarray(d, dtype=np.float64).ravel(order="F")) for d in DPhi]
+        self.DPhi = [d if isinstance(d, DeviceArray) else
+                     DeviceArray.from_host(np.asarray(d, dtype=np.float64).ravel(order="F")) for d in DPhi]
         self.c_bar = DeviceArray(self.n_int)
         self.sin_phi = DeviceArray(self.n_int)
         self._lambda = DeviceArray(1)
@@ -61,7 +79,10 @@ class CONSTITUTIVE:
         t0 = time.perf_counter()
         c_all = True
         if self.Davis_type == "C":
-            c_all = bool(np.all(np.arctan(np.tan(self._phi) / lam) > self._psi))
+            c_all = self._c_all.get(float(lam))
+            if c_all is None:
+                c_all = bool(np.all(np.arctan(np.tan(self._phi_u) / lam) > self._psi_u))
+                self._c_all[float(lam)] = c_all
         self._lambda.set(np.array([float(lam)]))
         CM.reduction_dev(self.n_int, self.Davis_type, self.c0, self.phi, self.psi, self._lambda, False,
                          self.c_bar, self.sin_phi, davis_c_all=c_all)
@@ -133,12 +154,13 @@ class CONSTITUTIVE:
             raise RuntimeError("init_K_r_pattern(Q) must be called first")
         return self.pattern
 
-    def init_K_r_pattern(self, Q):
+    def init_K_r_pattern(self, Q, pattern=None):
         """Node-block pattern over all nodes with the Dirichlet mask of Q and
-        the elastic values V_elast (CONSTITUTIVE.m:393-464)."""
+        the elastic values V_elast (CONSTITUTIVE.m:393-464).  ``pattern``: one
+        already built (on the device) for this mesh and Q."""
         t0 = time.perf_counter()
         self.Q = np.asarray(Q, dtype=bool)
-        self.pattern = A.Pattern(self.elem, self.Q, self.dim)
+        self.pattern = pattern if pattern is not None else A.Pattern(self.elem, self.Q, self.dim)
         P = self.pattern
         self.mask = P.d_mask
         ns = self.n_strain

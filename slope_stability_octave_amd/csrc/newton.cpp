@@ -52,8 +52,9 @@ int reduction(const NewtonCtx &x, double lam)
     int c_all = 1;
     if (C->davis == 'C') {
         // Octave's vector `if` of reduction.m:47 holds only when every point has
-        // atan(tan(phi)/lambda) > psi
-        for (int64_t i = 0; i < C->n_int; ++i)
+        // atan(tan(phi)/lambda) > psi; the distinct materials decide it
+        const int64_t n_host = C->n_friction > 0 ? C->n_friction : C->n_int;
+        for (int64_t i = 0; i < n_host; ++i)
             if (!(atan(tan(C->phi_host[i]) / lam) > C->psi_host[i])) {
                 c_all = 0;
                 break;

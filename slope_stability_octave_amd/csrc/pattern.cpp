@@ -22,41 +22,8 @@
 #include <algorithm>
 #include <vector>
 
+#include "pattern.h"
 #include "ssr_common.h"
-
-struct ssr_pattern {
-    int dim = 0;
-    int64_t n_n = 0, n_e = 0, n_p = 0;
-    int64_t nnzb = 0;      // number of blocks
-    int64_t n_free = 0;    // free DOFs
-    int64_t nnz_free = 0;  // entries of K(Q,Q)
-    // host copies
-    std::vector<int32_t> row_ptr, col, emap, tblk, diag, brow;
-    std::vector<int32_t> ublk;  // blocks on or above the block diagonal (row <= column), row-major order
-    std::vector<int64_t> ref_pos;
-    std::vector<uint8_t> mask;  // dim*n_n, 1 = free
-    // device copies
-    int32_t *d_row_ptr = nullptr, *d_col = nullptr, *d_emap = nullptr, *d_tblk = nullptr,
-            *d_diag = nullptr, *d_elem = nullptr, *d_brow = nullptr, *d_ublk = nullptr;
-    int64_t *d_ref_pos = nullptr;
-    double *d_mask = nullptr;  // dim*n_n, 1.0 free / 0.0 Dirichlet
-    // gather-form assembly: the element blocks that sum into each block, in
-    // element order, as code = (e * n_pair + pair) * 2 + transposed
-    std::vector<int32_t> cptr, ccode;
-    int32_t *d_cptr = nullptr, *d_ccode = nullptr;
-    // slot form: every element pair block has one slot in the contribution
-    // list of its upper block (row <= column); lists in ublk order, elements
-    // ascending inside a list.  pslot: n_e * n_pair slot indices, uptr: list
-    // bounds per upper block
-    std::vector<int32_t> uptr, pslot;
-    int32_t *d_uptr = nullptr, *d_pslot = nullptr;
-    mutable double *d_keblk = nullptr;  // n_e * n_pair * dim * dim element blocks (allocated on first use)
-    // gather-form internal force: the element nodes (code = e * n_p + a) that
-    // sum into each node, in element order
-    std::vector<int32_t> nptr, ncode;
-    int32_t *d_nptr = nullptr, *d_ncode = nullptr;
-    mutable double *d_fe = nullptr;  // n_e * n_p * dim element-node forces (allocated on first use)
-};
 
 namespace {
 template <class T>
@@ -173,6 +140,7 @@ int ssr_pattern_create(int dim, int64_t n_n, int64_t n_p, int64_t n_e, const int
     for (int64_t r = 0; r < n_n; ++r)
         for (int32_t k = p->row_ptr[r]; k < p->row_ptr[r + 1]; ++k)
             if (p->col[k] >= r) p->ublk.push_back(k);
+    p->n_ublk = (int64_t)p->ublk.size();
     p->emap.resize(n_e * n_p * n_p);
     for (int64_t e = 0; e < n_e; ++e)
         for (int64_t a = 0; a < n_p; ++a)
@@ -301,6 +269,25 @@ int ssr_pattern_info(const ssr_pattern *p, int64_t *info /* 8 */)
 int ssr_pattern_get(const ssr_pattern *p, int which, void *dst)
 {
     SSR_REQUIRE(p && dst, "pattern_get: null");
+    if (p->device_built && which != 6) {
+        // tables built on the device: copy the device table out
+        const void *src = nullptr;
+        size_t bytes = 0;
+        switch (which) {
+        case 0: src = p->d_row_ptr; bytes = sizeof(int32_t) * (p->n_n + 1); break;
+        case 1: src = p->d_col; bytes = sizeof(int32_t) * p->nnzb; break;
+        case 2: src = p->d_emap; bytes = sizeof(int32_t) * p->n_e * p->n_p * p->n_p; break;
+        case 3: src = p->d_tblk; bytes = sizeof(int32_t) * p->nnzb; break;
+        case 4: src = p->d_diag; bytes = sizeof(int32_t) * p->n_n; break;
+        case 5: src = p->d_ref_pos; bytes = sizeof(int64_t) * p->nnz_free; break;
+        default: SSR_REQUIRE(false, "pattern_get: bad table %d", which);
+        }
+        if (bytes) {
+            SSR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ssr_stream()));
+            SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+        }
+        return SSR_OK;
+    }
     switch (which) {
     case 0: memcpy(dst, p->row_ptr.data(), sizeof(int32_t) * p->row_ptr.size()); break;
     case 1: memcpy(dst, p->col.data(), sizeof(int32_t) * p->col.size()); break;
@@ -372,7 +359,7 @@ const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p) { return p->d_tblk; }
 const int32_t *ssr_pattern_d_ublk(const ssr_pattern *p) { return p->d_ublk; }
 const int32_t *ssr_pattern_d_uptr(const ssr_pattern *p) { return p->d_uptr; }
 const int32_t *ssr_pattern_d_pslot(const ssr_pattern *p) { return p->d_pslot; }
-int64_t ssr_pattern_n_ublk(const ssr_pattern *p) { return (int64_t)p->ublk.size(); }
+int64_t ssr_pattern_n_ublk(const ssr_pattern *p) { return p->n_ublk; }
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p) { return p->d_diag; }
 const int32_t *ssr_pattern_d_elem(const ssr_pattern *p) { return p->d_elem; }
 const int32_t *ssr_pattern_d_brow(const ssr_pattern *p) { return p->d_brow; }
@@ -414,7 +401,8 @@ void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_
 }
 
 // Device table addresses: which = 0 row_ptr, 1 col, 2 emap, 3 tblk, 4 diag,
-// 5 elem (int32 n_p x n_e), 6 ref_pos, 7 mask (double), 8 brow.
+// 5 elem (int32 n_p x n_e), 6 ref_pos, 7 mask (double), 8 brow, 9 ublk,
+// 10 uptr, 11 pslot, 12 cptr, 13 ccode, 14 nptr, 15 ncode (NULL when absent).
 extern "C" int ssr_pattern_dev_ptr(const ssr_pattern *p, int which, void **out)
 {
     SSR_REQUIRE(p && out, "pattern_dev_ptr: null");
@@ -428,6 +416,13 @@ extern "C" int ssr_pattern_dev_ptr(const ssr_pattern *p, int which, void **out)
     case 6: *out = p->d_ref_pos; break;
     case 7: *out = p->d_mask; break;
     case 8: *out = p->d_brow; break;
+    case 9: *out = p->d_ublk; break;
+    case 10: *out = p->d_uptr; break;
+    case 11: *out = p->d_pslot; break;
+    case 12: *out = p->d_cptr; break;
+    case 13: *out = p->d_ccode; break;
+    case 14: *out = p->d_nptr; break;
+    case 15: *out = p->d_ncode; break;
     default: SSR_REQUIRE(false, "pattern_dev_ptr: bad table %d", which);
     }
     return SSR_OK;

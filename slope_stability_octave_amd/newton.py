@@ -58,7 +58,7 @@ class _Constitutive(ctypes.Structure):
                 ("c0", _vp), ("phi", _vp), ("psi", _vp), ("shear", _vp), ("bulk", _vp), ("lame", _vp),
                 ("phi_host", _vp), ("psi_host", _vp), ("lambda_", _vp), ("c_bar", _vp), ("sin_phi", _vp),
                 ("E", _vp), ("S", _vp), ("DS", _vp), ("K_elast", _vp), ("K_tang", _vp), ("K_r", _vp),
-                ("mask", _vp), ("f", _vp)]
+                ("mask", _vp), ("f", _vp), ("n_friction", _i64)]
 
 
 def native_description(cmb, f):
@@ -77,10 +77,11 @@ def native_description(cmb, f):
         raise ValueError("native Newton driver: DPhi or f too small for the mesh")
     host = getattr(cmb, "_native_host", None)
     if host is None:
-        host = cmb._native_host = (np.ascontiguousarray(cmb._phi, dtype=np.float64).ravel(),
-                                   np.ascontiguousarray(cmb._psi, dtype=np.float64).ravel())
-    if host[0].size != n or host[1].size != n:
-        raise ValueError("native Newton driver: phi and psi must be given per integration point")
+        # the (phi, psi) of the distinct materials: the Davis C test needs no more
+        host = cmb._native_host = (np.ascontiguousarray(cmb._phi_u, dtype=np.float64).ravel(),
+                                   np.ascontiguousarray(cmb._psi_u, dtype=np.float64).ravel())
+    if host[0].size != host[1].size or host[0].size == 0:
+        raise ValueError("native Newton driver: phi and psi of the materials must pair up")
     d = _Constitutive()
     d.dim, d.n_q, d.davis, d.s_rows, d.n_int = cmb.dim, cmb.n_q, ord(cmb.Davis_type), cmb.s_rows, cmb.n_int
     d.pattern = cmb.pattern.handle.value
@@ -91,6 +92,7 @@ def native_description(cmb, f):
         setattr(d, name, ptr(getattr(cmb, name)).value)
     d.lambda_ = ptr(cmb._lambda).value
     d.phi_host, d.psi_host = host[0].ctypes.data, host[1].ctypes.data
+    d.n_friction = host[0].size
     d.f = ptr(f).value
     return d
 

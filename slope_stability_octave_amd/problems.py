@@ -37,7 +37,10 @@ HETERO_3D_LAYERS = [dict(c0=50, phi=40, psi=0, young=100000, poisson=0.3, gamma_
 HETERO_3D_LEVELS = [7.5, 15.0, 22.5]
 
 
-def _finish(coord, elem, Q, dim, mats, mat_id):
+def _finish(coord, elem, Q, dim, mats, mat_id, device=False):
+    if device:
+        from .device_setup import finish_device
+        return finish_device(coord, elem, Q, dim, mats, mat_id)
     DPhi, WEIGHT, HatP, WF = fem.gradients(coord, elem, dim)
     n_q = WF.size
     m = fem.materials_per_point(mat_id, n_q, mats)
@@ -49,15 +52,16 @@ def _finish(coord, elem, Q, dim, mats, mat_id):
                 mats=m, f=f, n_int=n_int, n_dof_free=int(Q.sum()))
 
 
-def slope_2d(h=0.5, x1=15.0, x2=10.0, x3=15.0, y1=10.0, y2=10.0, materials=None):
+def slope_2d(h=0.5, x1=15.0, x2=10.0, x3=15.0, y1=10.0, y2=10.0, materials=None, device=False):
     """2D homogeneous slope (10 m foundation, 10 m high 1:1 slope, 15 m crest
-    and toe berms by default)."""
+    and toe berms by default).  ``device``: gradients, fields, load and
+    pattern built on the GPU (device_setup.finish_device)."""
     coord, elem, Q = mesh.mesh_P2_2D(h, x1, x2, x3, y1, y2)
     mats = materials or HOMO_2D
-    return _finish(coord, elem, Q, 2, mats, np.zeros(elem.shape[1], dtype=np.int64))
+    return _finish(coord, elem, Q, 2, mats, np.zeros(elem.shape[1], dtype=np.int64), device=device)
 
 
-def slope_3d(counts, x1, x2, x3, y1, y2, z, materials, layers=None):
+def slope_3d(counts, x1, x2, x3, y1, y2, z, materials, layers=None, device=False):
     """3D slope from mesh_P2_3D with cell counts (nx1, nx2, nx3, ny1, ny2, nz).
     ``layers``: list of y-levels splitting the body into horizontal layers
     (material id 0 at the bottom), assigned by element centroid."""
@@ -67,17 +71,27 @@ def slope_3d(counts, x1, x2, x3, y1, y2, z, materials, layers=None):
         yc = coord[1, elem[:4]].mean(axis=0)
         for lv in sorted(layers):
             mat_id += (yc > lv)
-    return _finish(coord, elem, Q, 3, materials, mat_id)
+    return _finish(coord, elem, Q, 3, materials, mat_id, device=device)
 
 
 def make_constitutive(prob, Davis_type="B"):
     from .constitutive import CONSTITUTIVE
     m = prob["mats"]
     dim = prob["dim"]
+    pairs = None
+    if prob.get("device"):
+        tab = prob["material_table"]
+        pairs = (tab[:, 1], tab[:, 2])
     cmb = CONSTITUTIVE(prob["elem"], prob["DPhi"], m["c0"], m["phi"], m["psi"], Davis_type, m["shear"], m["bulk"],
-                       m["lame"], prob["WEIGHT"], 6 if dim == 3 else 3, prob["n_int"], dim)
-    cmb.init_K_r_pattern(prob["Q"])
+                       m["lame"], prob["WEIGHT"], 6 if dim == 3 else 3, prob["n_int"], dim, friction_pairs=pairs)
+    cmb.init_K_r_pattern(prob["Q"], pattern=prob.get("pattern"))
     return cmb
+
+
+def load_vector(prob):
+    """f as a device field (n_n, dim)."""
+    f = prob["f"]
+    return f if isinstance(f, DeviceArray) else DeviceArray.from_host(f.T)
 
 
 def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", log=None, predictor=None,
@@ -100,7 +114,7 @@ def run_ssr(prob, params=None, solver="dfgmres", preconditioner="block_jacobi", 
         amg_options.setdefault("coord", prob["coord"])
     lin = cls(cmb.pattern, p["linear_tol"], p["linear_maxit"], p["deflation_tol"], preconditioner,
               amg_options=amg_options, precision=precision)
-    f = DeviceArray.from_host(prob["f"].T)
+    f = load_vector(prob)
     if predictor is not None and getattr(predictor, "is_factory", False):
         predictor = predictor(cmb, f)
     if recorder is not None and getattr(recorder, "is_factory", False):
@@ -170,7 +184,7 @@ def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_option
         amg_options.setdefault("coord", prob["coord"])
     lin = DFGMRES(cmb.pattern, p["linear_tol"], p["linear_maxit"], p["deflation_tol"], preconditioner,
                   amg_options=amg_options, precision=precision)
-    f = DeviceArray.from_host(prob["f"].T)
+    f = load_vector(prob)
     t_setup = time.perf_counter() - t0
     t1 = time.perf_counter()
     U, lam_h, om_h, umax_h, work_h = SSR_direct_continuation(
@@ -247,7 +261,7 @@ def run_ll(prob, params=None, preconditioner="amg", log=None, amg_options=None, 
         amg_options.setdefault("coord", prob["coord"])
     lin = DFGMRES(cmb.pattern, p["linear_tol"], p["linear_maxit"], p["deflation_tol"], preconditioner,
                   amg_options=amg_options, precision=precision)
-    f = DeviceArray.from_host(prob["f"].T)
+    f = load_vector(prob)
     cmb.reduction(1.0)
     # elastic solve (the reference solves it with its Krylov solver; here to elastic_tol)
     el = DFGMRES(cmb.pattern, elastic_tol, 1000, p["deflation_tol"], preconditioner, amg_options=amg_options)

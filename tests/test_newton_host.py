@@ -54,7 +54,7 @@ def _fake_cmb(dim=2, n_n=10, n_e=3, n_q=7, n_p=6, nnzb=40, short=None):
     cmb = types.SimpleNamespace(dim=dim, n_q=n_q, n_int=n_int, n_p=n_p, n_strain=ns, s_rows=s_rows, Davis_type="B",
                                 pattern=pat, pattern_initialized=True, mask=_Dev(dim * n_n),
                                 DPhi=[_Dev(n_p * n_int) for _ in range(dim)], _lambda=_Dev(1),
-                                _phi=np.full(n_int, 0.5), _psi=np.zeros(n_int))
+                                _phi_u=np.array([0.5]), _psi_u=np.zeros(1))
     for k, v in sizes.items():
         setattr(cmb, k, _Dev(v))
     return cmb
@@ -65,7 +65,7 @@ def test_description_of_a_consistent_problem():
     d = newton.native_description(cmb, _Dev(cmb.pattern.n_dof, 0x3000))
     assert (d.dim, d.n_q, d.davis, d.s_rows, d.n_int) == (2, 7, ord("B"), 4, 21)
     assert d.pattern == 0x2000 and d.f == 0x3000 and not d.DPhi3
-    assert d.phi_host == cmb._native_host[0].ctypes.data
+    assert d.phi_host == cmb._native_host[0].ctypes.data and d.n_friction == 1
 
 
 @pytest.mark.parametrize("short", ["E", "DS", "K_r", "c_bar", "WEIGHT"])
@@ -75,13 +75,13 @@ def test_description_refuses_short_arrays(short):
         newton.native_description(cmb, _Dev(cmb.pattern.n_dof))
 
 
-def test_description_refuses_short_load_and_scalar_materials():
+def test_description_refuses_short_load_and_unpaired_materials():
     cmb = _fake_cmb()
     with pytest.raises(ValueError, match="DPhi or f"):
         newton.native_description(cmb, _Dev(cmb.pattern.n_dof - 1))
     cmb = _fake_cmb()
-    cmb._phi = np.array([0.5])
-    with pytest.raises(ValueError, match="per integration point"):
+    cmb._phi_u = np.array([0.5, 0.6])
+    with pytest.raises(ValueError, match="must pair up"):
         newton.native_description(cmb, _Dev(cmb.pattern.n_dof))
 
 
