@@ -127,7 +127,9 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         LDS-staged point data);
  *  key 9: coarse AMG operators, prolongators and restrictions as block CSR
  *         over whole nodes / aggregates (1, default) or scalar CSR (0);
- *         applies to hierarchies built after the call. */
+ *         applies to hierarchies built after the call;
+ *  key 10: AMG hierarchy built on the device (1, default; the host build is
+ *         used when an aggregate loses a near-null vector) or on the host (0). */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 

@@ -201,6 +201,7 @@ TUNE_SPMM_GROUP = 6
 TUNE_SPMV_UNROLL = 7
 TUNE_ELEMENT_KERNEL = 8
 TUNE_AMG_COARSE_BLOCKS = 9
+TUNE_AMG_DEVICE_SETUP = 10
 
 
 def set_tuning(key: int, value: int):

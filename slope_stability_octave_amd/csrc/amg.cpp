@@ -31,6 +31,7 @@
 #include <memory>
 #include <vector>
 
+#include "amg_device.h"
 #include "linalg.h"
 #include "ssr_common.h"
 
@@ -43,6 +44,7 @@ void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_
                        int64_t *nnz_free);
 
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_brow(const ssr_pattern *p);
 
 extern "C" {
 int ssr_pattern_get(const ssr_pattern *p, int which, void *dst);
@@ -305,6 +307,45 @@ struct DCsr {
         nblk = nb;
         return SSR_OK;
     }
+    // take ownership of device block arrays built elsewhere
+    void adopt(int r_b, int c_b, int64_t n_br, int64_t n_blk, int32_t *d_brp, int32_t *d_bcol, T *d_bval)
+    {
+        release();
+        rb = r_b;
+        cb = c_b;
+        nbr = n_br;
+        nblk = n_blk;
+        nr = n_br * r_b;
+        nnz = n_blk * r_b * c_b;
+        brp = d_brp;
+        bcol = d_bcol;
+        bval = d_bval;
+    }
+    // a copy in precision T of a block-form matrix
+    template <class S>
+    int mirror(const DCsr<S> &src)
+    {
+        release();
+        if (!src.rb) {
+            ssr_set_error("AMG: float mirror of a scalar-CSR level");
+            return SSR_ERR_STATE;
+        }
+        SSR_HIP_CHECK(hipMalloc((void **)&brp, sizeof(int32_t) * (src.nbr + 1)));
+        SSR_HIP_CHECK(hipMalloc((void **)&bcol, sizeof(int32_t) * (src.nblk > 0 ? src.nblk : 1)));
+        SSR_HIP_CHECK(hipMalloc((void **)&bval, sizeof(T) * (src.nblk > 0 ? src.nblk : 1) * src.rb * src.cb));
+        SSR_HIP_CHECK(hipMemcpyAsync(brp, src.brp, sizeof(int32_t) * (src.nbr + 1), hipMemcpyDeviceToDevice,
+                                     ssr_stream()));
+        if (src.nblk)
+            SSR_HIP_CHECK(hipMemcpyAsync(bcol, src.bcol, sizeof(int32_t) * src.nblk, hipMemcpyDeviceToDevice,
+                                         ssr_stream()));
+        rb = src.rb;
+        cb = src.cb;
+        nbr = src.nbr;
+        nblk = src.nblk;
+        nr = src.nr;
+        nnz = src.nnz;
+        return src.nblk ? ssr::convert<S, T>(src.nblk * src.rb * src.cb, 1.0, src.bval, bval) : SSR_OK;
+    }
     int spmv(const T *x, T *y, double a, double b, const T *z = nullptr) const
     {
         if (rb) return ssr::spmv_brc<T>(rb, cb, nbr, brp, bcol, bval, x, y, a, b, z, nblk);
@@ -469,6 +510,9 @@ struct Amg : ssr_precond {
     int64_t n_setup = 0;
     int64_t generation = 0;  // bumped whenever device level data is (re)uploaded
     bool built = false;
+    bool on_device = false;  // the hierarchy was built on the device (amg_device.hip)
+    double *d_B = nullptr;   // device copy of the near-null space (device build)
+    double dev_phase[8] = {0};
     bool force_rebuild = false;  // set by invalidate(): rebuild at the next setup
 
     Amg(const ssr_pattern *P, const double *near_null, int n_null, const Options &o) : Pat(P), opt(o)
@@ -495,6 +539,7 @@ struct Amg : ssr_precond {
         (void)hipFree(d_coarse_inv32);
         (void)hipFree(binv64);
         (void)hipFree(binv32);
+        (void)hipFree(d_B);
         vg64.reset();
         vg32.reset();
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -640,37 +685,7 @@ struct Amg : ssr_precond {
                 }
             }
         }
-        std::vector<int32_t> agg(nn, -1);
-        n_agg = 0;
-        // pass 1: roots whose strong neighbours are all free
-        for (int64_t i = 0; i < nn; ++i) {
-            if (agg[i] >= 0 || sp[i + 1] == sp[i]) continue;
-            bool ok = true;
-            for (int64_t t = sp[i]; t < sp[i + 1] && ok; ++t) ok = agg[sc[t]] < 0;
-            if (!ok) continue;
-            agg[i] = n_agg;
-            for (int64_t t = sp[i]; t < sp[i + 1]; ++t) agg[sc[t]] = n_agg;
-            n_agg++;
-        }
-        // pass 2: attach to a neighbouring aggregate (strongest connection)
-        std::vector<int32_t> agg2 = agg;
-        for (int64_t i = 0; i < nn; ++i) {
-            if (agg[i] >= 0) continue;
-            int32_t best = -1;
-            for (int64_t t = sp[i]; t < sp[i + 1]; ++t)
-                if (agg[sc[t]] >= 0) { best = agg[sc[t]]; break; }
-            agg2[i] = best;
-        }
-        agg.swap(agg2);
-        // pass 3: leftovers with strong neighbours form new aggregates
-        for (int64_t i = 0; i < nn; ++i) {
-            if (agg[i] >= 0 || sp[i + 1] == sp[i]) continue;
-            agg[i] = n_agg;
-            for (int64_t t = sp[i]; t < sp[i + 1]; ++t)
-                if (agg[sc[t]] < 0) agg[sc[t]] = n_agg;
-            n_agg++;
-        }
-        return agg;
+        return ssr_amg::aggregate_graph(nn, sp, sc, n_agg);
     }
 
     // Tentative prolongator of level lv given aggregates; returns coarse
@@ -810,6 +825,159 @@ struct Amg : ssr_precond {
     }
 
     PhaseClock clk;
+
+    // float copy of a hierarchy built on the device
+    int mirror_float(size_t l)
+    {
+        generation++;
+        Level &lv = *L[l];
+        DevLevel<double> &d = lv.d64;
+        DevLevel<float> &f = lv.d32;
+        int rc;
+        if (l > 0 && (rc = f.A.mirror(d.A))) return rc;
+        if (l + 1 < L.size() && ((rc = f.P.mirror(d.P)) || (rc = f.R.mirror(d.R)))) return rc;
+        (void)hipFree(f.dinv);
+        f.dinv = nullptr;
+        SSR_HIP_CHECK(hipMalloc((void **)&f.dinv, sizeof(float) * (lv.A.nr > 0 ? lv.A.nr : 1)));
+        if ((rc = ssr::convert<double, float>(lv.A.nr, 1.0, d.dinv, f.dinv))) return rc;
+        return alloc_level_vectors<float>(lv);
+    }
+
+    // The hierarchy on the device (amg_device.hip): the same algorithm and
+    // aggregates as build(), every level in block CSR.  *fallback is set
+    // when an aggregate loses a near-null vector (non-uniform coarse
+    // unknowns) or the problem is too small for a second level; build() then
+    // runs instead.
+    int build_device(bool *fallback)
+    {
+        *fallback = false;
+        const auto t_start = std::chrono::steady_clock::now();
+        int rc;
+        if (!d_B) {
+            SSR_HIP_CHECK(hipMalloc((void **)&d_B, sizeof(double) * B.size()));
+            SSR_HIP_CHECK(hipMemcpyAsync(d_B, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice,
+                                         ssr_stream()));
+        }
+        std::vector<ssr_amg::DevLevelOut> out;
+        ssr_amg::DevOptions o;
+        o.theta = opt.theta;
+        o.max_levels = opt.max_levels;
+        o.max_coarse = opt.max_coarse;
+        rc = ssr_amg::amg_device_build(dim, n_n, ssr_pattern_d_row_ptr(Pat), ssr_pattern_d_col(Pat),
+                                       ssr_pattern_d_brow(Pat), ssr_pattern_d_diag(Pat), Kdev, row_ptr, col, d_B, k,
+                                       o, out, dev_phase);
+        if (rc == ssr_amg::kNotUniform || (rc == SSR_OK && out.size() < 2)) {
+            for (auto &l : out) l.release();
+            *fallback = true;
+            return SSR_OK;
+        }
+        if (rc) return rc;
+        L.clear();
+        for (size_t l = 0; l < out.size(); ++l) {
+            ssr_amg::DevLevelOut &o_l = out[l];
+            L.emplace_back(new Level());
+            Level &lv = *L.back();
+            const int64_t nl = (int64_t)o_l.rb * o_l.n_nodes;
+            lv.A.nr = lv.A.nc = nl;
+            lv.rho = o_l.rho;
+            lv.rho_up = o_l.rho_up;
+            lv.node_ptr.resize(o_l.n_nodes + 1);
+            for (int64_t i = 0; i <= o_l.n_nodes; ++i) lv.node_ptr[i] = i * o_l.rb;
+            DevLevel<double> &dl = lv.d64;
+            if (l > 0) {
+                dl.A.adopt(o_l.A.rb, o_l.A.cb, o_l.A.nbr, o_l.A.nblk, o_l.A.brp, o_l.A.bcol, o_l.A.bval);
+                (void)hipFree(o_l.A.brow);
+                o_l.A = ssr_amg::DevBlocks();
+            }
+            if (l + 1 < out.size()) {
+                dl.P.adopt(o_l.P.rb, o_l.P.cb, o_l.P.nbr, o_l.P.nblk, o_l.P.brp, o_l.P.bcol, o_l.P.bval);
+                dl.R.adopt(o_l.R.rb, o_l.R.cb, o_l.R.nbr, o_l.R.nblk, o_l.R.brp, o_l.R.bcol, o_l.R.bval);
+                (void)hipFree(o_l.P.brow);
+                (void)hipFree(o_l.R.brow);
+                o_l.P = ssr_amg::DevBlocks();
+                o_l.R = ssr_amg::DevBlocks();
+            }
+            (void)hipFree(dl.dinv);
+            dl.dinv = o_l.dinv;
+            o_l.dinv = nullptr;
+            (void)hipFree(o_l.diag);
+            o_l.diag = nullptr;
+            if ((rc = alloc_level_vectors<double>(lv))) return rc;
+        }
+        generation++;
+        // the coarsest operator on the host for the dense inverse
+        {
+            Level &c = *L.back();
+            const DCsr<double> &Ad = c.d64.A;
+            const int rbk = Ad.rb, cbk = Ad.cb;
+            std::vector<int32_t> brp(Ad.nbr + 1), bcol(Ad.nblk);
+            std::vector<double> bval((size_t)Ad.nblk * rbk * cbk);
+            SSR_HIP_CHECK(hipMemcpyAsync(brp.data(), Ad.brp, sizeof(int32_t) * brp.size(), hipMemcpyDeviceToHost,
+                                         ssr_stream()));
+            SSR_HIP_CHECK(hipMemcpyAsync(bcol.data(), Ad.bcol, sizeof(int32_t) * bcol.size(), hipMemcpyDeviceToHost,
+                                         ssr_stream()));
+            SSR_HIP_CHECK(hipMemcpyAsync(bval.data(), Ad.bval, sizeof(double) * bval.size(), hipMemcpyDeviceToHost,
+                                         ssr_stream()));
+            SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+            HCsr &A = c.A;
+            A.nr = A.nc = Ad.nbr * rbk;
+            A.rp.assign(A.nr + 1, 0);
+            A.ci.clear();
+            A.v.clear();
+            for (int64_t I = 0; I < Ad.nbr; ++I)
+                for (int r = 0; r < rbk; ++r) {
+                    for (int32_t t = brp[I]; t < brp[I + 1]; ++t)
+                        for (int q = 0; q < cbk; ++q) {
+                            A.ci.push_back(bcol[t] * cbk + q);
+                            A.v.push_back(bval[((size_t)t * rbk + r) * cbk + q]);
+                        }
+                    A.rp[I * rbk + r + 1] = (int64_t)A.ci.size();
+                }
+        }
+        if (block0 &&
+            (rc = ssr_amg::fine_block_gershgorin(dim, n_n, ssr_pattern_d_row_ptr(Pat), ssr_pattern_d_diag(Pat), Kdev,
+                                                 &rho_up_block)))
+            return rc;
+        if (want32)
+            for (size_t l = 0; l < L.size(); ++l)
+                if ((rc = mirror_float(l))) return rc;
+        if ((rc = build_coarse())) return rc;
+        built = true;
+        on_device = true;
+        if (getenv("SSR_AMG_VERBOSE")) {
+            for (size_t l = 0; l < L.size(); ++l)
+                fprintf(stderr, "[amg] device level %zu: n=%lld blocks=%lld rho=%.6g rho_up=%.6g\n", l,
+                        (long long)L[l]->A.nr, (long long)(l ? L[l]->d64.A.nblk : (int64_t)col.size()), L[l]->rho,
+                        L[l]->rho_up);
+            const double tot =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+            fprintf(stderr,
+                    "[amg] device setup %.3f s (cumulative phases: dinv+rho %.3f, strength+aggregation %.3f, "
+                    "tentative %.3f, P %.3f, AP %.3f, R %.3f, RAP %.3f)\n",
+                    tot, dev_phase[0], dev_phase[1], dev_phase[2], dev_phase[3], dev_phase[4], dev_phase[5],
+                    dev_phase[6]);
+        }
+        return SSR_OK;
+    }
+
+    // build on the device when enabled, else (or when the device build
+    // declines) on the host from a downloaded copy of the operator
+    int build_any()
+    {
+        if (ssr_tuning(SSR_TUNE_AMG_DEVICE_SETUP) && ssr_tuning(SSR_TUNE_AMG_COARSE_BLOCKS)) {
+            bool fallback = false;
+            int rc = build_device(&fallback);
+            if (rc || !fallback) return rc;
+        }
+        on_device = false;
+        int rc;
+        if ((rc = Kh.reserve((size_t)nnzb * dim * dim))) return rc;
+        SSR_HIP_CHECK(hipMemcpyAsync(Kh.data(), Kdev, sizeof(double) * (size_t)nnzb * dim * dim,
+                                     hipMemcpyDeviceToHost, ssr_stream()));
+        SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+        clk.lap(0);
+        return build();
+    }
 
     int build()
     {
@@ -1026,14 +1194,17 @@ struct Amg : ssr_precond {
         force_rebuild = false;
         const bool refresh_now = !rebuild && opt.refresh_every > 0 && n_setup % opt.refresh_every == 0;
         int rc = SSR_OK;
-        if (rebuild || refresh_now) {
+        if (rebuild || (refresh_now && on_device)) {
+            clk.lap(11);
+            rc = build_any();
+        } else if (refresh_now) {
             clk.lap(11);
             if ((rc = Kh.reserve((size_t)nnzb * dim * dim))) return rc;
             SSR_HIP_CHECK(hipMemcpyAsync(Kh.data(), Kvals, sizeof(double) * (size_t)nnzb * dim * dim,
                                          hipMemcpyDeviceToHost, ssr_stream()));
             SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
             clk.lap(0);
-            rc = rebuild ? build() : refresh();
+            rc = refresh();
         }
         if (!rc && block0) {
             const int64_t nb = n_n * dim * dim;
@@ -1051,7 +1222,7 @@ struct Amg : ssr_precond {
         }
         if (!(rebuild || refresh_now) && need32) {
             // first float use of an existing hierarchy
-            for (size_t l = 0; l < L.size() && !rc; ++l) rc = upload_level<float>(l);
+            for (size_t l = 0; l < L.size() && !rc; ++l) rc = on_device ? mirror_float(l) : upload_level<float>(l);
             if (!rc) {
                 std::vector<float> ci32(coarse_inv.begin(), coarse_inv.end());
                 rc = dupload(&d_coarse_inv32, ci32);

@@ -30,10 +30,13 @@
 
 #include <vector>
 
+#include "devutil.h"
 #include "pattern.h"
 #include "ssr_common.h"
 
 extern "C" int ssr_pattern_destroy(ssr_pattern *p);
+
+using namespace ssr_dev;
 
 namespace {
 
@@ -413,101 +416,6 @@ __global__ void k_ref_fill(int dim, int64_t n_n, const int32_t *__restrict__ row
             if (Q[(int64_t)dim * rn + ri]) ref_pos[o++] = (int64_t)blk * bs + ri * dim + cj;
     }
 }
-
-// ---- helpers ------------------------------------------------------------------
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() { (void)hipFree(p); }
-    int alloc(size_t bytes)
-    {
-        (void)hipFree(p);
-        p = nullptr;
-        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
-            ssr_set_error("hipMalloc(%zu) failed in the device setup", bytes);
-            return SSR_ERR_ALLOC;
-        }
-        return SSR_OK;
-    }
-    template <class T>
-    T *as() const
-    {
-        return (T *)p;
-    }
-    void *release()
-    {
-        void *q = p;
-        p = nullptr;
-        return q;
-    }
-};
-
-int bits_for(uint64_t maxval)
-{
-    int b = 1;
-    while (b < 64 && (maxval >> b) != 0) ++b;
-    return b;
-}
-
-template <class T>
-int exclusive_sum(const T *in, T *out, int64_t n, hipStream_t s)
-{
-    size_t bytes = 0;
-    SSR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
-    DevBuf tmp;
-    int rc = tmp.alloc(bytes);
-    if (rc) return rc;
-    SSR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, (int)n, s));
-    return SSR_OK;
-}
-
-template <class K, class V>
-int sort_pairs(const K *kin, K *kout, const V *vin, V *vout, int64_t n, int end_bit, hipStream_t s)
-{
-    size_t bytes = 0;
-    SSR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
-    DevBuf tmp;
-    int rc = tmp.alloc(bytes);
-    if (rc) return rc;
-    SSR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
-    return SSR_OK;
-}
-
-template <class K>
-int sort_keys(const K *kin, K *kout, int64_t n, int end_bit, hipStream_t s)
-{
-    size_t bytes = 0;
-    SSR_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, kin, kout, (int)n, 0, end_bit, s));
-    DevBuf tmp;
-    int rc = tmp.alloc(bytes);
-    if (rc) return rc;
-    SSR_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, kin, kout, (int)n, 0, end_bit, s));
-    return SSR_OK;
-}
-
-template <class T>
-int read_scalar(const T *dev, T *host, hipStream_t s)
-{
-    SSR_HIP_CHECK(hipMemcpyAsync(host, dev, sizeof(T), hipMemcpyDeviceToHost, s));
-    SSR_HIP_CHECK(hipStreamSynchronize(s));
-    return SSR_OK;
-}
-
-// counts (n + 1 entries, the last zero on entry) -> exclusive prefix sums in place
-// through a second buffer; returns the total
-int counts_to_ptr(DevBuf &cnt, int64_t n, DevBuf &ptr, int32_t *total, hipStream_t s)
-{
-    int rc;
-    if ((rc = ptr.alloc(sizeof(int32_t) * (n + 1))) || (rc = exclusive_sum(cnt.as<int32_t>(), ptr.as<int32_t>(), n + 1, s)) ||
-        (rc = read_scalar(ptr.as<int32_t>() + n, total, s)))
-        return rc;
-    return SSR_OK;
-}
-
-#define SSR_LAUNCH(kernel, n, ...)                                                             \
-    do {                                                                                       \
-        if ((n) > 0) kernel<<<ssr_blocks((n), 256), 256, 0, s>>>(__VA_ARGS__);                 \
-        SSR_HIP_CHECK(hipGetLastError());                                                      \
-    } while (0)
 
 }  // namespace
 

@@ -81,5 +81,6 @@ struct ssr_precond {
                                            2 = per-lane products from LDS-staged point data (k_assemble_pairs) */
 #define SSR_TUNE_AMG_COARSE_BLOCKS 9     /* 1: coarse AMG operators and transfers as block CSR over whole nodes
                                            (default), 0: scalar CSR */
-#define SSR_TUNE_COUNT 10
+#define SSR_TUNE_AMG_DEVICE_SETUP 10     /* 1: build the AMG hierarchy on the device (default), 0: on the host */
+#define SSR_TUNE_COUNT 11
 int ssr_tuning(int key);
