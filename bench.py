@@ -320,9 +320,10 @@ def run_fos(args):
             log(f"building the mesh and FE data (cells {counts})")
         if hetero:
             prob = problems.slope_3d(counts, 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HETERO_3D_LAYERS,
-                                     layers=problems.HETERO_3D_LEVELS)
+                                     layers=problems.HETERO_3D_LEVELS, device=not args.host_setup)
         else:
-            prob = problems.slope_3d(counts, 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HOMO_2D)
+            prob = problems.slope_3d(counts, 20.0, 20.0, 20.0, 10.0, 20.0, 20.0, problems.HOMO_2D,
+                                     device=not args.host_setup)
         if log:
             log(f"mesh ready: {prob['n_dof_free']} free DOFs, {prob['n_int']} integration points")
         params = dict(problems.PARAMS_3D)
@@ -337,7 +338,7 @@ def run_fos(args):
             params["omega_max_stop"] = args.omega_max
         name = f"2D slope family seed {args.seed} (angle, c, phi = {tuple(round(x, 2) for x in mat)}), P2 h={args.h}"
     else:
-        prob = problems.slope_2d(h=args.h)
+        prob = problems.slope_2d(h=args.h, device=not args.host_setup)
         params = dict(problems.PARAMS_2D)
         params["lambda_init"] = args.lambda_init if args.lambda_init is not None else 1.0
         params["omega_max_stop"] = args.omega_max
@@ -370,6 +371,7 @@ def run_fos(args):
         "lambda_hist": [float(x) for x in res["lambda_hist"]],
         "omega_hist": [float(x) for x in res["omega_hist"]],
         "setup_s": res["setup_s"],
+        "device_setup_s": prob.get("setup_s"),
         "timings_s": res["timings"],
         "counts": res["counts"],
     }
@@ -618,6 +620,9 @@ def main():
                     help="rebuild the AMG hierarchy every n-th Newton iteration (0: once per continuation step)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of known size first (counter calibration)")
+    ap.add_argument("--host-setup", action="store_true",
+                    help="FoS workloads: FE data, material fields, load and pattern from the host restatements "
+                         "instead of the device setup kernels")
     ap.add_argument("--precision", type=int, default=64, choices=[32, 64],
                     help="Krylov/preconditioner arithmetic of the FoS workloads (32: float with double refinement)")
     ap.add_argument("--amg-theta", type=float, default=None, help="AMG strength threshold (FoS workloads)")

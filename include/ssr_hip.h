@@ -129,7 +129,10 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         over whole nodes / aggregates (1, default) or scalar CSR (0);
  *         applies to hierarchies built after the call;
  *  key 10: AMG hierarchy built on the device (1, default; the host build is
- *         used when an aggregate loses a near-null vector) or on the host (0). */
+ *         used when an aggregate loses a near-null vector) or on the host (0);
+ *  key 11: node-block SpMV with its block values staged through LDS by
+ *         16-byte loads (1; for the default lane shapes) or read by each
+ *         lane (0, default). */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 

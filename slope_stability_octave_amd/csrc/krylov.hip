@@ -113,6 +113,127 @@ __global__ __launch_bounds__(256) void k_spmv_bsr(int64_t n_rows, const int32_t 
     }
 }
 
+// The same product with the block values staged through LDS.  Lanes that
+// each read a whole 72-byte block issue nine 8-byte loads whose addresses
+// stride by the block size, so every load instruction touches about as many
+// cache lines as it has lanes and the vector L1 serves each line nine times.
+// Here the one wavefront of a workgroup first copies the value span of its
+// rows (contiguous in the block array) into LDS with 16-byte loads that
+// consecutive lanes issue for consecutive addresses, then forms the same
+// products as k_spmv_bsr from LDS, in the same order (the same bits).  A span
+// longer than CAP blocks is read from global memory directly.
+// The products of one lane of k_spmv_bsr over a block row, reading block b
+// at v[(b - vb0) * dim^2] (the lane's sum in k_spmv_bsr's order).
+template <int DIM, int G, class T, int U>
+__device__ __forceinline__ void bsr_row_products(int32_t b0, int32_t b1, int l, const int32_t *__restrict__ col,
+                                                 const T *__restrict__ x, const T *__restrict__ v, int32_t vb0,
+                                                 T acc[DIM])
+{
+    constexpr int DD = DIM * DIM;
+    if (U == 1) {
+        for (int32_t b = b0 + l; b < b1; b += G) {
+            const int64_t c = col[b];
+            T xv[DIM];
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) xv[j] = x[c * DIM + j];
+            const T *vv = v + (int64_t)(b - vb0) * DD;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) acc[i] += vv[i * DIM + j] * xv[j];
+        }
+    } else {
+        for (int32_t b = b0 + l; b < b1; b += 2 * G) {
+            const int32_t bb = b + G;
+            const bool two = bb < b1;
+            const int64_t c0 = col[b];
+            const int64_t c1 = two ? (int64_t)col[bb] : c0;
+            T v0[DD], v1[DD], x0[DIM], x1[DIM];
+#pragma unroll
+            for (int e = 0; e < DD; ++e) v0[e] = v[(int64_t)(b - vb0) * DD + e];
+#pragma unroll
+            for (int e = 0; e < DD; ++e) v1[e] = two ? v[(int64_t)(bb - vb0) * DD + e] : T(0);
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) x0[j] = x[c0 * DIM + j];
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) x1[j] = x[c1 * DIM + j];
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) acc[i] += v0[i * DIM + j] * x0[j] + v1[i * DIM + j] * x1[j];
+        }
+    }
+}
+
+template <int DIM, int G, class T, int U, int CAP>
+__global__ __launch_bounds__(64) void k_spmv_bsr_lds(int64_t n_rows, const int32_t *__restrict__ row_ptr,
+                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                     const T *__restrict__ x, T *y, double alpha, double beta,
+                                                     const T *z)
+{
+    constexpr int DD = DIM * DIM, RW = 64 / G;
+    constexpr int BYTES = CAP * DD * (int)sizeof(T);
+    __shared__ __align__(16) unsigned char sh[BYTES + 16];
+    const int lane = threadIdx.x;
+    const int64_t r_first = (int64_t)blockIdx.x * RW;
+    const int64_t r = r_first + lane / G;
+    const int l = lane % G;
+    const int64_t r_last = r_first + RW < n_rows ? r_first + RW : n_rows;
+    const int32_t bw0 = row_ptr[r_first], bw1 = row_ptr[r_last];
+    const bool staged = bw1 - bw0 <= CAP;
+    int mis = 0;
+    if (staged) {
+        const unsigned char *base = (const unsigned char *)val;
+        const int64_t s0 = (int64_t)bw0 * DD * (int64_t)sizeof(T), s1 = (int64_t)bw1 * DD * (int64_t)sizeof(T);
+        const int64_t a0 = s0 & ~(int64_t)15;
+        mis = (int)(s0 - a0);
+        const int n16 = (int)((s1 - a0) >> 4);
+        typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+        // every load of the span is issued before the first LDS write
+        constexpr int NCH = (BYTES + 16 + 16 * 64 - 1) / (16 * 64);
+        u32x4 buf[NCH];
+        const u32x4 *src = (const u32x4 *)(base + a0);
+#pragma unroll
+        for (int t = 0; t < NCH; ++t) {
+            const int c = lane + 64 * t;
+            if (c < n16) buf[t] = __builtin_nontemporal_load(src + c);
+        }
+#pragma unroll
+        for (int t = 0; t < NCH; ++t) {
+            const int c = lane + 64 * t;
+            if (c < n16) *(u32x4 *)(sh + 16 * c) = buf[t];
+        }
+        const int tail = (int)((s1 - a0) & 15);
+        if (lane < tail / 4)
+            *(uint32_t *)(sh + 16 * n16 + 4 * lane) = *(const uint32_t *)(base + a0 + 16 * n16 + 4 * lane);
+    }
+    __syncthreads();  // one wavefront per workgroup: its LDS writes are visible to all its lanes
+    T acc[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) acc[i] = T(0);
+    if (r < n_rows) {
+        const int32_t b0 = row_ptr[r], b1 = row_ptr[r + 1];
+        // staged block b sits at LDS offset (b - bw0) blocks: small positive
+        // offsets from the array (an LDS pointer displaced by -bw0 blocks
+        // wraps its 32-bit offset and leaves the LDS aperture when added back)
+        if (staged)
+            bsr_row_products<DIM, G, T, U>(b0, b1, l, col, x, (const T *)(sh + mis), bw0, acc);
+        else
+            bsr_row_products<DIM, G, T, U>(b0, b1, l, col, x, val, 0, acc);
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, G);
+    if (r < n_rows && l == 0) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            const T o = T(alpha) * acc[i];
+            y[r * DIM + i] = beta == 0.0 ? o : o + T(beta) * z[r * DIM + i];
+        }
+    }
+}
+
 // The same product with one wavefront per block row and lanes over the
 // row's values: lane l owns entry q = l mod dim^2 of block l / dim^2 in each
 // pass, so every pass reads 64 / dim^2 consecutive blocks (63 of 64 lanes in
@@ -806,6 +927,17 @@ int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
     if (U == 0) U = 1;
     const int64_t threads = n_rows * G;
     const dim3 grid(ssr_blocks(threads, 256)), blk(256);
+    if (ssr_tuning(SSR_TUNE_SPMV_STAGE) && ((dim == 3 && G == 16 && U == 2) || (dim == 2 && G == 4 && U == 1))) {
+        const dim3 gw(ssr_blocks(threads, 64)), bw(64);
+        if (dim == 3)
+            hipLaunchKernelGGL((k_spmv_bsr_lds<3, 16, T, 2, 192>), gw, bw, 0, ssr_stream(), n_rows, row_ptr, col, val,
+                               x, y, alpha, beta, z);
+        else
+            hipLaunchKernelGGL((k_spmv_bsr_lds<2, 4, T, 1, 320>), gw, bw, 0, ssr_stream(), n_rows, row_ptr, col, val,
+                               x, y, alpha, beta, z);
+        SSR_HIP_CHECK(hipGetLastError());
+        return SSR_OK;
+    }
 #define SPMV(D_, G_)                                                                                          \
     do {                                                                                                      \
         if (U == 2)                                                                                           \
@@ -869,6 +1001,9 @@ int spmv_brc(int rb, int cb, int64_t n_rows, const int32_t *rp, const int32_t *c
     if (!z) z = y;
     const int64_t avg = nblk / n_rows;
     int G = avg <= 6 ? 4 : avg <= 12 ? 8 : avg <= 40 ? 16 : avg <= 96 ? 32 : 64;
+    // few long rows (the first coarse AMG level: some 10^4 aggregate rows of
+    // about 70 blocks): more lanes per row until the grid fills the device
+    while (G < 64 && n_rows * G < (int64_t)ssr_cu_count() * 2048 && avg > G / 2) G *= 2;
     const dim3 blk(256);
 #define BRC(RB_, CB_)                                                                                              \
     do {                                                                                                           \

@@ -82,5 +82,7 @@ struct ssr_precond {
 #define SSR_TUNE_AMG_COARSE_BLOCKS 9     /* 1: coarse AMG operators and transfers as block CSR over whole nodes
                                            (default), 0: scalar CSR */
 #define SSR_TUNE_AMG_DEVICE_SETUP 10     /* 1: build the AMG hierarchy on the device (default), 0: on the host */
-#define SSR_TUNE_COUNT 11
+#define SSR_TUNE_SPMV_STAGE 11          /* 1: node-block SpMV with the block values staged through LDS,
+                                           0: read from global memory by each lane (default) */
+#define SSR_TUNE_COUNT 12
 int ssr_tuning(int key);

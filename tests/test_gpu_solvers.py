@@ -73,6 +73,27 @@ def test_spmv_matches_scipy(dim, group):
     assert not np.any(yfull[np.setdiff1d(np.arange(yfull.size), qf)])
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_spmv_lds_staged_form_equals_direct(dim):
+    """The SpMV whose block values go through LDS gives the same bits as the
+    direct form (the same products in the same order), on a mesh with rows
+    of every length."""
+    cmb, K, Kq, qf = operator(dim, counts=(3, 2, 3, 2, 3, 2))
+    rng = np.random.default_rng(4)
+    x = DeviceArray.from_host(rng.normal(size=(cmb.n_n, dim)))
+    out = {}
+    old = _lib.get_tuning(_lib.TUNE_SPMV_STAGE)
+    try:
+        for stage in (0, 1):
+            _lib.set_tuning(_lib.TUNE_SPMV_STAGE, stage)
+            y = DeviceArray((cmb.n_n, dim))
+            spmv(cmb.pattern, K, x, y)
+            out[stage] = y.get()
+    finally:
+        _lib.set_tuning(_lib.TUNE_SPMV_STAGE, old)
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("cls", [DFGMRES, DCG])
 def test_krylov_solves_to_tolerance(cls):
     cmb, K, Kq, qf = operator(2, r=1.0)
@@ -112,12 +133,24 @@ def test_A_orthogonalize_and_deflated_solve():
     assert np.linalg.norm(x.get().ravel()[qf] - vecs[2]) <= 1e-6 * np.linalg.norm(vecs[2])
 
 
-def test_A_orthogonalize_large_basis_matches_oracle():
+@pytest.mark.parametrize("spmm_group", [0, 32])
+def test_A_orthogonalize_large_basis_matches_oracle(spmm_group):
     """72 columns (two dependent) on the 3D operator: two 64-column tiles of
-    the Gram product, a partial 8-column group of the multi-vector SpMV and
-    several row chunks; the basis must equal the oracle's column for column,
-    and a second pass must leave an A-orthonormal basis unchanged."""
+    the Gram product, a partial 8-column group of the multi-vector SpMV (two
+    lane shapes) and several row chunks; the basis must equal the oracle's
+    column for column, and a second pass must leave an A-orthonormal basis
+    unchanged."""
     from oracle import krylov_oracle as KO
+    from slope_stability_octave_amd import _lib
+    old = _lib.get_tuning(_lib.TUNE_SPMM_GROUP)
+    _lib.set_tuning(_lib.TUNE_SPMM_GROUP, spmm_group)
+    try:
+        _check_A_orthogonalize_large_basis(KO)
+    finally:
+        _lib.set_tuning(_lib.TUNE_SPMM_GROUP, old)
+
+
+def _check_A_orthogonalize_large_basis(KO):
     cmb, K, Kq, qf = operator(3, counts=(2, 2, 2, 2, 2, 2))
     rng = np.random.default_rng(5)
     n = cmb.pattern.n_dof
