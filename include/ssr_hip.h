@@ -222,6 +222,13 @@ int ssr_dev_ds_elastic(int dim, int64_t n_int, const double *shear, const double
 /* y = alpha K x + beta y on the node-block matrix (the librsb SpMV of newton_ind_SSR.m:91) */
 int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col,
                      const double *val, const double *x, double *y, double alpha, double beta);
+/* The same product on float block values and vectors (the operator copy of
+ * the float Krylov iterations; sums in float), and the double-to-float copy
+ * that produces such an operator: no reference counterpart (the reference
+ * runs in double only). */
+int ssr_dev_spmv_bsr_f32(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col,
+                         const float *val, const float *x, float *y, double alpha, double beta);
+int ssr_dev_convert_f64_f32(int64_t n, const double *x, float *y);
 int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result);
 int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result);
 int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h);

@@ -1322,6 +1322,14 @@ int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int3
     return ssr::spmv_bsr<double>(dim, n_rows, row_ptr, col, val, x, y, alpha, beta);
 }
 
+int ssr_dev_spmv_bsr_f32(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const float *val,
+                         const float *x, float *y, double alpha, double beta)
+{
+    return ssr::spmv_bsr<float>(dim, n_rows, row_ptr, col, val, x, y, alpha, beta);
+}
+
+int ssr_dev_convert_f64_f32(int64_t n, const double *x, float *y) { return ssr::convert<double, float>(n, 1.0, x, y); }
+
 // result[0] = sum_i a_i b_i (m_i) written to DEVICE memory
 int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result)
 {

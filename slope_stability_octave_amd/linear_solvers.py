@@ -44,6 +44,8 @@ _lib.extend_signatures({
     "ssr_solver_get_precision": [_vp, _vp],
     "ssr_dev_spmv_bsr": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
     "ssr_dev_dot": [_i64, _vp, _vp, _vp, _vp],
+    "ssr_dev_spmv_bsr_f32": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
+    "ssr_dev_convert_f64_f32": [_i64, _vp, _vp],
     "ssr_dot": [_i64, _vp, _vp, _vp, _vp],
     "ssr_dev_multidot": [_i64, _int, _vp, _i64, _vp, _vp],
     "ssr_dev_multidot_noalloc": [_i64, _int, _vp, _i64, _vp, _vp],

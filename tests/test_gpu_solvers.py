@@ -74,6 +74,26 @@ def test_spmv_matches_scipy(dim, group):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+def test_float_spmv_matches_scipy(dim):
+    """The float operator copy (ssr_dev_convert_f64_f32) and the float SpMV
+    against scipy in double: within float rounding of the row sums."""
+    cmb, K, Kq, qf = operator(dim, counts=(2, 2, 2, 2, 2, 2))
+    P = cmb.pattern
+    nv = P.nnzb * P.bs
+    K32 = DeviceArray(nv, dtype=np.float32)
+    _lib.call("ssr_dev_convert_f64_f32", nv, _lib.ptr(K), _lib.ptr(K32))
+    rng = np.random.default_rng(7)
+    xq = rng.normal(size=qf.size)
+    x = DeviceArray.from_host(_full(xq, qf, P.n_dof, dim).astype(np.float32))
+    y = DeviceArray((cmb.n_n, dim), dtype=np.float32)
+    _lib.call("ssr_dev_spmv_bsr_f32", dim, P.n_n, _lib.ptr(P.d_row_ptr), _lib.ptr(P.d_col), _lib.ptr(K32),
+              _lib.ptr(x), _lib.ptr(y), 1.0, 0.0)
+    ref = Kq @ xq.astype(np.float32).astype(np.float64)
+    yq = y.get().astype(np.float64).ravel()[qf]
+    assert np.abs(yq - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_spmv_lds_staged_form_equals_direct(dim):
     """The SpMV whose block values go through LDS gives the same bits as the
     direct form (the same products in the same order), on a mesh with rows
