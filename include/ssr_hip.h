@@ -88,6 +88,14 @@ int ssr_dev_constitutive_3D(int64_t n_int, const double *E, const double *c_bar,
 int ssr_dev_constitutive_2D(int64_t n_int, const double *E, const double *c_bar,
                             const double *sin_phi, const double *shear, const double *bulk,
                             const double *lame, double *S, double *DS, int8_t *branch);
+/* The same with the five material fields per material (n_mat entries each): point p takes
+ * those of material mat_id[p / n_q], its element's.  With c_bar and sin_phi reduced per
+ * material (ssr_dev_reduction on the material table) this is CONSTITUTIVE.build_F_all's
+ * reduction (CONSTITUTIVE.m:318,332,472) folded into the return mapping's material loads
+ * for a scalar lambda: the same values at every point, so the same S and DS. */
+int ssr_dev_constitutive_mat(int dim, int64_t n_int, int n_q, const double *E, const int32_t *mat_id,
+                             int64_t n_mat, const double *c_bar, const double *sin_phi, const double *shear,
+                             const double *bulk, const double *lame, double *S, double *DS, int8_t *branch);
 
 /* [c_bar, sin_phi] = reduction(c0, phi, psi, lambda, Davis_type) per point
  * replaces +CONSTITUTIVE_PROBLEM/reduction.m:25-58 (called by CONSTITUTIVE.m:197).
@@ -360,6 +368,14 @@ typedef struct ssr_constitutive {
     const double *mask;                       /* dim*n_n: 1 at free DOFs, 0 at Dirichlet DOFs */
     const double *f;                          /* dim*n_n load vector */
     int64_t n_friction;                       /* entries of phi_host, psi_host (0: n_int) */
+    /* optional per-material fields (mat_id NULL: the per-point arrays above are used).  With
+     * them the reduction runs once per material and the return mappings read the material
+     * of each point's element; c_bar and sin_phi above are then not written. */
+    const int32_t *mat_id;                    /* n_int / n_q: material of each element */
+    int64_t n_mat;
+    const double *c0_m, *phi_m, *psi_m;       /* n_mat, unreduced */
+    const double *shear_m, *bulk_m, *lame_m;  /* n_mat */
+    double *c_bar_m, *sin_phi_m;              /* n_mat, reduced */
 } ssr_constitutive;
 
 #define SSR_NEWTON_UNKNOWN 0

@@ -56,6 +56,7 @@ SIGNATURES = {
     "ssr_constitutive_problem_2D_S": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_constitutive_3D": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_constitutive_2D": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "ssr_dev_constitutive_mat": [_int, _i64, _int, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_reduction": [_i64, _int, _vp, _vp, _vp, _vp, _int, _int, _vp, _vp],
     "ssr_dev_potential": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_set_tuning": [_int, _int],

@@ -23,11 +23,14 @@ from ._lib import DeviceArray
 
 class CONSTITUTIVE:
     def __init__(self, elem, DPhi, c0, phi, psi, Davis_type, shear, bulk, lame, WEIGHT, n_strain, n_int, dim,
-                 friction_pairs=None):
+                 friction_pairs=None, materials=None):
         """Host arrays are uploaded; DeviceArrays (the device setup path) are
         used in place -- DPhi then in the device layout (n_p values of a
         point contiguous), and ``friction_pairs`` = (phi, psi) of the
-        distinct materials, which the Davis C test needs on the host."""
+        distinct materials, which the Davis C test needs on the host.
+        ``materials`` = (mat_id per element, table n_mat x (c0, phi, psi,
+        shear, bulk, lame, ...)) describes the same per-point fields per
+        material (see set_material_table)."""
         self.dim = int(dim)
         self.n_strain = int(n_strain)
         self.n_int = int(n_int)
@@ -66,6 +69,9 @@ class CONSTITUTIVE:
         self.DS = DeviceArray((self.n_int, ns * ns))
         self.pattern = None
         self.pattern_initialized = False
+        self.material_fields = None
+        if materials is not None:
+            self.set_material_table(*materials)
         self.profiler = None
         self.timings = {}
         self.counts = {}
@@ -73,6 +79,25 @@ class CONSTITUTIVE:
     def _t(self, name, t0):
         self.timings[name] = self.timings.get(name, 0.0) + time.perf_counter() - t0
         self.counts[name] = self.counts.get(name, 0) + 1
+
+    def set_material_table(self, mat_id, table):
+        """The per-point material fields as a per-element material id and a
+        table (rows c0, phi, psi, shear, bulk, lame per material; the values
+        the per-point arrays hold).  The native Newton loops then reduce the
+        strength once per material and the return mappings read each point's
+        material from its element (the same values, so the same results)."""
+        mat_id = np.asarray(mat_id).ravel()
+        tab = np.asarray(table, dtype=np.float64)
+        if mat_id.size != self.n_e or tab.ndim != 2 or tab.shape[1] < 6:
+            raise ValueError("material table: one id per element and rows (c0, phi, psi, shear, bulk, lame)")
+        if mat_id.size and (mat_id.min() < 0 or mat_id.max() >= tab.shape[0]):
+            raise ValueError("material table: material id out of range")
+        n_mat = tab.shape[0]
+        mf = dict(mat_id=DeviceArray.from_host(mat_id.astype(np.int32)), n_mat=n_mat,
+                  c_bar=DeviceArray(n_mat), sin_phi=DeviceArray(n_mat))
+        for k, name in enumerate(("c0", "phi", "psi", "shear", "bulk", "lame")):
+            mf[name] = DeviceArray.from_host(np.ascontiguousarray(tab[:, k]))
+        self.material_fields = mf
 
     # -- material reduction (CONSTITUTIVE.m:178-200, reduction.m) -------------
     def reduction(self, lam):
@@ -86,6 +111,10 @@ class CONSTITUTIVE:
         self._lambda.set(np.array([float(lam)]))
         CM.reduction_dev(self.n_int, self.Davis_type, self.c0, self.phi, self.psi, self._lambda, False,
                          self.c_bar, self.sin_phi, davis_c_all=c_all)
+        mf = self.material_fields
+        if mf is not None:
+            CM.reduction_dev(mf["n_mat"], self.Davis_type, mf["c0"], mf["phi"], mf["psi"], self._lambda, False,
+                             mf["c_bar"], mf["sin_phi"], davis_c_all=c_all)
         self._t("CONSTITUTIVE.reduction", t0)
 
     # -- stress / tangent (CONSTITUTIVE.m:202-266) -----------------------------

@@ -93,6 +93,15 @@ def constitutive_dev(dim, n, E, c_bar, sin_phi, shear, bulk, lame, S, DS=None, b
          ptr(DS), ptr(branch))
 
 
+def constitutive_mat_dev(dim, n, n_q, E, mat_id, n_mat, c_bar, sin_phi, shear, bulk, lame, S, DS=None,
+                         branch=None):
+    """S (and DS) with per-material fields: point p takes material
+    mat_id[p // n_q] (device arrays)."""
+    call("ssr_dev_constitutive_mat", int(dim), int(n), int(n_q), ptr(E), ptr(mat_id), int(n_mat), ptr(c_bar),
+         ptr(sin_phi), ptr(shear), ptr(bulk), ptr(lame), ptr(S), ptr(DS) if DS is not None else None,
+         ptr(branch) if branch is not None else None)
+
+
 def reduction_dev(n, davis, c0, phi, psi, lam, lam_per_point, c_bar, sin_phi, davis_c_all=True):
     """Davis reduction (reduction.m:25-58) on device arrays; writes c_bar, sin_phi."""
     call("ssr_dev_reduction", int(n), ord(davis), ptr(c0), ptr(phi), ptr(psi), ptr(lam),

@@ -515,14 +515,19 @@ __global__ __launch_bounds__(64) void k_mc3d_x36(int64_t n, const double *__rest
 // the four material values) is issued up front, so a single memory latency
 // precedes the arithmetic and no LDS round trip or barrier is spent on E.
 // DS is staged in phases of H points.
-template <bool WITH_DS, bool NT, int H, int MINW = 1, bool SDIRECT = false>
+// MI: the material fields are per material, indexed by the material of the
+// point's element (mat_id[p / n_q]) -- the reduced strength of a scalar
+// lambda then costs one evaluation per material instead of a pass over the
+// points (the same values, so the same bits).
+template <bool WITH_DS, bool NT, int H, int MINW = 1, bool SDIRECT = false, bool MI = false>
 __global__ __launch_bounds__(64, MINW) void k_mc3d_x36d(int64_t n, const double *__restrict__ E,
                                                   const double *__restrict__ cbar,
                                                   const double *__restrict__ sphi,
                                                   const double *__restrict__ shear,
                                                   const double *__restrict__ lame,
                                                   double *__restrict__ S, double *__restrict__ DS,
-                                                  int8_t *__restrict__ branch)
+                                                  int8_t *__restrict__ branch,
+                                                  const int32_t *__restrict__ mat_id = nullptr, int n_q = 1)
 {
     constexpr int W = 64;
     static_assert(W % H == 0, "phase size");
@@ -536,10 +541,11 @@ __global__ __launch_bounds__(64, MINW) void k_mc3d_x36d(int64_t n, const double 
     if (lane < cnt) {
         const double2 *e2 = reinterpret_cast<const double2 *>(E + 6 * p);
         const double2 a = e2[0], b = e2[1], c = e2[2];
-        cb = cbar[p];
-        sp = sphi[p];
-        G = shear[p];
-        L = lame[p];
+        const int64_t m = MI ? (int64_t)mat_id[(uint32_t)p / (uint32_t)n_q] : p;
+        cb = cbar[m];
+        sp = sphi[m];
+        G = shear[m];
+        L = lame[m];
         eps[0] = a.x; eps[1] = a.y; eps[2] = b.x; eps[3] = b.y; eps[4] = c.x; eps[5] = c.y;
     }
     double s[6], d[21];
@@ -893,20 +899,22 @@ __host__ __device__ __forceinline__ int mc2d_point(const double eps[3], double c
 
 __host__ __device__ constexpr int sym3(int i, int j) { return i * 3 - (i * (i - 1)) / 2 + (j - i); }
 
-template <bool WITH_DS>
+template <bool WITH_DS, bool MI = false>
 __global__ __launch_bounds__(256) void k_mc2d_aos(int64_t n, const double *__restrict__ E,
                                                   const double *__restrict__ cbar,
                                                   const double *__restrict__ sphi,
                                                   const double *__restrict__ shear,
                                                   const double *__restrict__ lame,
                                                   double *__restrict__ S, double *__restrict__ DS,
-                                                  int8_t *__restrict__ branch)
+                                                  int8_t *__restrict__ branch,
+                                                  const int32_t *__restrict__ mat_id = nullptr, int n_q = 1)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     double eps[3] = {E[3 * p], E[3 * p + 1], E[3 * p + 2]};
     double s[4], d[6];
-    const int br = mc2d_point<WITH_DS>(eps, cbar[p], sphi[p], shear[p], lame[p], s, d);
+    const int64_t m = MI ? (int64_t)mat_id[(uint32_t)p / (uint32_t)n_q] : p;
+    const int br = mc2d_point<WITH_DS>(eps, cbar[m], sphi[m], shear[m], lame[m], s, d);
     double2 *s2 = reinterpret_cast<double2 *>(S + 4 * p);
     s2[0] = make_double2(s[0], s[1]);
     s2[1] = make_double2(s[2], s[3]);
@@ -1042,6 +1050,41 @@ extern "C" int ssr_dev_constitutive_3D(int64_t n, const double *E, const double 
         }
 #undef MCP
 #undef MC3
+    }
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+// The same return mappings with per-material fields: point p takes the
+// fields of material mat_id[p / n_q] (its element's).  3D runs the default
+// kernel form.
+extern "C" int ssr_dev_constitutive_mat(int dim, int64_t n, int n_q, const double *E, const int32_t *mat_id,
+                                        int64_t n_mat, const double *c_bar, const double *sin_phi,
+                                        const double *shear, const double *bulk, const double *lame, double *S,
+                                        double *DS, int8_t *branch)
+{
+    (void)bulk;
+    SSR_REQUIRE(dim == 2 || dim == 3, "constitutive_mat: dim must be 2 or 3");
+    SSR_REQUIRE(n >= 0 && n_q > 0 && n_mat > 0, "constitutive_mat: bad sizes");
+    SSR_REQUIRE(n < ((int64_t)1 << 32), "constitutive_mat: too many points for 32-bit element indices");
+    if (n == 0) return SSR_OK;
+    SSR_REQUIRE(E && mat_id && c_bar && sin_phi && shear && lame && S, "constitutive_mat: null argument");
+    if (dim == 3) {
+        const dim3 grid(ssr_blocks(n, 64)), blk(64);
+        if (DS)
+            hipLaunchKernelGGL((k_mc3d_x36d<true, true, 16, 1, false, true>), grid, blk, 0, ssr_stream(), n, E, c_bar,
+                               sin_phi, shear, lame, S, DS, branch, mat_id, n_q);
+        else
+            hipLaunchKernelGGL((k_mc3d_x36d<false, true, 16, 1, false, true>), grid, blk, 0, ssr_stream(), n, E,
+                               c_bar, sin_phi, shear, lame, S, DS, branch, mat_id, n_q);
+    } else {
+        const int th = 256;
+        if (DS)
+            hipLaunchKernelGGL((k_mc2d_aos<true, true>), dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n, E,
+                               c_bar, sin_phi, shear, lame, S, DS, branch, mat_id, n_q);
+        else
+            hipLaunchKernelGGL((k_mc2d_aos<false, true>), dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n, E,
+                               c_bar, sin_phi, shear, lame, S, DS, branch, mat_id, n_q);
     }
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;

@@ -63,6 +63,9 @@ int reduction(const NewtonCtx &x, double lam)
     // lambda reaches the device by a one-element fill on the stream: no host
     // synchronisation (a host copy would drain the queue)
     NCHECK(ssr_dev_fill(1, lam, C->lambda));
+    if (C->mat_id)  // one evaluation per material (the same values as per point)
+        return ssr_dev_reduction(C->n_mat, C->davis, C->c0_m, C->phi_m, C->psi_m, C->lambda, 0, c_all, C->c_bar_m,
+                                 C->sin_phi_m);
     return ssr_dev_reduction(C->n_int, C->davis, C->c0, C->phi, C->psi, C->lambda, 0, c_all, C->c_bar,
                              C->sin_phi);
 }
@@ -75,7 +78,11 @@ int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with
     const ssr_constitutive *C = x.C;
     if (reduce) NCHECK(reduction(x, lam));
     NCHECK(ssr_dev_strain(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, U, C->E));
-    if (C->dim == 3)
+    if (C->mat_id)
+        NCHECK(ssr_dev_constitutive_mat(C->dim, C->n_int, C->n_q, C->E, C->mat_id, C->n_mat, C->c_bar_m,
+                                        C->sin_phi_m, C->shear_m, C->bulk_m, C->lame_m, C->S,
+                                        with_ds ? C->DS : nullptr, nullptr));
+    else if (C->dim == 3)
         NCHECK(ssr_dev_constitutive_3D(C->n_int, C->E, C->c_bar, C->sin_phi, C->shear, C->bulk, C->lame, C->S,
                                        with_ds ? C->DS : nullptr, nullptr));
     else
@@ -174,6 +181,9 @@ int prepare(const ssr_constitutive *C, ssr_solver *solver, int method, int it_ne
                     C->E && C->S && C->DS && C->K_elast && C->K_tang && C->K_r && C->mask && C->f,
                 "newton: incomplete constitutive description");
     SSR_REQUIRE(C->dim == 2 || C->dim == 3, "newton: dim must be 2 or 3");
+    SSR_REQUIRE(!C->mat_id || (C->n_mat > 0 && C->c0_m && C->phi_m && C->psi_m && C->shear_m && C->bulk_m &&
+                               C->lame_m && C->c_bar_m && C->sin_phi_m),
+                "newton: incomplete per-material fields");
     SSR_REQUIRE(C->s_rows == (C->dim == 3 ? 6 : 4), "newton: S has 6 rows in 3D, 4 in 2D");
     SSR_REQUIRE(C->davis == 'A' || C->davis == 'B' || C->davis == 'C', "Incorrect choice of the Davis approach.");
     SSR_REQUIRE(C->davis != 'C' || (C->phi_host && C->psi_host), "newton: Davis C needs phi, psi on the host");

@@ -58,7 +58,9 @@ class _Constitutive(ctypes.Structure):
                 ("c0", _vp), ("phi", _vp), ("psi", _vp), ("shear", _vp), ("bulk", _vp), ("lame", _vp),
                 ("phi_host", _vp), ("psi_host", _vp), ("lambda_", _vp), ("c_bar", _vp), ("sin_phi", _vp),
                 ("E", _vp), ("S", _vp), ("DS", _vp), ("K_elast", _vp), ("K_tang", _vp), ("K_r", _vp),
-                ("mask", _vp), ("f", _vp), ("n_friction", _i64)]
+                ("mask", _vp), ("f", _vp), ("n_friction", _i64),
+                ("mat_id", _vp), ("n_mat", _i64), ("c0_m", _vp), ("phi_m", _vp), ("psi_m", _vp),
+                ("shear_m", _vp), ("bulk_m", _vp), ("lame_m", _vp), ("c_bar_m", _vp), ("sin_phi_m", _vp)]
 
 
 def native_description(cmb, f):
@@ -93,6 +95,13 @@ def native_description(cmb, f):
     d.lambda_ = ptr(cmb._lambda).value
     d.phi_host, d.psi_host = host[0].ctypes.data, host[1].ctypes.data
     d.n_friction = host[0].size
+    mt = getattr(cmb, "material_fields", None)
+    if mt is not None:
+        # per-material fields: one reduction per material, the kernels read each element's
+        d.mat_id = ptr(mt["mat_id"]).value
+        d.n_mat = mt["n_mat"]
+        for name in ("c0", "phi", "psi", "shear", "bulk", "lame", "c_bar", "sin_phi"):
+            setattr(d, name + "_m", ptr(mt[name]).value)
     d.f = ptr(f).value
     return d
 

@@ -48,8 +48,10 @@ def _finish(coord, elem, Q, dim, mats, mat_id, device=False):
     f_int = np.zeros((dim, n_int))
     f_int[1] = -m["gamma"]
     f = fem.volume_force(elem, coord.shape[1], HatP, WEIGHT, f_int)
+    from .device_setup import material_table
     return dict(coord=coord, elem=elem, Q=Q, dim=dim, DPhi=DPhi, WEIGHT=WEIGHT, HatP=HatP, n_q=n_q,
-                mats=m, f=f, n_int=n_int, n_dof_free=int(Q.sum()))
+                mats=m, f=f, n_int=n_int, n_dof_free=int(Q.sum()), material_table=material_table(mats),
+                mat_id=np.asarray(mat_id))
 
 
 def slope_2d(h=0.5, x1=15.0, x2=10.0, x3=15.0, y1=10.0, y2=10.0, materials=None, device=False):
@@ -82,8 +84,12 @@ def make_constitutive(prob, Davis_type="B"):
     if prob.get("device"):
         tab = prob["material_table"]
         pairs = (tab[:, 1], tab[:, 2])
+    materials = None
+    if prob.get("material_table") is not None and prob.get("mat_id") is not None:
+        materials = (prob["mat_id"], prob["material_table"])
     cmb = CONSTITUTIVE(prob["elem"], prob["DPhi"], m["c0"], m["phi"], m["psi"], Davis_type, m["shear"], m["bulk"],
-                       m["lame"], prob["WEIGHT"], 6 if dim == 3 else 3, prob["n_int"], dim, friction_pairs=pairs)
+                       m["lame"], prob["WEIGHT"], 6 if dim == 3 else 3, prob["n_int"], dim, friction_pairs=pairs,
+                       materials=materials)
     cmb.init_K_r_pattern(prob["Q"], pattern=prob.get("pattern"))
     return cmb
 

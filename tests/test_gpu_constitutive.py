@@ -90,3 +90,42 @@ def test_reduction_kernel_vs_oracle():
     cm.reduction_dev(50001, "B", *args, True, cb, sp)
     assert np.allclose(cb.get(), cb0, rtol=1e-14, atol=0)
     assert np.allclose(sp.get(), sp0, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_material_indexed_fields_equal_per_point(dim):
+    """The return mapping with per-material fields (each element's material
+    id, the Davis reduction once per material) gives the per-point path's S
+    and DS bit for bit: the same values reach every point."""
+    rng = np.random.default_rng(21)
+    n_q = 11 if dim == 3 else 7
+    n_e = 3001
+    n = n_e * n_q
+    ns = 6 if dim == 3 else 3
+    G, K, L = W.moduli()
+    tab = np.array([[15.0, np.pi / 6, 0.0, G, K, L],
+                    [6.0, np.pi / 4, 0.0, 0.5 * G, 0.5 * K, 0.5 * L],
+                    [50.0, 40 * np.pi / 180, 0.05, 2 * G, 2 * K, 2 * L]])
+    mat_id = rng.integers(0, tab.shape[0], size=n_e)
+    per_point = tab[np.repeat(mat_id, n_q)]
+    E = rng.normal(0.0, 2e-3, size=(n, ns))
+    E[rng.random(n) < 0.3] *= 5.0
+    lam = DeviceArray.from_host(np.array([1.3]))
+    Ed = DeviceArray.from_host(E)
+    out = {}
+    for mode in ("point", "material"):
+        src = per_point if mode == "point" else tab
+        m = src.shape[0]
+        c0, phi, psi, sh, bu, la = (DeviceArray.from_host(np.ascontiguousarray(src[:, k])) for k in range(6))
+        cb, sp = DeviceArray(m), DeviceArray(m)
+        cm.reduction_dev(m, "B", c0, phi, psi, lam, False, cb, sp)
+        S = DeviceArray((n, 6 if dim == 3 else 4))
+        DS = DeviceArray((n, ns * ns))
+        if mode == "point":
+            cm.constitutive_dev(dim, n, Ed, cb, sp, sh, bu, la, S, DS)
+        else:
+            mid = DeviceArray.from_host(mat_id.astype(np.int32))
+            cm.constitutive_mat_dev(dim, n, n_q, Ed, mid, m, cb, sp, sh, bu, la, S, DS)
+        out[mode] = (S.get(), DS.get())
+    assert np.array_equal(out["point"][0], out["material"][0])
+    assert np.array_equal(out["point"][1], out["material"][1])

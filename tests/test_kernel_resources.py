@@ -41,10 +41,11 @@ def resource_usage(src, tmp_path):
 @pytest.mark.skipif(not HIPCC, reason="hipcc not available")
 def test_default_constitutive_kernels_use_no_scratch(tmp_path):
     usage = resource_usage(os.path.join(build.CSRC, "constitutive.hip"), tmp_path)
-    # k_mc3d_x36d<WITH_DS, NT=true, H=16, MINW=1, SDIRECT=false> (both WITH_DS forms) and the 2D kernels
-    default = [n for n in usage if re.search(r"k_mc3d_x36dILb[01]ELb1ELi16ELi1ELb0E", n)]
+    # k_mc3d_x36d<WITH_DS, NT=true, H=16, MINW=1, SDIRECT=false, MI> (both WITH_DS forms, per-point and
+    # per-material fields) and the 2D kernels
+    default = [n for n in usage if re.search(r"k_mc3d_x36dILb[01]ELb1ELi16ELi1ELb0ELb[01]E", n)]
     two_d = [n for n in usage if "k_mc2d" in n]
-    assert len(default) == 2 and two_d, sorted(usage)
+    assert len(default) == 4 and two_d, sorted(usage)
     for n in default + two_d:
         assert usage[n].get("ScratchSize") == 0, (n, usage[n])
         assert usage[n].get("VGPRs Spill", 0) == 0, (n, usage[n])
