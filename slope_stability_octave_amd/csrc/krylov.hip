@@ -616,7 +616,7 @@ __global__ void k_sum_chunks(int nc, int k, const double *__restrict__ part, dou
 
 __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, const double *__restrict__ X,
                                                    const double *__restrict__ C, double *__restrict__ Y,
-                                                   const int *__restrict__ qstart)
+                                                   const int *__restrict__ qstart, float *__restrict__ Y32)
 {
     constexpr int PER = kSlab * kTile / 256;
     __shared__ double xs[kSlab][kTile + 1];  // [q][row]
@@ -673,7 +673,10 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int64_t n, int k, int m, cons
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             const int64_t r = r0 + tx + 16 * p;
-            if (r < n) Y[(int64_t)c * n + r] = acc[p][q];
+            if (r < n) {
+                Y[(int64_t)c * n + r] = acc[p][q];
+                if (Y32) Y32[(int64_t)c * n + r] = (float)acc[p][q];
+            }
         }
     }
 }
@@ -1260,15 +1263,18 @@ int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *
     return SSR_OK;
 }
 
-int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y, const int *qstart)
+int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y, const int *qstart, float *Y32)
 {
     SSR_REQUIRE(k >= 0 && m >= 0 && m <= 65535 * kTile, "gemm_tall: sizes out of range");
     if (n == 0 || m == 0) return SSR_OK;
-    if (k == 0) return fill<double>(n * m, 0.0, Y);
+    if (k == 0) {
+        int rc = fill<double>(n * m, 0.0, Y);
+        return (rc || !Y32) ? rc : fill<float>(n * m, 0.0, Y32);
+    }
     const int64_t rb = (n + kTile - 1) / kTile;
     SSR_REQUIRE(rb <= 0x7fffffff, "gemm_tall: too many rows");
     hipLaunchKernelGGL(k_gemm_tall, dim3((unsigned)rb, (unsigned)((m + kTile - 1) / kTile)), dim3(256), 0,
-                       ssr_stream(), n, k, m, X, C, Y, qstart);
+                       ssr_stream(), n, k, m, X, C, Y, qstart, Y32);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

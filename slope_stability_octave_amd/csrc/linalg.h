@@ -26,7 +26,9 @@ int gram(int64_t n, int k, const double *X, const double *Y, double *G, double *
 // Y (n x m) = X (n x k) C (k x m), all column-major (ld n, k, n); qstart
 // (device, one int per 64-column tile of C, or null) gives the first row of
 // C that is nonzero in that tile
-int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y, const int *qstart = nullptr);
+// (Y32, if given, receives the same product rounded to float)
+int gemm_tall(int64_t n, int k, int m, const double *X, const double *C, double *Y, const int *qstart = nullptr,
+              float *Y32 = nullptr);
 // y = alpha*A*x + beta*z on scalar CSR (64-bit row pointers); nnz (when
 // known) picks the lanes per row
 template <class T>

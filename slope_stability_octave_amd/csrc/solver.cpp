@@ -667,21 +667,28 @@ int ssr_solver_A_orthogonalize(ssr_solver *s)
         return rc;
     SSR_HIP_CHECK(hipMemcpyAsync(s->Td.p, Th.data(), sizeof(double) * Th.size(), hipMemcpyHostToDevice, ssr_stream()));
     SSR_HIP_CHECK(hipMemcpyAsync(s->Tq.p, qst.data(), sizeof(int) * qst.size(), hipMemcpyHostToDevice, ssr_stream()));
-    if ((rc = ssr::gemm_tall(n, k, acc, s->W.p, s->Td.p, s->W2.p, s->Tq.p))) return rc;
+    // in float mode the same passes also write the float copies the float
+    // solves read (the double results rounded: what converting them gives)
+    const bool f32 = s->precision == 32;
+    const int64_t capn = n * (int64_t)s->capW;
+    if (f32 && (rc = s->W32.alloc(capn))) return rc;
+    if ((rc = ssr::gemm_tall(n, k, acc, s->W.p, s->Td.p, s->W2.p, s->Tq.p, f32 ? s->W32.p : nullptr))) return rc;
     std::swap(s->W.p, s->W2.p);
     std::swap(s->W.n, s->W2.n);
     if (s->symmetric) {
         // A (W T) = (A W) T: the projector of the solves that follow reads it
-        if ((rc = s->AW2.alloc(n * (int64_t)s->capW))) return rc;
-        if ((rc = ssr::gemm_tall(n, k, acc, s->AW.p, s->Td.p, s->AW2.p, s->Tq.p))) return rc;
+        if ((rc = s->AW2.alloc(capn))) return rc;
+        if (f32 && (rc = s->AW32.alloc(capn))) return rc;
+        if ((rc = ssr::gemm_tall(n, k, acc, s->AW.p, s->Td.p, s->AW2.p, s->Tq.p, f32 ? s->AW32.p : nullptr)))
+            return rc;
         std::swap(s->AW.p, s->AW2.p);
         std::swap(s->AW.n, s->AW2.n);
     }
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));  // Th leaves scope
     s->kW = acc;
-    s->kW32 = -1;
+    s->kW32 = f32 ? acc : -1;
     s->kAW = s->symmetric ? acc : -1;
-    s->kAW32 = -1;
+    s->kAW32 = f32 && s->symmetric ? acc : -1;
     return SSR_OK;
 }
 
