@@ -347,6 +347,8 @@ def run_fos(args):
         params["linear_tol"] = args.linear_tol
     if args.lambda_init is not None:
         params["lambda_init"] = args.lambda_init
+    if args.stored_gradients:
+        prob["stored_gradients"] = True
     amg_opts = amg_options(args)
     res = problems.run_ssr(prob, params, preconditioner=args.precond, log=log, amg_options=amg_opts,
                            precision=args.precision, driver=args.newton_driver)
@@ -623,6 +625,9 @@ def main():
     ap.add_argument("--host-setup", action="store_true",
                     help="FoS workloads: FE data, material fields, load and pattern from the host restatements "
                          "instead of the device setup kernels")
+    ap.add_argument("--stored-gradients", action="store_true",
+                    help="FoS workloads with the device setup: strain and force read the stored DPhi instead of "
+                         "computing each point's gradients from the node coordinates (same bits)")
     ap.add_argument("--precision", type=int, default=64, choices=[32, 64],
                     help="Krylov/preconditioner arithmetic of the FoS workloads (32: float with double refinement)")
     ap.add_argument("--amg-theta", type=float, default=None, help="AMG strength threshold (FoS workloads)")

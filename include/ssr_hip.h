@@ -211,6 +211,16 @@ int ssr_dev_strain(const ssr_pattern *P, int n_q, const double *DPhi1, const dou
 /* F = B'*(WEIGHT.*S(1:ns,:)) (CONSTITUTIVE.m:268-285); S has s_rows rows per point. */
 int ssr_dev_force(const ssr_pattern *P, int n_q, const double *DPhi1, const double *DPhi2,
                   const double *DPhi3, const double *S, int s_rows, const double *WEIGHT, double *F);
+/* ssr_dev_strain and ssr_dev_force with each integration point's gradients
+ * computed from the node coordinates (dim x n_n node-major, device) instead of
+ * read from DPhi: the arithmetic of ssr_dev_fe_gradients, so with DPhi from
+ * ssr_dev_fe_gradients on the same coordinates the results are the same bits,
+ * and the 3 x n_p doubles of gradients per point are not read from memory.
+ * The same products as CONSTITUTIVE.m:218,251 and :268-285 (no counterpart
+ * entry in the reference, which stores B). */
+int ssr_dev_strain_geo(const ssr_pattern *P, int n_q, const double *coord, const double *U, double *E);
+int ssr_dev_force_geo(const ssr_pattern *P, int n_q, const double *coord, const double *S, int s_rows,
+                      const double *WEIGHT, double *F);
 /* K_r = r K_el + (1-r) K_t, (K_r + K_r')/2, Dirichlet masked (CONSTITUTIVE.m:515-532). */
 int ssr_dev_regularize(const ssr_pattern *P, double r, const double *Kel, const double *Kt, double *Kr);
 /* K_r straight from DS: the tangent assembly and the regularisation of
@@ -376,6 +386,11 @@ typedef struct ssr_constitutive {
     const double *c0_m, *phi_m, *psi_m;       /* n_mat, unreduced */
     const double *shear_m, *bulk_m, *lame_m;  /* n_mat */
     double *c_bar_m, *sin_phi_m;              /* n_mat, reduced */
+    /* optional node coordinates (dim x n_n node-major; NULL: not used).  Set only when
+     * DPhi1..3 came from ssr_dev_fe_gradients on them: the strain and the internal force
+     * then compute each point's gradients from the coordinates (ssr_dev_strain_geo,
+     * ssr_dev_force_geo, the same bits) instead of reading DPhi. */
+    const double *coord;
 } ssr_constitutive;
 
 #define SSR_NEWTON_UNKNOWN 0

@@ -34,6 +34,8 @@ _lib.extend_signatures({
     "ssr_dev_assemble_bsr": [_vp, _int, _vp, _vp, _vp, _vp, _int, _vp, _vp],
     "ssr_dev_strain": [_vp, _int, _vp, _vp, _vp, _vp, _vp],
     "ssr_dev_force": [_vp, _int, _vp, _vp, _vp, _vp, _int, _vp, _vp],
+    "ssr_dev_strain_geo": [_vp, _int, _vp, _vp, _vp],
+    "ssr_dev_force_geo": [_vp, _int, _vp, _vp, _int, _vp, _vp],
     "ssr_dev_regularize": [_vp, _dbl, _vp, _vp, _vp],
     "ssr_dev_assemble_K_r": [_vp, _int, _vp, _vp, _vp, _vp, _int, _vp, _dbl, _vp, _vp],
     "ssr_dev_extract_ref": [_vp, _vp, _vp],
@@ -174,6 +176,14 @@ class Pattern:
     def force(self, n_q, DPhi, S, s_rows, WEIGHT, F):
         call("ssr_dev_force", self.handle, int(n_q), ptr(DPhi[0]), ptr(DPhi[1]),
              ptr(DPhi[2]) if self.dim == 3 else _vp(0), ptr(S), int(s_rows), ptr(WEIGHT), ptr(F))
+
+    def strain_geo(self, n_q, coord, U, E):
+        """strain() with the gradients computed from node coordinates (n_n x dim)."""
+        call("ssr_dev_strain_geo", self.handle, int(n_q), ptr(coord), ptr(U), ptr(E))
+
+    def force_geo(self, n_q, coord, S, s_rows, WEIGHT, F):
+        """force() with the gradients computed from node coordinates (n_n x dim)."""
+        call("ssr_dev_force_geo", self.handle, int(n_q), ptr(coord), ptr(S), int(s_rows), ptr(WEIGHT), ptr(F))
 
     def regularize(self, r, Kel, Kt, Kr):
         call("ssr_dev_regularize", self.handle, float(r), ptr(Kel), ptr(Kt), ptr(Kr))

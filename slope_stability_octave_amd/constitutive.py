@@ -70,6 +70,7 @@ class CONSTITUTIVE:
         self.pattern = None
         self.pattern_initialized = False
         self.material_fields = None
+        self.coord_dev = None  # set_node_coordinates
         if materials is not None:
             self.set_material_table(*materials)
         self.profiler = None
@@ -118,8 +119,20 @@ class CONSTITUTIVE:
         self._t("CONSTITUTIVE.reduction", t0)
 
     # -- stress / tangent (CONSTITUTIVE.m:202-266) -----------------------------
+    def set_node_coordinates(self, coord_dev):
+        """Node coordinates on the device (n_n x dim, node-major) from which
+        DPhi was computed (device_setup.gradients_dev): strain and force then
+        compute each point's gradients from them, the same bits as reading
+        DPhi, without its 3 x n_p doubles per point of traffic."""
+        if not isinstance(coord_dev, DeviceArray) or coord_dev.nbytes < 8 * self.dim * self.n_n:
+            raise ValueError("node coordinates: a device array of n_n x dim doubles expected")
+        self.coord_dev = coord_dev
+
     def _strain(self, U):
-        self.pattern_or_raise().strain(self.n_q, self.DPhi, U, self.E)
+        if self.coord_dev is not None:
+            self.pattern_or_raise().strain_geo(self.n_q, self.coord_dev, U, self.E)
+        else:
+            self.pattern_or_raise().strain(self.n_q, self.DPhi, U, self.E)
 
     def constitutive_problem_stress(self, U):
         t0 = time.perf_counter()
@@ -140,7 +153,10 @@ class CONSTITUTIVE:
         t0 = time.perf_counter()
         if F is None:
             F = DeviceArray((self.n_n, self.dim))
-        self.pattern_or_raise().force(self.n_q, self.DPhi, self.S, self.s_rows, self.WEIGHT, F)
+        if self.coord_dev is not None:
+            self.pattern_or_raise().force_geo(self.n_q, self.coord_dev, self.S, self.s_rows, self.WEIGHT, F)
+        else:
+            self.pattern_or_raise().force(self.n_q, self.DPhi, self.S, self.s_rows, self.WEIGHT, F)
         self._t("CONSTITUTIVE.build_F", t0)
         return F
 

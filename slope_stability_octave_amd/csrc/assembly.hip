@@ -17,6 +17,7 @@
 // accumulates its dim x dim block over the points.  Ke is symmetric, so the
 // lane adds its block at (a,b) and its transpose at (b,a).  Accumulation into
 // the global values uses hardware FP64 atomics.
+#include "p2geom.h"
 #include "ssr_common.h"
 
 #include <algorithm>
@@ -864,28 +865,39 @@ __global__ __launch_bounds__(256) void k_gather_regularize(int64_t n_ub, const i
 
 // ---------------------------------------------------------------------------
 // Strain gather: one thread per integration point.  U is dim x n_n
-// (node-major), E is point-major (ns per point).
+// (node-major), E is point-major (ns per point).  GEO: the point's gradients
+// are computed from the node coordinates (p2geom.h, the arithmetic that
+// produced the stored DPhi on the device) instead of read from D1..D3.
 // ---------------------------------------------------------------------------
-template <int DIM, int NP, int NQ>
+template <int DIM, int NP, int NQ, bool GEO = false>
 __global__ __launch_bounds__(256) void k_strain(int64_t n_e, const int32_t *__restrict__ elem,
                                                 const double *__restrict__ D1,
                                                 const double *__restrict__ D2,
                                                 const double *__restrict__ D3,
-                                                const double *__restrict__ U, double *__restrict__ E)
+                                                const double *__restrict__ U, double *__restrict__ E,
+                                                const double *__restrict__ coord = nullptr)
 {
     using V = Voigt<DIM>;
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_e * NQ) return;
     const int64_t e = g / NQ;
+    double G[DIM][NP];
+    if (GEO) p2geom::point_gradients<DIM, NP>(coord, elem + e * NP, (int)(g - e * NQ), G);
     double eps[V::NS];
 #pragma unroll
     for (int r = 0; r < V::NS; ++r) eps[r] = 0.0;
+#pragma unroll
     for (int a = 0; a < NP; ++a) {
         const int64_t node = elem[e * NP + a];
         double gr[DIM];
-        gr[0] = D1[g * NP + a];
-        gr[1] = D2[g * NP + a];
-        if (DIM == 3) gr[DIM - 1] = D3[g * NP + a];
+        if (GEO) {
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) gr[c] = G[c][a];
+        } else {
+            gr[0] = D1[g * NP + a];
+            gr[1] = D2[g * NP + a];
+            if (DIM == 3) gr[DIM - 1] = D3[g * NP + a];
+        }
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
             const double u = U[node * DIM + c];
@@ -939,6 +951,68 @@ __global__ __launch_bounds__(256) void k_force(int64_t n_e, const int32_t *__res
     if (GATHER) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) F[t * DIM + c] = f[c];
+    } else {
+        const int64_t node = elem[e * NP + a];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) atomic_add_f64(F + node * DIM + c, f[c]);
+    }
+}
+
+// k_force with the gradients computed from the node coordinates: EB elements
+// per block.  Each lane of the first phase takes one point, computes its
+// gradients (p2geom.h) and the products v(a, c) of every node a, which the
+// second phase, one lane per (element, node), sums over the points in order.
+// The products and the sums are k_force's, term for term, so the results are
+// the same bits as k_force on the stored gradients.
+template <int DIM, int NP, int NQ, int EB, bool GATHER>
+__global__ __launch_bounds__(256) void k_force_geo(int64_t n_e, const int32_t *__restrict__ elem,
+                                                   const double *__restrict__ coord,
+                                                   const double *__restrict__ S, int s_rows,
+                                                   const double *__restrict__ W, double *__restrict__ F)
+{
+    using V = Voigt<DIM>;
+    __shared__ double sV[EB][NQ][NP][DIM];
+    const int t = threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * EB;
+    if (t < EB * NQ) {
+        const int el = t / NQ, q = t - (t / NQ) * NQ;
+        const int64_t e = e0 + el;
+        if (e < n_e) {
+            const int64_t g = e * NQ + q;
+            double G[DIM][NP];
+            p2geom::point_gradients<DIM, NP>(coord, elem + e * NP, q, G);
+            double ws[V::NS];
+#pragma unroll
+            for (int r = 0; r < V::NS; ++r) ws[r] = W[g] * S[g * s_rows + r];
+#pragma unroll
+            for (int a = 0; a < NP; ++a) {
+                double gr[DIM];
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) gr[c] = G[c][a];
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int k = 0; k < V::NNZ; ++k) v += gr[V::grad(c, k)] * ws[V::row(c, k)];
+                    sV[el][q][a][c] = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t >= EB * NP) return;
+    const int el = t / NP, a = t - (t / NP) * NP;
+    const int64_t e = e0 + el;
+    if (e >= n_e) return;
+    double f[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) f[c] = 0.0;
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) f[c] += sV[el][q][a][c];
+    if (GATHER) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) F[(e * NP + a) * DIM + c] = f[c];
     } else {
         const int64_t node = elem[e * NP + a];
 #pragma unroll
@@ -1311,6 +1385,78 @@ int ssr_dev_force(const ssr_pattern *P, int n_q, const double *DPhi1, const doub
     else
         hipLaunchKernelGGL((k_force<2, 6, 7, false>), dim3(grid), dim3(th), 0, ssr_stream(), n_e, elem, DPhi1,
                            DPhi2, DPhi3, S, s_rows, WEIGHT, F);
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+// ssr_dev_strain with each point's gradients computed from the node
+// coordinates (dim x n_n node-major) instead of read from DPhi.
+int ssr_dev_strain_geo(const ssr_pattern *P, int n_q, const double *coord, const double *U, double *E)
+{
+    SSR_REQUIRE(P && coord && U && E, "strain_geo: null argument");
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    const int64_t n = n_e * n_q;
+    const int th = 256;
+    if (dim == 3) {
+        SSR_REQUIRE(n_p == 10 && n_q == 11, "strain_geo: P2 tetrahedra with 11 points supported");
+        hipLaunchKernelGGL((k_strain<3, 10, 11, true>), dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n_e,
+                           ssr_pattern_d_elem(P), nullptr, nullptr, nullptr, U, E, coord);
+    } else {
+        SSR_REQUIRE(n_p == 6 && n_q == 7, "strain_geo: P2 triangles with 7 points supported");
+        hipLaunchKernelGGL((k_strain<2, 6, 7, true>), dim3(ssr_blocks(n, th)), dim3(th), 0, ssr_stream(), n_e,
+                           ssr_pattern_d_elem(P), nullptr, nullptr, nullptr, U, E, coord);
+    }
+    SSR_HIP_CHECK(hipGetLastError());
+    return SSR_OK;
+}
+
+// ssr_dev_force with each point's gradients computed from the node
+// coordinates; the same forms (ordered per-node sums or atomics) and bits.
+int ssr_dev_force_geo(const ssr_pattern *P, int n_q, const double *coord, const double *S, int s_rows,
+                      const double *WEIGHT, double *F)
+{
+    SSR_REQUIRE(P && coord && S && WEIGHT && F, "force_geo: null argument");
+    int dim;
+    int64_t n_n, n_p, n_e, nnzb, nnzf;
+    ssr_pattern_sizes(P, &dim, &n_n, &n_p, &n_e, &nnzb, &nnzf);
+    if (dim == 3)
+        SSR_REQUIRE(n_p == 10 && n_q == 11 && s_rows >= 6, "force_geo: P2 tetrahedra with 11 points supported");
+    else
+        SSR_REQUIRE(n_p == 6 && n_q == 7 && s_rows >= 3, "force_geo: P2 triangles with 7 points supported");
+    constexpr int EB3 = 23, EB2 = 36;  // elements per 256-lane block: 253 and 252 points
+    const unsigned grid = (unsigned)((n_e + (dim == 3 ? EB3 : EB2) - 1) / (dim == 3 ? EB3 : EB2));
+    const int32_t *elem = ssr_pattern_d_elem(P);
+    const bool gather = ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) == 0 && ssr_pattern_d_nptr(P);
+    if (gather) {
+        double *fe = nullptr;
+        int rc = ssr_pattern_fe(P, &fe);
+        if (rc) return rc;
+        if (dim == 3)
+            hipLaunchKernelGGL((k_force_geo<3, 10, 11, EB3, true>), dim3(grid), dim3(256), 0, ssr_stream(), n_e, elem,
+                               coord, S, s_rows, WEIGHT, fe);
+        else
+            hipLaunchKernelGGL((k_force_geo<2, 6, 7, EB2, true>), dim3(grid), dim3(256), 0, ssr_stream(), n_e, elem,
+                               coord, S, s_rows, WEIGHT, fe);
+        SSR_HIP_CHECK(hipGetLastError());
+        const unsigned g2 = ssr_blocks(n_n * dim, 256);
+        if (dim == 3)
+            hipLaunchKernelGGL(k_gather_force<3>, dim3(g2), dim3(256), 0, ssr_stream(), n_n, ssr_pattern_d_nptr(P),
+                               ssr_pattern_d_ncode(P), fe, F);
+        else
+            hipLaunchKernelGGL(k_gather_force<2>, dim3(g2), dim3(256), 0, ssr_stream(), n_n, ssr_pattern_d_nptr(P),
+                               ssr_pattern_d_ncode(P), fe, F);
+        SSR_HIP_CHECK(hipGetLastError());
+        return SSR_OK;
+    }
+    SSR_HIP_CHECK(hipMemsetAsync(F, 0, sizeof(double) * dim * n_n, ssr_stream()));
+    if (dim == 3)
+        hipLaunchKernelGGL((k_force_geo<3, 10, 11, EB3, false>), dim3(grid), dim3(256), 0, ssr_stream(), n_e, elem,
+                           coord, S, s_rows, WEIGHT, F);
+    else
+        hipLaunchKernelGGL((k_force_geo<2, 6, 7, EB2, false>), dim3(grid), dim3(256), 0, ssr_stream(), n_e, elem,
+                           coord, S, s_rows, WEIGHT, F);
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }

@@ -77,7 +77,10 @@ int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with
 {
     const ssr_constitutive *C = x.C;
     if (reduce) NCHECK(reduction(x, lam));
-    NCHECK(ssr_dev_strain(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, U, C->E));
+    if (C->coord)
+        NCHECK(ssr_dev_strain_geo(C->pattern, C->n_q, C->coord, U, C->E));
+    else
+        NCHECK(ssr_dev_strain(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, U, C->E));
     if (C->mat_id)
         NCHECK(ssr_dev_constitutive_mat(C->dim, C->n_int, C->n_q, C->E, C->mat_id, C->n_mat, C->c_bar_m,
                                         C->sin_phi_m, C->shear_m, C->bulk_m, C->lame_m, C->S,
@@ -88,6 +91,7 @@ int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with
     else
         NCHECK(ssr_dev_constitutive_2D(C->n_int, C->E, C->c_bar, C->sin_phi, C->shear, C->bulk, C->lame, C->S,
                                        with_ds ? C->DS : nullptr, nullptr));
+    if (C->coord) return ssr_dev_force_geo(C->pattern, C->n_q, C->coord, C->S, C->s_rows, C->WEIGHT, F);
     return ssr_dev_force(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->S, C->s_rows, C->WEIGHT, F);
 }
 

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "devutil.h"
+#include "p2geom.h"
 #include "pattern.h"
 #include "ssr_common.h"
 
@@ -41,38 +42,14 @@ using namespace ssr_dev;
 namespace {
 
 // ---- P2 reference element ---------------------------------------------------
-// 11-point rule on the reference tetrahedron (quadrature_volume_3D.m:39-42)
-__constant__ double kXi3[3][11] = {
-    {1.0 / 4, 0.0714285714285714, 0.785714285714286, 0.0714285714285714, 0.0714285714285714, 0.399403576166799,
-     0.100596423833201, 0.100596423833201, 0.399403576166799, 0.399403576166799, 0.100596423833201},
-    {1.0 / 4, 0.0714285714285714, 0.0714285714285714, 0.785714285714286, 0.0714285714285714, 0.100596423833201,
-     0.399403576166799, 0.100596423833201, 0.399403576166799, 0.100596423833201, 0.399403576166799},
-    {1.0 / 4, 0.0714285714285714, 0.0714285714285714, 0.0714285714285714, 0.785714285714286, 0.100596423833201,
-     0.100596423833201, 0.399403576166799, 0.100596423833201, 0.399403576166799, 0.399403576166799}};
-__constant__ double kWF3[11] = {-0.013155555555555, 0.007622222222222, 0.007622222222222, 0.007622222222222,
-                                0.007622222222222,  0.024888888888888, 0.024888888888888, 0.024888888888888,
-                                0.024888888888888,  0.024888888888888, 0.024888888888888};
-// 7-point rule on the reference triangle (quadrature_volume_2D.m:28-33)
-__constant__ double kXi2[2][7] = {
-    {0.1012865073235, 0.7974269853531, 0.1012865073235, 0.4701420641051, 0.4701420641051, 0.0597158717898, 1.0 / 3},
-    {0.1012865073235, 0.1012865073235, 0.7974269853531, 0.0597158717898, 0.4701420641051, 0.4701420641051, 1.0 / 3}};
-__constant__ double kWF2[7] = {0.1259391805448 / 2, 0.1259391805448 / 2, 0.1259391805448 / 2, 0.1323941527885 / 2,
-                               0.1323941527885 / 2, 0.1323941527885 / 2, 0.225 / 2};
+// (tables, reference derivatives and the per-point Jacobian: p2geom.h)
+using p2geom::kWF2;
+using p2geom::kWF3;
+using p2geom::kXi2;
+using p2geom::kXi3;
 
 #pragma clang fp contract(off)
 
-// reference derivatives of the ten P2 tetrahedron functions at (x1, x2, x3)
-// (local_basis_volume_3D.m:75-106)
-__device__ inline void p2_tet_dhat(double x1, double x2, double x3, double D1[10], double D2[10], double D3[10])
-{
-    const double x0 = 1 - x1 - x2 - x3;
-    D1[0] = -4 * x0 + 1; D1[1] = 4 * x1 - 1; D1[2] = 0;  D1[3] = 0;
-    D1[4] = 4 * (x0 - x1); D1[5] = 4 * x2; D1[6] = -4 * x2; D1[7] = 4 * x3; D1[8] = 0; D1[9] = -4 * x3;
-    D2[0] = -4 * x0 + 1; D2[1] = 0; D2[2] = 4 * x2 - 1; D2[3] = 0;
-    D2[4] = -4 * x1; D2[5] = 4 * x1; D2[6] = 4 * (x0 - x2); D2[7] = 0; D2[8] = 4 * x3; D2[9] = -4 * x3;
-    D3[0] = -4 * x0 + 1; D3[1] = 0; D3[2] = 0; D3[3] = 4 * x3 - 1;
-    D3[4] = -4 * x1; D3[5] = 0; D3[6] = -4 * x2; D3[7] = 4 * x1; D3[8] = 4 * x2; D3[9] = 4 * (x0 - x3);
-}
 // P2 tetrahedron values (local_basis_volume_3D.m:64-73)
 __device__ inline double p2_tet_hat(int a, double x1, double x2, double x3)
 {
@@ -90,13 +67,6 @@ __device__ inline double p2_tet_hat(int a, double x1, double x2, double x3)
     default: return 4 * x0 * x3;
     }
 }
-// reference derivatives of the six P2 triangle functions (local_basis_volume_2D.m:44-47)
-__device__ inline void p2_tri_dhat(double x1, double x2, double D1[6], double D2[6])
-{
-    const double x0 = 1 - x1 - x2;
-    D1[0] = -4 * x0 + 1; D1[1] = 4 * x1 - 1; D1[2] = 0; D1[3] = 4 * x2; D1[4] = -4 * x2; D1[5] = 4 * (x0 - x1);
-    D2[0] = -4 * x0 + 1; D2[1] = 0; D2[2] = 4 * x2 - 1; D2[3] = 4 * x1; D2[4] = 4 * (x0 - x2); D2[5] = -4 * x1;
-}
 // P2 triangle values (local_basis_volume_2D.m:42-43)
 __device__ inline double p2_tri_hat(int a, double x1, double x2)
 {
@@ -112,7 +82,7 @@ __device__ inline double p2_tri_hat(int a, double x1, double x2)
 }
 
 // One lane per integration point p = e * 11 + q.  coord: 3 x n_n node-major;
-// DPhi_k: 10 x n_int point-major.
+// DPhi_k: 10 x n_int point-major (elastic_stiffness_matrix_3D.m:60-96,148).
 __global__ void __launch_bounds__(256) k_fe_grad3(int64_t n_int, const double *__restrict__ coord,
                                                   const int32_t *__restrict__ elem, double *__restrict__ DP1,
                                                   double *__restrict__ DP2, double *__restrict__ DP3,
@@ -122,36 +92,14 @@ __global__ void __launch_bounds__(256) k_fe_grad3(int64_t n_int, const double *_
     if (p >= n_int) return;
     const int64_t e = p / 11;
     const int q = (int)(p - e * 11);
-    double H1[10], H2[10], H3[10];
-    p2_tet_dhat(kXi3[0][q], kXi3[1][q], kXi3[2][q], H1, H2, H3);
-    // J_rc = sum_a X_c(a) * dHat_r(a), summed over a in order (elastic_stiffness_matrix_3D.m:66-74)
-    double J11 = 0, J12 = 0, J13 = 0, J21 = 0, J22 = 0, J23 = 0, J31 = 0, J32 = 0, J33 = 0;
+    double H1[10], H2[10], H3[10], iv[9], DET;
+    p2geom::tet_dhat(q, H1, H2, H3);
+    p2geom::tet_jinv(coord, elem + e * 10, H1, H2, H3, iv, DET);
 #pragma unroll
     for (int a = 0; a < 10; ++a) {
-        const int64_t v = elem[e * 10 + a];
-        const double X1 = coord[3 * v], X2 = coord[3 * v + 1], X3 = coord[3 * v + 2];
-        if (a == 0) {
-            J11 = X1 * H1[a]; J12 = X2 * H1[a]; J13 = X3 * H1[a];
-            J21 = X1 * H2[a]; J22 = X2 * H2[a]; J23 = X3 * H2[a];
-            J31 = X1 * H3[a]; J32 = X2 * H3[a]; J33 = X3 * H3[a];
-        } else {
-            J11 += X1 * H1[a]; J12 += X2 * H1[a]; J13 += X3 * H1[a];
-            J21 += X1 * H2[a]; J22 += X2 * H2[a]; J23 += X3 * H2[a];
-            J31 += X1 * H3[a]; J32 += X2 * H3[a]; J33 += X3 * H3[a];
-        }
-    }
-    const double DET = J11 * (J22 * J33 - J32 * J23) - J12 * (J21 * J33 - J23 * J31) + J13 * (J21 * J32 - J22 * J31);
-    const double i11 = (J22 * J33 - J23 * J32) / DET, i12 = -(J12 * J33 - J13 * J32) / DET,
-                 i13 = (J12 * J23 - J13 * J22) / DET;
-    const double i21 = -(J21 * J33 - J23 * J31) / DET, i22 = (J11 * J33 - J13 * J31) / DET,
-                 i23 = -(J11 * J23 - J13 * J21) / DET;
-    const double i31 = (J21 * J32 - J22 * J31) / DET, i32 = -(J11 * J32 - J12 * J31) / DET,
-                 i33 = (J11 * J22 - J12 * J21) / DET;
-#pragma unroll
-    for (int a = 0; a < 10; ++a) {
-        DP1[p * 10 + a] = i11 * H1[a] + i12 * H2[a] + i13 * H3[a];
-        DP2[p * 10 + a] = i21 * H1[a] + i22 * H2[a] + i23 * H3[a];
-        DP3[p * 10 + a] = i31 * H1[a] + i32 * H2[a] + i33 * H3[a];
+        DP1[p * 10 + a] = p2geom::tet_grad(iv, H1, H2, H3, 0, a);
+        DP2[p * 10 + a] = p2geom::tet_grad(iv, H1, H2, H3, 1, a);
+        DP3[p * 10 + a] = p2geom::tet_grad(iv, H1, H2, H3, 2, a);
     }
     W[p] = fabs(DET) * kWF3[q];
 }
@@ -165,25 +113,13 @@ __global__ void __launch_bounds__(256) k_fe_grad2(int64_t n_int, const double *_
     if (p >= n_int) return;
     const int64_t e = p / 7;
     const int q = (int)(p - e * 7);
-    double H1[6], H2[6];
-    p2_tri_dhat(kXi2[0][q], kXi2[1][q], H1, H2);
-    double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+    double H1[6], H2[6], iv[4], DET;
+    p2geom::tri_dhat(q, H1, H2);
+    p2geom::tri_jinv(coord, elem + e * 6, H1, H2, iv, DET);
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
-        const int64_t v = elem[e * 6 + a];
-        const double X1 = coord[2 * v], X2 = coord[2 * v + 1];
-        if (a == 0) {
-            J11 = X1 * H1[a]; J12 = X2 * H1[a]; J21 = X1 * H2[a]; J22 = X2 * H2[a];
-        } else {
-            J11 += X1 * H1[a]; J12 += X2 * H1[a]; J21 += X1 * H2[a]; J22 += X2 * H2[a];
-        }
-    }
-    const double DET = J11 * J22 - J12 * J21;
-    const double i11 = J22 / DET, i12 = -J12 / DET, i21 = -J21 / DET, i22 = J11 / DET;
-#pragma unroll
-    for (int a = 0; a < 6; ++a) {
-        DP1[p * 6 + a] = i11 * H1[a] + i12 * H2[a];
-        DP2[p * 6 + a] = i21 * H1[a] + i22 * H2[a];
+        DP1[p * 6 + a] = p2geom::tri_grad(iv, H1, H2, 0, a);
+        DP2[p * 6 + a] = p2geom::tri_grad(iv, H1, H2, 1, a);
     }
     W[p] = fabs(DET) * kWF2[q];
 }

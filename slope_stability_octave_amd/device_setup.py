@@ -104,6 +104,9 @@ def finish_device(coord, elem, Q, dim, mats, mat_id):
     f = volume_force_dev(pattern, n_q, W, m["gamma"], 1)
     call("ssr_synchronize")
     t = time.perf_counter() - t0
-    return dict(coord=coord, elem=elem, Q=Q, dim=dim, DPhi=DPhi, WEIGHT=W, n_q=n_q, mats=m, f=f, n_int=n_e * n_q,
+    # coord_dev stays with the problem: the Newton path computes each point's
+    # gradients from it (the bits of DPhi) instead of reading DPhi
+    return dict(coord=coord, coord_dev=coord_d, elem=elem, Q=Q, dim=dim, DPhi=DPhi, WEIGHT=W, n_q=n_q, mats=m, f=f,
+                n_int=n_e * n_q,
                 n_dof_free=int(np.asarray(Q).sum()), pattern=pattern, material_table=table,
                 mat_id=np.asarray(mat_id), device=True, setup_s=t)

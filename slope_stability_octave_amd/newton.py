@@ -60,7 +60,8 @@ class _Constitutive(ctypes.Structure):
                 ("E", _vp), ("S", _vp), ("DS", _vp), ("K_elast", _vp), ("K_tang", _vp), ("K_r", _vp),
                 ("mask", _vp), ("f", _vp), ("n_friction", _i64),
                 ("mat_id", _vp), ("n_mat", _i64), ("c0_m", _vp), ("phi_m", _vp), ("psi_m", _vp),
-                ("shear_m", _vp), ("bulk_m", _vp), ("lame_m", _vp), ("c_bar_m", _vp), ("sin_phi_m", _vp)]
+                ("shear_m", _vp), ("bulk_m", _vp), ("lame_m", _vp), ("c_bar_m", _vp), ("sin_phi_m", _vp),
+                ("coord", _vp)]
 
 
 def native_description(cmb, f):
@@ -102,6 +103,8 @@ def native_description(cmb, f):
         d.n_mat = mt["n_mat"]
         for name in ("c0", "phi", "psi", "shear", "bulk", "lame", "c_bar", "sin_phi"):
             setattr(d, name + "_m", ptr(mt[name]).value)
+    if getattr(cmb, "coord_dev", None) is not None:
+        d.coord = ptr(cmb.coord_dev).value
     d.f = ptr(f).value
     return d
 

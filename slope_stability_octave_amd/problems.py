@@ -91,6 +91,10 @@ def make_constitutive(prob, Davis_type="B"):
                        m["lame"], prob["WEIGHT"], 6 if dim == 3 else 3, prob["n_int"], dim, friction_pairs=pairs,
                        materials=materials)
     cmb.init_K_r_pattern(prob["Q"], pattern=prob.get("pattern"))
+    if prob.get("coord_dev") is not None and not prob.get("stored_gradients"):
+        # DPhi came from the device setup on these coordinates: strain and
+        # force compute the gradients from them (same bits, less traffic)
+        cmb.set_node_coordinates(prob["coord_dev"])
     return cmb
 
 
