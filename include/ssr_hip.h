@@ -249,6 +249,9 @@ int ssr_dev_spmv_bsr_f32(int dim, int64_t n_rows, const int32_t *row_ptr, const 
 int ssr_dev_convert_f64_f32(int64_t n, const double *x, float *y);
 int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result);
 int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result);
+/* max over nodes of the Euclidean norm of U's node vector (dim x n_n node-major), to the
+ * host: the U_max history of SSR_indirect_continuation.m:92-93,182 (max(sqrt(sum(U.^2, 1)))). */
+int ssr_node_norm_max(int dim, int64_t n_n, const double *U, double *host_result);
 int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h);
 /* ssr_dev_multidot without any allocation: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph

@@ -16,12 +16,13 @@ with the state the step started from (training-data capture).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 
 import numpy as np
 
-from ._lib import DeviceArray
+from ._lib import DeviceArray, call, ptr
 from .linear_solvers import dot, lincomb
 from .newton import Workspace, newton, newton_ind_SSR
 
@@ -40,8 +41,10 @@ def _rebuild_after_failure(solver):
         inv()
 
 def _umax(U):
-    u = U.get()
-    return float(np.sqrt((u * u).sum(axis=1)).max())
+    """max over nodes of |U(:, node)| (SSR_indirect_continuation.m:92-93,182), on the device."""
+    out = ctypes.c_double(0.0)
+    call("ssr_node_norm_max", int(U.shape[1]), int(U.shape[0]), ptr(U), ctypes.byref(out))
+    return out.value
 
 
 def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max,

@@ -354,6 +354,46 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial(int64_t n, const T 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// part[b] = largest u(1)^2 + ... + u(dim)^2 over the block's share of the
+// nodes of U (dim x n_n node-major); k_max_partials then takes the largest
+// partial and its square root (sqrt is monotone: the same as the largest root)
+template <int DIM>
+__global__ __launch_bounds__(kRedThreads) void k_node_sq_max_partial(int64_t n_n, const double *__restrict__ U,
+                                                                     double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[kRedThreads];
+    double m = 0.0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n_n; v += (int64_t)gridDim.x * blockDim.x) {
+        double q = U[v * DIM] * U[v * DIM];
+#pragma unroll
+        for (int c = 1; c < DIM; ++c) q = q + U[v * DIM + c] * U[v * DIM + c];
+        m = q > m ? q : m;
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && sh[threadIdx.x + o] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_sqrt_max_partials(int nb, const double *__restrict__ part,
+                                                                   double *__restrict__ out)
+{
+    __shared__ double sh[kRedThreads];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) m = part[i] > m ? part[i] : m;
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && sh[threadIdx.x + o] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sqrt(sh[0]);
+}
+
 // out[c] = sum_b part[c*nb + b] for c < k (one block per output)
 __global__ __launch_bounds__(kRedThreads) void k_reduce_partials(int nb, int k, const double *__restrict__ part,
                                                                  double *__restrict__ out)
@@ -1349,6 +1389,28 @@ int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, dou
     int rc = red_scratch(&sc);
     if (rc) return rc;
     if ((rc = ssr::dot<double>(n, a, b, mask, sc->red))) return rc;
+    SSR_HIP_CHECK(hipMemcpyAsync(host_result, sc->red, sizeof(double), hipMemcpyDeviceToHost, ssr_stream()));
+    SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
+    return SSR_OK;
+}
+
+// max over nodes of the Euclidean norm of U's node vector (U dim x n_n
+// node-major), returned to the host (synchronises): the U_max of the
+// continuation histories (SSR_indirect_continuation.m:92-93,182)
+int ssr_node_norm_max(int dim, int64_t n_n, const double *U, double *host_result)
+{
+    SSR_REQUIRE((dim == 2 || dim == 3) && n_n >= 0 && host_result && (n_n == 0 || U),
+                "node_norm_max: bad arguments");
+    RedScratch *sc = nullptr;
+    int rc = red_scratch(&sc);
+    if (rc) return rc;
+    const unsigned nb = red_blocks(n_n);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_node_sq_max_partial<3>, dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n_n, U, sc->part);
+    else
+        hipLaunchKernelGGL(k_node_sq_max_partial<2>, dim3(nb), dim3(kRedThreads), 0, ssr_stream(), n_n, U, sc->part);
+    hipLaunchKernelGGL(k_sqrt_max_partials, dim3(1), dim3(kRedThreads), 0, ssr_stream(), (int)nb, sc->part, sc->red);
+    SSR_HIP_CHECK(hipGetLastError());
     SSR_HIP_CHECK(hipMemcpyAsync(host_result, sc->red, sizeof(double), hipMemcpyDeviceToHost, ssr_stream()));
     SSR_HIP_CHECK(hipStreamSynchronize(ssr_stream()));
     return SSR_OK;

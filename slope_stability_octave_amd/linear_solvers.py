@@ -47,6 +47,7 @@ _lib.extend_signatures({
     "ssr_dev_spmv_bsr_f32": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
     "ssr_dev_convert_f64_f32": [_i64, _vp, _vp],
     "ssr_dot": [_i64, _vp, _vp, _vp, _vp],
+    "ssr_node_norm_max": [_int, _i64, _vp, _vp],
     "ssr_dev_multidot": [_i64, _int, _vp, _i64, _vp, _vp],
     "ssr_dev_multidot_noalloc": [_i64, _int, _vp, _i64, _vp, _vp],
     "ssr_dev_gemv_update": [_i64, _int, _vp, _i64, _vp, _dbl, _dbl, _vp],
@@ -106,7 +107,8 @@ class DFGMRES:
     method = "dfgmres"
 
     def __init__(self, pattern, tolerance=1e-1, max_iterations=100, tolerance_deflation_basis=1e-3,
-                 preconditioner="block_jacobi", verbose=0, amg_options=None, precision=64, symmetric=True):
+                 preconditioner="block_jacobi", verbose=0, amg_options=None, precision=64, symmetric=True, *,
+                 share_preconditioner_of=None):
         self.pattern = pattern
         self.precision = int(precision)
         self.tolerance = float(tolerance)
@@ -127,7 +129,11 @@ class DFGMRES:
         # the operators of the Newton solves (K_r) are symmetric
         self.symmetric = bool(symmetric)
         call("ssr_solver_set_symmetric", self.handle, 1 if self.symmetric else 0)
-        if preconditioner == "amg":
+        if share_preconditioner_of is not None:
+            # the other solver's preconditioner object (no AMG of its own is
+            # built: that would be a host near-null-space and an upload per copy)
+            call("ssr_solver_share_preconditioner", self.handle, share_preconditioner_of.handle)
+        elif preconditioner == "amg":
             from . import amg
             amg.attach(self, amg_options)
         self._K = None
@@ -207,11 +213,11 @@ class DFGMRES:
         preconditioner is rebuilt on the next setup (DFGMRES.m:204-221), or,
         with amg_options['reuse_hierarchy'] (the default), the copy shares
         this solver's AMG object so one hierarchy serves every copy."""
+        share = self.preconditioner_name == "amg" and _amg_option(self.amg_options, "reuse_hierarchy")
         new = type(self)(self.pattern, self.tolerance, self.max_iterations, self.tolerance_deflation_basis,
-                         self.preconditioner_name, self.verbose, self.amg_options, self.precision, self.symmetric)
+                         self.preconditioner_name, self.verbose, self.amg_options, self.precision, self.symmetric,
+                         share_preconditioner_of=self if share else None)
         call("ssr_solver_copy_deflation", new.handle, self.handle)
-        if self.preconditioner_name == "amg" and _amg_option(self.amg_options, "reuse_hierarchy"):
-            call("ssr_solver_share_preconditioner", new.handle, self.handle)
         new.profiler = self.profiler
         new.timings = self.timings  # shared, like the reference's profiler handle
         new.counters = self.counters

@@ -453,3 +453,21 @@ def test_amg_coarse_block_csr_matches_scalar_csr(prec, dim):
     rel = np.abs(a - b).max() / np.abs(b).max()
     print(f"prec {prec} dim {dim}: block vs scalar CSR, max rel {rel:.2e}", flush=True)
     assert rel <= (1e-12 if prec == 64 else 1e-5)
+
+
+@pytest.mark.parametrize("dim,n_n", [(3, 1), (3, 200_003), (2, 77_777)])
+def test_node_norm_max_matches_numpy(dim, n_n):
+    """U_max of the continuation histories (max over nodes of the node
+    vector's Euclidean norm) against numpy on the same field; the largest
+    node placed at the end, in the middle and (n_n = 1) alone."""
+    from slope_stability_octave_amd.continuation import _umax
+    rng = np.random.default_rng(n_n)
+    u = rng.standard_normal((n_n, dim))
+    u[n_n // 2] *= 5.0
+    want = float(np.sqrt((u * u).sum(axis=1)).max())
+    got = _umax(DeviceArray.from_host(u))
+    print(f"dim {dim} n_n {n_n}: {got!r} vs {want!r}", flush=True)
+    assert abs(got - want) <= 1e-15 * want
+    u[-1] *= 10.0
+    want = float(np.sqrt((u * u).sum(axis=1)).max())
+    assert abs(_umax(DeviceArray.from_host(u)) - want) <= 1e-15 * want
