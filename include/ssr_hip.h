@@ -262,6 +262,11 @@ int ssr_dev_multidot(int64_t n, int k, const double *V, int64_t ld, const double
 int ssr_dev_multidot_noalloc(int64_t n, int k, const double *V, int64_t ld, const double *u, double *h);
 int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const double *h, double alpha,
                         double beta, double *u);
+/* float basis and vector, double coefficients and sums: the forms the precision-32 Krylov
+ * iterations run (same call sites). */
+int ssr_dev_multidot_f32(int64_t n, int k, const float *V, int64_t ld, const float *u, double *h);
+int ssr_dev_gemv_update_f32(int64_t n, int k, const float *V, int64_t ld, const double *h, double alpha,
+                            double beta, float *u);
 int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y);
 int ssr_dev_lincomb(int64_t n, double a, const double *x, double b, const double *y, double *z);
 int ssr_dev_mul(int64_t n, const double *m, double *x);

@@ -1446,6 +1446,19 @@ int ssr_dev_gemv_update(int64_t n, int k, const double *V, int64_t ld, const dou
     return ssr::gemv_update<double, double>(n, k, V, ld, h, alpha, beta, u);
 }
 
+// the float forms of the two (float basis and vector, double coefficients
+// and sums): what the precision-32 Krylov iterations run
+int ssr_dev_multidot_f32(int64_t n, int k, const float *V, int64_t ld, const float *u, double *h)
+{
+    return ssr::multidot<float>(n, k, V, ld, u, h);
+}
+
+int ssr_dev_gemv_update_f32(int64_t n, int k, const float *V, int64_t ld, const double *h, double alpha, double beta,
+                            float *u)
+{
+    return ssr::gemv_update<float, float>(n, k, V, ld, h, alpha, beta, u);
+}
+
 int ssr_dev_axpby(int64_t n, double a, const double *x, double b, double *y) { return ssr::axpby<double>(n, a, x, b, y); }
 
 int ssr_dev_lincomb(int64_t n, double a, const double *x, double b, const double *y, double *z)

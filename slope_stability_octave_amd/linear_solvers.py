@@ -51,6 +51,8 @@ _lib.extend_signatures({
     "ssr_dev_multidot": [_i64, _int, _vp, _i64, _vp, _vp],
     "ssr_dev_multidot_noalloc": [_i64, _int, _vp, _i64, _vp, _vp],
     "ssr_dev_gemv_update": [_i64, _int, _vp, _i64, _vp, _dbl, _dbl, _vp],
+    "ssr_dev_multidot_f32": [_i64, _int, _vp, _i64, _vp, _vp],
+    "ssr_dev_gemv_update_f32": [_i64, _int, _vp, _i64, _vp, _dbl, _dbl, _vp],
     "ssr_dev_axpby": [_i64, _dbl, _vp, _dbl, _vp],
     "ssr_dev_lincomb": [_i64, _dbl, _vp, _dbl, _vp, _vp],
     "ssr_dev_mul": [_i64, _vp, _vp],

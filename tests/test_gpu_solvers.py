@@ -471,3 +471,26 @@ def test_node_norm_max_matches_numpy(dim, n_n):
     u[-1] *= 10.0
     want = float(np.sqrt((u * u).sum(axis=1)).max())
     assert abs(_umax(DeviceArray.from_host(u)) - want) <= 1e-15 * want
+
+
+@pytest.mark.parametrize("k", [1, 7, 8, 20, 92])
+def test_float_multidot_and_basis_update(k):
+    """The float forms the precision-32 iterations run: h = V' u with float
+    V, u summed in double, and u = a u + b V h, against numpy in double on
+    the same float inputs."""
+    n = 300_007
+    rng = np.random.default_rng(k)
+    V = rng.standard_normal((k, n)).astype(np.float32)
+    u = rng.standard_normal(n).astype(np.float32)
+    Vd, ud, h = DeviceArray.from_host(V), DeviceArray.from_host(u), DeviceArray(k)
+    _lib.call("ssr_dev_multidot_f32", n, k, _lib.ptr(Vd), n, _lib.ptr(ud), _lib.ptr(h))
+    want = V.astype(np.float64) @ u.astype(np.float64)
+    got = h.get()
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    hh = rng.standard_normal(k)
+    hd = DeviceArray.from_host(hh)
+    _lib.call("ssr_dev_gemv_update_f32", n, k, _lib.ptr(Vd), n, _lib.ptr(hd), 0.5, -2.0, _lib.ptr(ud))
+    want_u = (0.5 * u.astype(np.float64) - 2.0 * (V.astype(np.float64).T @ hh)).astype(np.float32)
+    got_u = ud.get()
+    assert np.abs(got_u.astype(np.float64) - want_u).max() <= 2e-6 * np.abs(want_u).max()
+
