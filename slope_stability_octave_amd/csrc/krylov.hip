@@ -507,7 +507,42 @@ __global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV 
 // ---------------------------------------------------------------------------
 constexpr int kSpmmNV = 8;
 
-template <int DIM, int G, class T>
+// values left per lane after the halving steps from distance o down to 1
+__host__ __device__ constexpr int spmm_final_count(int n, int o)
+{
+    return o == 0 ? n : spmm_final_count(n % 2 == 0 ? n / 2 : n, o / 2);
+}
+
+// One exchange distance O of the halving sum of k_spmm_bsr, then the smaller
+// ones.  w[0..N) are the lane's current values; on return w[0..final) hold
+// its kept sums, whose flat indices start at base.  A step that cannot halve
+// (N odd) adds the partner's copy of every value, and from then on only the
+// lane with bit O clear writes (writer).
+template <int N, int O, int G, class T, int NA>
+__device__ __forceinline__ void spmm_halving_sum(T (&w)[NA], int l, int &base, bool &writer)
+{
+    if constexpr (O > 0) {
+        if constexpr (N % 2 == 0) {
+            constexpr int H = N / 2;
+            const bool hi = (l & O) != 0;
+#pragma unroll
+            for (int i = 0; i < H; ++i) {
+                const T send = hi ? w[i] : w[H + i];
+                const T keep = hi ? w[H + i] : w[i];
+                w[i] = keep + __shfl_xor(send, O, G);
+            }
+            if (hi) base += H;
+            spmm_halving_sum<H, O / 2, G, T>(w, l, base, writer);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) w[i] += __shfl_xor(w[i], O, G);
+            if (l & O) writer = false;
+            spmm_halving_sum<N, O / 2, G, T>(w, l, base, writer);
+        }
+    }
+}
+
+template <int DIM, int G, class T, bool HALVE = true>
 __global__ __launch_bounds__(256) void k_spmm_bsr(int64_t n_rows, const int32_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
                                                   const T *__restrict__ X, int64_t ldx, T *__restrict__ Y,
@@ -542,18 +577,47 @@ __global__ __launch_bounds__(256) void k_spmm_bsr(int64_t n_rows, const int32_t 
             }
         }
     }
+    // Sum the G lanes' partials of all kSpmmNV*DIM values by halving: at each
+    // exchange distance o a lane keeps half of its values (the half chosen by
+    // its bit o) and adds the partner's copy of that half; once the count is
+    // odd the remaining steps exchange everything.  Every value is summed over
+    // the same pairs, in the same order, as the full xor butterfly (lane + its
+    // partner at each o), so the results are the same bits, for a fraction of
+    // the shuffles (3D, 16 lanes: 24 instead of 96), and the writes end up
+    // spread over the lanes that hold the final values.
+    if (!HALVE) {
+        // the full xor butterfly on every value, lane 0 writing the row
+#pragma unroll
+        for (int s = 0; s < kSpmmNV; ++s)
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                for (int o = G / 2; o > 0; o >>= 1) acc[s][i] += __shfl_xor(acc[s][i], o, G);
+        if (r < n_rows && l == 0) {
+#pragma unroll
+            for (int s = 0; s < kSpmmNV; ++s)
+                if (s < nv)
+#pragma unroll
+                    for (int i = 0; i < DIM; ++i) Y[s * ldy + r * DIM + i] = acc[s][i];
+        }
+        return;
+    }
+    constexpr int NVAL = kSpmmNV * DIM;
+    T w[NVAL];
 #pragma unroll
     for (int s = 0; s < kSpmmNV; ++s)
 #pragma unroll
-        for (int i = 0; i < DIM; ++i)
+        for (int i = 0; i < DIM; ++i) w[s * DIM + i] = acc[s][i];
+    int base = 0;      // flat index of the lane's first kept value
+    bool writer = true;
+    spmm_halving_sum<NVAL, G / 2, G, T>(w, l, base, writer);
+    constexpr int NFIN = spmm_final_count(NVAL, G / 2);
+    if (r < n_rows && writer) {
 #pragma unroll
-            for (int o = G / 2; o > 0; o >>= 1) acc[s][i] += __shfl_xor(acc[s][i], o, G);
-    if (r < n_rows && l == 0) {
-#pragma unroll
-        for (int s = 0; s < kSpmmNV; ++s)
-            if (s < nv)
-#pragma unroll
-                for (int i = 0; i < DIM; ++i) Y[s * ldy + r * DIM + i] = acc[s][i];
+        for (int t = 0; t < NFIN; ++t) {
+            const int f = base + t, s = f / DIM, i = f - (f / DIM) * DIM;
+            if (s < nv) Y[s * ldy + r * DIM + i] = w[t];
+        }
     }
 }
 
@@ -1240,8 +1304,15 @@ int spmm_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
     if (n_rows == 0 || nv == 0) return SSR_OK;
     const int G = ssr_tuning(SSR_TUNE_SPMM_GROUP) > 0 ? ssr_tuning(SSR_TUNE_SPMM_GROUP) : (dim == 3 ? 16 : 4);
     const dim3 grid(ssr_blocks(n_rows * G, 256)), blk(256);
-#define SPMM(D_, G_) hipLaunchKernelGGL((k_spmm_bsr<D_, G_, T>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, \
-                                        val, X, ldx, Y, ldy, nv)
+#define SPMM(D_, G_)                                                                                          \
+    do {                                                                                                      \
+        if (ssr_tuning(SSR_TUNE_SPMM_REDUCE))                                                                 \
+            hipLaunchKernelGGL((k_spmm_bsr<D_, G_, T, true>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, \
+                               X, ldx, Y, ldy, nv);                                                           \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_spmm_bsr<D_, G_, T, false>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col,     \
+                               val, X, ldx, Y, ldy, nv);                                                      \
+    } while (0)
     if (dim == 3) {
         if (G == 4) SPMM(3, 4);
         else if (G == 8) SPMM(3, 8);

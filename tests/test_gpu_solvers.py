@@ -170,6 +170,35 @@ def test_A_orthogonalize_large_basis_matches_oracle(spmm_group):
         _lib.set_tuning(_lib.TUNE_SPMM_GROUP, old)
 
 
+@pytest.mark.parametrize("dim,spmm_group", [(3, 0), (3, 8), (3, 32), (2, 0), (2, 16)])
+def test_A_orthogonalize_spmm_halving_sum_same_bits(dim, spmm_group):
+    """Tuning key 12: the multi-vector SpMV's lane sums by halving exchanges
+    give the basis of the full-butterfly form bit for bit (70 columns: full
+    and partial 8-vector passes; lane shapes with odd and even final counts)."""
+    from slope_stability_octave_amd import _lib
+    cmb, K, Kq, qf = operator(dim, counts=(2, 2, 2, 2, 2, 2)) if dim == 3 else operator(dim)
+    rng = np.random.default_rng(9)
+    n = cmb.pattern.n_dof
+    Wq = rng.normal(size=(qf.size, 70))
+    out = {}
+    old = (_lib.get_tuning(_lib.TUNE_SPMM_GROUP), _lib.get_tuning(_lib.TUNE_SPMM_REDUCE))
+    try:
+        _lib.set_tuning(_lib.TUNE_SPMM_GROUP, spmm_group)
+        for mode in (0, 1):
+            _lib.set_tuning(_lib.TUNE_SPMM_REDUCE, mode)
+            s = DFGMRES(cmb.pattern, 1e-8, 100, 1e-3)
+            s.setup_preconditioner(K)
+            for c in range(Wq.shape[1]):
+                s.expand_deflation_basis(DeviceArray.from_host(_full(Wq[:, c], qf, n, dim)))
+            s.A_orthogonalize(K)
+            out[mode] = s.deflation_basis()
+    finally:
+        _lib.set_tuning(_lib.TUNE_SPMM_GROUP, old[0])
+        _lib.set_tuning(_lib.TUNE_SPMM_REDUCE, old[1])
+    assert out[0].shape == out[1].shape and out[0].shape[1] == 70
+    assert np.array_equal(out[0], out[1])
+
+
 def _check_A_orthogonalize_large_basis(KO):
     cmb, K, Kq, qf = operator(3, counts=(2, 2, 2, 2, 2, 2))
     rng = np.random.default_rng(5)
