@@ -141,7 +141,8 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *  key 11: node-block SpMV with its block values staged through LDS by
  *         16-byte loads (1; for the default lane shapes) or read by each
  *         lane (0, default);
- *  key 12: lane sums of the multi-vector SpMV by halving exchanges (1,
+ *  key 12: lane-group sums of the multi-vector SpMV and of the block-CSR
+ *         SpMV (AMG coarse levels and transfers) by halving exchanges (1,
  *         default) or by a full butterfly per value (0): the same bits. */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);

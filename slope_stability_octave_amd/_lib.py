@@ -204,7 +204,7 @@ TUNE_ELEMENT_KERNEL = 8
 TUNE_AMG_COARSE_BLOCKS = 9
 TUNE_AMG_DEVICE_SETUP = 10
 TUNE_SPMV_STAGE = 11
-TUNE_SPMM_REDUCE = 12
+TUNE_HALVING_SUMS = 12
 
 
 def set_tuning(key: int, value: int):

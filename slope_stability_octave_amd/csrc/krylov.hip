@@ -41,6 +41,47 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     return s;
 }
 
+// values left per lane after the halving steps from distance o down to 1
+__host__ __device__ constexpr int group_final_count(int n, int o)
+{
+    return o == 0 ? n : group_final_count(n % 2 == 0 ? n / 2 : n, o / 2);
+}
+
+// Sum of N values over a group of G lanes by halving exchanges: at distance
+// O a lane keeps half of its values (the half chosen by its bit O) and adds
+// the partner's copy of that half; once the count is odd the remaining
+// distances exchange everything.  Every value is summed over the same pairs,
+// in the same order, as the full xor butterfly (lane + partner at each
+// distance), so the sums are the same bits, for fewer shuffles, and the
+// final values end up spread over the writer lanes.  This is distance O, then
+// the smaller ones.  w[0..N) are the lane's current values; on return
+// w[0..group_final_count(N, O)) hold its kept sums, whose flat indices start
+// at base.  After a step that cannot halve, only the lane with bit O clear
+// writes (writer).
+template <int N, int O, int G, class T, int NA>
+__device__ __forceinline__ void group_halving_sum(T (&w)[NA], int l, int &base, bool &writer)
+{
+    if constexpr (O > 0) {
+        if constexpr (N % 2 == 0) {
+            constexpr int H = N / 2;
+            const bool hi = (l & O) != 0;
+#pragma unroll
+            for (int i = 0; i < H; ++i) {
+                const T send = hi ? w[i] : w[H + i];
+                const T keep = hi ? w[H + i] : w[i];
+                w[i] = keep + __shfl_xor(send, O, G);
+            }
+            if (hi) base += H;
+            group_halving_sum<H, O / 2, G, T>(w, l, base, writer);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) w[i] += __shfl_xor(w[i], O, G);
+            if (l & O) writer = false;
+            group_halving_sum<N, O / 2, G, T>(w, l, base, writer);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // SpMV y = alpha K x + beta z on node-block CSR.  A group of G lanes handles
 // one block row: lanes stride over the row's blocks, each forming dim partial
@@ -280,7 +321,7 @@ __global__ __launch_bounds__(256) void k_spmv_bsr_w(int64_t n_rows, const int32_
 // 3D, 3 in 2D) and nodes: one column index per block instead of one per
 // entry, 8.1 instead of 12 bytes per FP64 value of a 6 x 6 block.  G lanes per
 // block row as in k_spmv_bsr.
-template <int RB, int CB, int G, class T>
+template <int RB, int CB, int G, class T, bool HALVE = true>
 __global__ __launch_bounds__(256) void k_spmv_brc(int64_t n_rows, const int32_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
                                                   const T *__restrict__ x, T *y, double alpha, double beta,
@@ -306,14 +347,32 @@ __global__ __launch_bounds__(256) void k_spmv_brc(int64_t n_rows, const int32_t 
                 for (int j = 0; j < CB; ++j) acc[i] += v[i * CB + j] * xv[j];
         }
     }
+    if (!HALVE) {
+        // the full xor butterfly on every row, lane 0 writing
 #pragma unroll
-    for (int i = 0; i < RB; ++i)
+        for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, G);
-    if (r < n_rows && l == 0) {
+            for (int o = G / 2; o > 0; o >>= 1) acc[i] += __shfl_xor(acc[i], o, G);
+        if (r < n_rows && l == 0) {
 #pragma unroll
-        for (int i = 0; i < RB; ++i) {
-            const T o = T(alpha) * acc[i];
+            for (int i = 0; i < RB; ++i) {
+                const T o = T(alpha) * acc[i];
+                y[r * RB + i] = beta == 0.0 ? o : o + T(beta) * z[r * RB + i];
+            }
+        }
+        return;
+    }
+    // the lanes' partials summed by halving exchanges (group_halving_sum: the
+    // butterfly's bits; 6 rows over 64 lanes: 18 shuffles instead of 36)
+    int base = 0;
+    bool writer = true;
+    group_halving_sum<RB, G / 2, G, T>(acc, l, base, writer);
+    constexpr int NFIN = group_final_count(RB, G / 2);
+    if (r < n_rows && writer) {
+#pragma unroll
+        for (int t = 0; t < NFIN; ++t) {
+            const int i = base + t;
+            const T o = T(alpha) * acc[t];
             y[r * RB + i] = beta == 0.0 ? o : o + T(beta) * z[r * RB + i];
         }
     }
@@ -507,41 +566,6 @@ __global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV 
 // ---------------------------------------------------------------------------
 constexpr int kSpmmNV = 8;
 
-// values left per lane after the halving steps from distance o down to 1
-__host__ __device__ constexpr int spmm_final_count(int n, int o)
-{
-    return o == 0 ? n : spmm_final_count(n % 2 == 0 ? n / 2 : n, o / 2);
-}
-
-// One exchange distance O of the halving sum of k_spmm_bsr, then the smaller
-// ones.  w[0..N) are the lane's current values; on return w[0..final) hold
-// its kept sums, whose flat indices start at base.  A step that cannot halve
-// (N odd) adds the partner's copy of every value, and from then on only the
-// lane with bit O clear writes (writer).
-template <int N, int O, int G, class T, int NA>
-__device__ __forceinline__ void spmm_halving_sum(T (&w)[NA], int l, int &base, bool &writer)
-{
-    if constexpr (O > 0) {
-        if constexpr (N % 2 == 0) {
-            constexpr int H = N / 2;
-            const bool hi = (l & O) != 0;
-#pragma unroll
-            for (int i = 0; i < H; ++i) {
-                const T send = hi ? w[i] : w[H + i];
-                const T keep = hi ? w[H + i] : w[i];
-                w[i] = keep + __shfl_xor(send, O, G);
-            }
-            if (hi) base += H;
-            spmm_halving_sum<H, O / 2, G, T>(w, l, base, writer);
-        } else {
-#pragma unroll
-            for (int i = 0; i < N; ++i) w[i] += __shfl_xor(w[i], O, G);
-            if (l & O) writer = false;
-            spmm_halving_sum<N, O / 2, G, T>(w, l, base, writer);
-        }
-    }
-}
-
 template <int DIM, int G, class T, bool HALVE = true>
 __global__ __launch_bounds__(256) void k_spmm_bsr(int64_t n_rows, const int32_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
@@ -577,14 +601,9 @@ __global__ __launch_bounds__(256) void k_spmm_bsr(int64_t n_rows, const int32_t 
             }
         }
     }
-    // Sum the G lanes' partials of all kSpmmNV*DIM values by halving: at each
-    // exchange distance o a lane keeps half of its values (the half chosen by
-    // its bit o) and adds the partner's copy of that half; once the count is
-    // odd the remaining steps exchange everything.  Every value is summed over
-    // the same pairs, in the same order, as the full xor butterfly (lane + its
-    // partner at each o), so the results are the same bits, for a fraction of
-    // the shuffles (3D, 16 lanes: 24 instead of 96), and the writes end up
-    // spread over the lanes that hold the final values.
+    // the G lanes' partials of all kSpmmNV*DIM values summed by halving
+    // exchanges (group_halving_sum: the butterfly's bits; 3D, 16 lanes: 24
+    // shuffles instead of 96)
     if (!HALVE) {
         // the full xor butterfly on every value, lane 0 writing the row
 #pragma unroll
@@ -610,8 +629,8 @@ __global__ __launch_bounds__(256) void k_spmm_bsr(int64_t n_rows, const int32_t 
         for (int i = 0; i < DIM; ++i) w[s * DIM + i] = acc[s][i];
     int base = 0;      // flat index of the lane's first kept value
     bool writer = true;
-    spmm_halving_sum<NVAL, G / 2, G, T>(w, l, base, writer);
-    constexpr int NFIN = spmm_final_count(NVAL, G / 2);
+    group_halving_sum<NVAL, G / 2, G, T>(w, l, base, writer);
+    constexpr int NFIN = group_final_count(NVAL, G / 2);
     if (r < n_rows && writer) {
 #pragma unroll
         for (int t = 0; t < NFIN; ++t) {
@@ -1112,24 +1131,24 @@ int spmv_brc(int rb, int cb, int64_t n_rows, const int32_t *rp, const int32_t *c
     // about 70 blocks): more lanes per row until the grid fills the device
     while (G < 64 && n_rows * G < (int64_t)ssr_cu_count() * 2048 && avg > G / 2) G *= 2;
     const dim3 blk(256);
+    const bool halve = ssr_tuning(SSR_TUNE_HALVING_SUMS) != 0;
+#define BRC_G(RB_, CB_, G_)                                                                                        \
+    do {                                                                                                           \
+        const dim3 grid(ssr_blocks(n_rows * G_, 256));                                                             \
+        if (halve)                                                                                                 \
+            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, G_, T, true>), grid, blk, 0, ssr_stream(), n_rows, rp, col, val, \
+                               x, y, alpha, beta, z);                                                              \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, G_, T, false>), grid, blk, 0, ssr_stream(), n_rows, rp, col,    \
+                               val, x, y, alpha, beta, z);                                                         \
+    } while (0)
 #define BRC(RB_, CB_)                                                                                              \
     do {                                                                                                           \
-        const dim3 grid(ssr_blocks(n_rows * G, 256));                                                              \
-        if (G == 4)                                                                                                \
-            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, 4, T>), grid, blk, 0, ssr_stream(), n_rows, rp, col, val, x, y, \
-                               alpha, beta, z);                                                                    \
-        else if (G == 8)                                                                                           \
-            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, 8, T>), grid, blk, 0, ssr_stream(), n_rows, rp, col, val, x, y, \
-                               alpha, beta, z);                                                                    \
-        else if (G == 16)                                                                                          \
-            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, 16, T>), grid, blk, 0, ssr_stream(), n_rows, rp, col, val, x, \
-                               y, alpha, beta, z);                                                                 \
-        else if (G == 32)                                                                                          \
-            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, 32, T>), grid, blk, 0, ssr_stream(), n_rows, rp, col, val, x, \
-                               y, alpha, beta, z);                                                                 \
-        else                                                                                                       \
-            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, 64, T>), grid, blk, 0, ssr_stream(), n_rows, rp, col, val, x, \
-                               y, alpha, beta, z);                                                                 \
+        if (G == 4) BRC_G(RB_, CB_, 4);                                                                            \
+        else if (G == 8) BRC_G(RB_, CB_, 8);                                                                       \
+        else if (G == 16) BRC_G(RB_, CB_, 16);                                                                     \
+        else if (G == 32) BRC_G(RB_, CB_, 32);                                                                     \
+        else BRC_G(RB_, CB_, 64);                                                                                  \
     } while (0)
     if (rb == 6 && cb == 6) BRC(6, 6);
     else if (rb == 3 && cb == 6) BRC(3, 6);
@@ -1142,6 +1161,7 @@ int spmv_brc(int rb, int cb, int64_t n_rows, const int32_t *rp, const int32_t *c
         return SSR_ERR_ARG;
     }
 #undef BRC
+#undef BRC_G
     SSR_HIP_CHECK(hipGetLastError());
     return SSR_OK;
 }
@@ -1306,7 +1326,7 @@ int spmm_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col
     const dim3 grid(ssr_blocks(n_rows * G, 256)), blk(256);
 #define SPMM(D_, G_)                                                                                          \
     do {                                                                                                      \
-        if (ssr_tuning(SSR_TUNE_SPMM_REDUCE))                                                                 \
+        if (ssr_tuning(SSR_TUNE_HALVING_SUMS))                                                                 \
             hipLaunchKernelGGL((k_spmm_bsr<D_, G_, T, true>), grid, blk, 0, ssr_stream(), n_rows, row_ptr, col, val, \
                                X, ldx, Y, ldy, nv);                                                           \
         else                                                                                                  \

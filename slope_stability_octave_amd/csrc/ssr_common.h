@@ -84,7 +84,7 @@ struct ssr_precond {
 #define SSR_TUNE_AMG_DEVICE_SETUP 10     /* 1: build the AMG hierarchy on the device (default), 0: on the host */
 #define SSR_TUNE_SPMV_STAGE 11          /* 1: node-block SpMV with the block values staged through LDS,
                                            0: read from global memory by each lane (default) */
-#define SSR_TUNE_SPMM_REDUCE 12         /* multi-vector SpMV lane sums: 1 = halving exchange (default), 0 = full
-                                           butterfly per value; the same bits */
+#define SSR_TUNE_HALVING_SUMS 12        /* lane-group sums of the multi-vector and block-CSR SpMVs: 1 = halving
+                                           exchanges (default), 0 = full butterfly per value; the same bits */
 #define SSR_TUNE_COUNT 13
 int ssr_tuning(int key);

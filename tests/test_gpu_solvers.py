@@ -181,11 +181,11 @@ def test_A_orthogonalize_spmm_halving_sum_same_bits(dim, spmm_group):
     n = cmb.pattern.n_dof
     Wq = rng.normal(size=(qf.size, 70))
     out = {}
-    old = (_lib.get_tuning(_lib.TUNE_SPMM_GROUP), _lib.get_tuning(_lib.TUNE_SPMM_REDUCE))
+    old = (_lib.get_tuning(_lib.TUNE_SPMM_GROUP), _lib.get_tuning(_lib.TUNE_HALVING_SUMS))
     try:
         _lib.set_tuning(_lib.TUNE_SPMM_GROUP, spmm_group)
         for mode in (0, 1):
-            _lib.set_tuning(_lib.TUNE_SPMM_REDUCE, mode)
+            _lib.set_tuning(_lib.TUNE_HALVING_SUMS, mode)
             s = DFGMRES(cmb.pattern, 1e-8, 100, 1e-3)
             s.setup_preconditioner(K)
             for c in range(Wq.shape[1]):
@@ -194,7 +194,7 @@ def test_A_orthogonalize_spmm_halving_sum_same_bits(dim, spmm_group):
             out[mode] = s.deflation_basis()
     finally:
         _lib.set_tuning(_lib.TUNE_SPMM_GROUP, old[0])
-        _lib.set_tuning(_lib.TUNE_SPMM_REDUCE, old[1])
+        _lib.set_tuning(_lib.TUNE_HALVING_SUMS, old[1])
     assert out[0].shape == out[1].shape and out[0].shape[1] == 70
     assert np.array_equal(out[0], out[1])
 
@@ -523,3 +523,46 @@ def test_float_multidot_and_basis_update(k):
     got_u = ud.get()
     assert np.abs(got_u.astype(np.float64) - want_u).max() <= 2e-6 * np.abs(want_u).max()
 
+
+
+@pytest.mark.parametrize("prec,dim", [(64, 3), (32, 3), (64, 2), (32, 2)])
+def test_vcycle_halving_sums_same_bits(prec, dim):
+    """Tuning key 12 on the V-cycle: the block-CSR SpMVs of the coarse levels,
+    prolongators and restrictions with lane sums by halving exchanges give the
+    full butterfly's results bit for bit (one V-cycle in double; a whole float
+    solve, V-cycles in float, in precision 32)."""
+    from slope_stability_octave_amd import _lib
+    if dim == 3:
+        prob = problems.slope_3d((3, 3, 3, 3, 3, 3), 3.0, 3.0, 3.0, 3.0, 3.0, 3.0, problems.HOMO_2D)
+    else:
+        prob = problems.slope_2d(h=0.5)
+    cmb = problems.make_constitutive(prob)
+    cmb.reduction(1.0)
+    U = DeviceArray((cmb.n_n, cmb.dim), zero=True)
+    cmb.build_F_and_DS_all(1.0, U)
+    K = cmb.build_K_r_QQ_vals(1e-4)
+    rng = np.random.default_rng(13)
+    x = DeviceArray.from_host(rng.normal(size=(cmb.n_n, cmb.dim)) * cmb.Q.T)
+    old = _lib.get_tuning(_lib.TUNE_HALVING_SUMS)
+    out = {}
+    try:
+        for mode in (0, 1):
+            _lib.set_tuning(_lib.TUNE_HALVING_SUMS, mode)
+            s = DFGMRES(cmb.pattern, 1e-8, 200, 1e-3, "amg", amg_options=dict(coord=prob["coord"], max_coarse=50),
+                        precision=prec)
+            s.setup_preconditioner(K)
+            if prec == 64:
+                y = DeviceArray((cmb.n_n, cmb.dim))
+                _lib.call("ssr_solver_precond_apply", s.handle, _lib.ptr(x), _lib.ptr(y))
+                out[mode] = y.get()
+            else:
+                xs, it = s.solve(K, x)
+                out[mode] = (xs.get(), it)
+    finally:
+        _lib.set_tuning(_lib.TUNE_HALVING_SUMS, old)
+    if prec == 64:
+        assert np.abs(out[1]).max() > 0
+        assert np.array_equal(out[0], out[1])
+    else:
+        assert out[0][1] == out[1][1]
+        assert np.array_equal(out[0][0], out[1][0])
