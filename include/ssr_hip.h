@@ -143,7 +143,9 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         lane (0, default);
  *  key 12: lane-group sums of the multi-vector SpMV and of the block-CSR
  *         SpMV (AMG coarse levels and transfers) by halving exchanges (1,
- *         default) or by a full butterfly per value (0): the same bits. */
+ *         default) or by a full butterfly per value (0): the same bits;
+ *  key 13: finest-level Chebyshev block step with one thread per unknown (1,
+ *         default) or per node (0): the same bits. */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 

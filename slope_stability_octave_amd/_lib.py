@@ -205,6 +205,7 @@ TUNE_AMG_COARSE_BLOCKS = 9
 TUNE_AMG_DEVICE_SETUP = 10
 TUNE_SPMV_STAGE = 11
 TUNE_HALVING_SUMS = 12
+TUNE_CHEB_PER_DOF = 13
 
 
 def set_tuning(key: int, value: int):

@@ -945,6 +945,27 @@ __global__ void k_cheb_block(int64_t n_n, double c2, double c3, const T *__restr
     }
 }
 
+// k_cheb_block with one thread per unknown q = dim*node + i: the thread
+// reads row i of the node's block and the node's three residual entries and
+// forms the same sum and update as k_cheb_block's row i (the same bits), and
+// consecutive lanes read and write consecutive entries of r, d and x.
+template <int DIM, class T>
+__global__ void k_cheb_block_dof(int64_t n_n, double c2, double c3, const T *__restrict__ Binv,
+                                 const T *__restrict__ r, T *__restrict__ d, T *__restrict__ x, int first)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_n * DIM) return;
+    const int64_t nd = q / DIM;
+    const int i = (int)(q - nd * DIM);
+    const T *B = Binv + nd * DIM * DIM + i * DIM;
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) s += B[j] * r[nd * DIM + j];
+    const T dn = (first || c2 == 0.0) ? T(c3) * s : T(c2) * d[q] + T(c3) * s;
+    d[q] = dn;
+    x[q] = first ? dn : x[q] + dn;
+}
+
 // y = M x for a dense n x n row-major matrix (one row per wavefront)
 template <class T>
 __global__ __launch_bounds__(256) void k_dense_gemv(int n, const T *__restrict__ M, const T *__restrict__ x,
@@ -1300,6 +1321,10 @@ int cheb_step(int64_t n, double c2, double c3, const T *dinv, const T *r, T *d, 
 template <class T>
 int cheb_block_step(int dim, int64_t n_n, double c2, double c3, const T *Binv, const T *r, T *d, T *x, int first)
 {
+    if (ssr_tuning(SSR_TUNE_CHEB_PER_DOF)) {
+        if (dim == 3) SSR_LAUNCH1((k_cheb_block_dof<3, T>), n_n * 3, n_n, c2, c3, Binv, r, d, x, first);
+        SSR_LAUNCH1((k_cheb_block_dof<2, T>), n_n * 2, n_n, c2, c3, Binv, r, d, x, first);
+    }
     if (dim == 3) SSR_LAUNCH1((k_cheb_block<3, T>), n_n, n_n, c2, c3, Binv, r, d, x, first);
     SSR_LAUNCH1((k_cheb_block<2, T>), n_n, n_n, c2, c3, Binv, r, d, x, first);
 }

@@ -86,5 +86,7 @@ struct ssr_precond {
                                            0: read from global memory by each lane (default) */
 #define SSR_TUNE_HALVING_SUMS 12        /* lane-group sums of the multi-vector and block-CSR SpMVs: 1 = halving
                                            exchanges (default), 0 = full butterfly per value; the same bits */
-#define SSR_TUNE_COUNT 13
+#define SSR_TUNE_CHEB_PER_DOF 13        /* finest-level Chebyshev block step: 1 = one thread per unknown (default),
+                                           0 = one thread per node; the same bits */
+#define SSR_TUNE_COUNT 14
 int ssr_tuning(int key);

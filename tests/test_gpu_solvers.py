@@ -525,12 +525,14 @@ def test_float_multidot_and_basis_update(k):
 
 
 
+@pytest.mark.parametrize("key", ["TUNE_HALVING_SUMS", "TUNE_CHEB_PER_DOF"])
 @pytest.mark.parametrize("prec,dim", [(64, 3), (32, 3), (64, 2), (32, 2)])
-def test_vcycle_halving_sums_same_bits(prec, dim):
-    """Tuning key 12 on the V-cycle: the block-CSR SpMVs of the coarse levels,
-    prolongators and restrictions with lane sums by halving exchanges give the
-    full butterfly's results bit for bit (one V-cycle in double; a whole float
-    solve, V-cycles in float, in precision 32)."""
+def test_vcycle_halving_sums_same_bits(prec, dim, key):
+    """Tuning keys 12 and 13 on the V-cycle: the block-CSR SpMVs of the coarse
+    levels, prolongators and restrictions with lane sums by halving exchanges
+    (12), and the finest-level Chebyshev block step with one thread per
+    unknown (13), give the other forms' results bit for bit (one V-cycle in
+    double; a whole float solve, V-cycles in float, in precision 32)."""
     from slope_stability_octave_amd import _lib
     if dim == 3:
         prob = problems.slope_3d((3, 3, 3, 3, 3, 3), 3.0, 3.0, 3.0, 3.0, 3.0, 3.0, problems.HOMO_2D)
@@ -543,11 +545,11 @@ def test_vcycle_halving_sums_same_bits(prec, dim):
     K = cmb.build_K_r_QQ_vals(1e-4)
     rng = np.random.default_rng(13)
     x = DeviceArray.from_host(rng.normal(size=(cmb.n_n, cmb.dim)) * cmb.Q.T)
-    old = _lib.get_tuning(_lib.TUNE_HALVING_SUMS)
+    old = _lib.get_tuning(getattr(_lib, key))
     out = {}
     try:
         for mode in (0, 1):
-            _lib.set_tuning(_lib.TUNE_HALVING_SUMS, mode)
+            _lib.set_tuning(getattr(_lib, key), mode)
             s = DFGMRES(cmb.pattern, 1e-8, 200, 1e-3, "amg", amg_options=dict(coord=prob["coord"], max_coarse=50),
                         precision=prec)
             s.setup_preconditioner(K)
@@ -559,7 +561,7 @@ def test_vcycle_halving_sums_same_bits(prec, dim):
                 xs, it = s.solve(K, x)
                 out[mode] = (xs.get(), it)
     finally:
-        _lib.set_tuning(_lib.TUNE_HALVING_SUMS, old)
+        _lib.set_tuning(getattr(_lib, key), old)
     if prec == 64:
         assert np.abs(out[1]).max() > 0
         assert np.array_equal(out[0], out[1])
