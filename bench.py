@@ -598,6 +598,8 @@ def run_batch(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a kernel-form / launch-shape knob (ssr_set_tuning) before the run; repeatable")
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=N_HEX[0] * N_HEX[1] * N_HEX[2] * 6 * N_Q)
@@ -659,6 +661,10 @@ def main():
               "MI355X; the multi-GPU decomposition is a design note, not a launch mode)",
               file=sys.stderr)
         sys.exit(2)
+    for kv in args.tune:
+        from slope_stability_octave_amd import _lib as L
+        k, v = (int(t) for t in kv.split("="))
+        L.set_tuning(k, v)
     if args.variant is not None:
         from slope_stability_octave_amd import _lib as L
         L.set_tuning(L.TUNE_CONSTITUTIVE_VARIANT, args.variant)

@@ -145,7 +145,13 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         SpMV (AMG coarse levels and transfers) by halving exchanges (1,
  *         default) or by a full butterfly per value (0): the same bits;
  *  key 13: finest-level Chebyshev block step with one thread per unknown (1,
- *         default) or per node (0): the same bits. */
+ *         default) or per node (0): the same bits;
+ *  key 14: node-block SpMV of a solver's symmetric operator (and of the AMG
+ *         finest level it hands over) reading only the blocks on and above
+ *         the block diagonal, the others as transposed mirrors (1) or every
+ *         block (0, default): the same bits for an operator whose mirrored
+ *         blocks are transposes bit for bit, as K_r is (slower on the
+ *         structured slope meshes, whose mirrors lie far apart). */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 
@@ -252,6 +258,19 @@ int ssr_dev_spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int3
 int ssr_dev_spmv_bsr_f32(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col,
                          const float *val, const float *x, float *y, double alpha, double beta);
 int ssr_dev_convert_f64_f32(int64_t n, const double *x, float *y);
+/* The same products for a symmetric operator (librsb's symmetric SpMV behind
+ * the same call sites): diag = the diagonal block of each row, tblk = the
+ * transposed block of each block (pattern tables 4 and 3).  With tuning key
+ * 14 set, blocks left of the diagonal are read as the transposes of their
+ * mirrors, so only the blocks on and above the diagonal stream from memory;
+ * for values whose mirrored blocks are transposes bit for bit the result is
+ * the same bits as ssr_dev_spmv_bsr. */
+int ssr_dev_spmv_bsr_sym(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col,
+                         const int32_t *diag, const int32_t *tblk, const double *val, const double *x,
+                         double *y, double alpha, double beta);
+int ssr_dev_spmv_bsr_sym_f32(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col,
+                             const int32_t *diag, const int32_t *tblk, const float *val, const float *x,
+                             float *y, double alpha, double beta);
 int ssr_dev_dot(int64_t n, const double *a, const double *b, const double *mask, double *result);
 int ssr_dot(int64_t n, const double *a, const double *b, const double *mask, double *host_result);
 /* max over nodes of the Euclidean norm of U's node vector (dim x n_n node-major), to the

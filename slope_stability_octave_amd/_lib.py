@@ -206,6 +206,7 @@ TUNE_AMG_DEVICE_SETUP = 10
 TUNE_SPMV_STAGE = 11
 TUNE_HALVING_SUMS = 12
 TUNE_CHEB_PER_DOF = 13
+TUNE_SPMV_MIRROR = 14
 
 
 def set_tuning(key: int, value: int):

@@ -44,6 +44,7 @@ void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_
                        int64_t *nnz_free);
 
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_brow(const ssr_pattern *p);
 
 extern "C" {
@@ -509,6 +510,7 @@ struct Amg : ssr_precond {
     } Kh;
     int64_t n_setup = 0;
     int64_t generation = 0;  // bumped whenever device level data is (re)uploaded
+    bool fine_sym = false;   // finest-level values symmetric bit for bit (mirror SpMV allowed)
     bool built = false;
     bool on_device = false;  // the hierarchy was built on the device (amg_device.hip)
     double *d_B = nullptr;   // device copy of the near-null space (device build)
@@ -1179,6 +1181,7 @@ struct Amg : ssr_precond {
     }
 
     void invalidate() override { force_rebuild = true; }
+    void set_symmetric(bool v) override { fine_sym = v; }
 
     int setup(const double *Kvals, const float *K32) override
     {
@@ -1240,7 +1243,8 @@ struct Amg : ssr_precond {
     {
         if (l == 0)
             return ssr::spmv_bsr<T>(dim, n_n, ssr_pattern_d_row_ptr(Pat), ssr_pattern_d_col(Pat), fine_vals<T>(), x, y,
-                                    a, b, z);
+                                    a, b, z, fine_sym ? ssr_pattern_d_diag(Pat) : nullptr,
+                                    fine_sym ? ssr_pattern_d_tblk(Pat) : nullptr);
         return L[l]->dev<T>().A.spmv(x, y, a, b, z);
     }
 

@@ -7,10 +7,14 @@
 namespace ssr {
 
 // y = alpha*K*x + beta*z on the node-block CSR (dim x dim row-major blocks);
-// z == nullptr means z = y.
+// z == nullptr means z = y.  With diag (diagonal block per row) and tblk
+// (transposed block per block) given, an operator whose mirrored blocks are
+// transposes bit for bit may be read from its blocks on and above the
+// diagonal only (SSR_TUNE_SPMV_MIRROR): the same result bits.
 template <class T>
 int spmv_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const T *val, const T *x,
-             T *y, double alpha, double beta, const T *z = nullptr);
+             T *y, double alpha, double beta, const T *z = nullptr, const int32_t *diag = nullptr,
+             const int32_t *tblk = nullptr);
 // Y(:, s) = K X(:, s) for s < nv <= 8 (columns at strides ldx, ldy)
 template <class T>
 int spmm_bsr(int dim, int64_t n_rows, const int32_t *row_ptr, const int32_t *col, const T *val, const T *X,

@@ -47,6 +47,7 @@ struct ssr_pattern;
 const int32_t *ssr_pattern_d_row_ptr(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_col(const ssr_pattern *p);
 const int32_t *ssr_pattern_d_diag(const ssr_pattern *p);
+const int32_t *ssr_pattern_d_tblk(const ssr_pattern *p);
 const double *ssr_pattern_d_mask(const ssr_pattern *p);
 void ssr_pattern_sizes(const ssr_pattern *p, int *dim, int64_t *n_n, int64_t *n_p, int64_t *n_e,
                        int64_t *nnzb, int64_t *nnz_free);
@@ -181,7 +182,9 @@ template <class T>
 int spmv(ssr_solver *s, const T *x, T *y, double alpha = 1.0, double beta = 0.0, const T *z = nullptr)
 {
     s->n_spmv++;
-    return ssr::spmv_bsr<T>(s->dim, s->n_n, s->row_ptr, s->col, op_vals<T>(s), x, y, alpha, beta, z);
+    return ssr::spmv_bsr<T>(s->dim, s->n_n, s->row_ptr, s->col, op_vals<T>(s), x, y, alpha, beta, z,
+                            s->symmetric ? ssr_pattern_d_diag(s->P) : nullptr,
+                            s->symmetric ? ssr_pattern_d_tblk(s->P) : nullptr);
 }
 
 double hnorm(ssr_solver *s, const double *x, int *rc)
@@ -499,7 +502,10 @@ int ssr_solver_set_matrix(ssr_solver *s, const double *Kvals, int setup_precond)
         if ((rc = ssr::convert<double, float>(nv, 1.0, Kvals, s->K32.p))) return rc;
         s->K32_valid = true;
     }
-    if (setup_precond) return s->M->setup(Kvals, s->precision == 32 ? s->K32.p : nullptr);
+    if (setup_precond) {
+        s->M->set_symmetric(s->symmetric);
+        return s->M->setup(Kvals, s->precision == 32 ? s->K32.p : nullptr);
+    }
     return SSR_OK;
 }
 

@@ -61,6 +61,8 @@ struct ssr_precond {
     virtual int apply(const float *x, float *y) = 0;    // same in float
     // the next setup rebuilds whatever it would otherwise keep
     virtual void invalidate() {}
+    // the values of the next setups are symmetric bit for bit (K_r)
+    virtual void set_symmetric(bool) {}
 };
 
 // Kernel-variant / launch-shape knobs (set by the autotuner or the caller).
@@ -88,5 +90,8 @@ struct ssr_precond {
                                            exchanges (default), 0 = full butterfly per value; the same bits */
 #define SSR_TUNE_CHEB_PER_DOF 13        /* finest-level Chebyshev block step: 1 = one thread per unknown (default),
                                            0 = one thread per node; the same bits */
-#define SSR_TUNE_COUNT 14
+#define SSR_TUNE_SPMV_MIRROR 14         /* 1: node-block SpMV of a symmetric operator reads the blocks on and
+                                           above the diagonal only, the others as transposed mirrors; the
+                                           same bits; 0 (default): every block */
+#define SSR_TUNE_COUNT 15
 int ssr_tuning(int key);

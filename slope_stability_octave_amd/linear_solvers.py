@@ -45,6 +45,8 @@ _lib.extend_signatures({
     "ssr_dev_spmv_bsr": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
     "ssr_dev_dot": [_i64, _vp, _vp, _vp, _vp],
     "ssr_dev_spmv_bsr_f32": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
+    "ssr_dev_spmv_bsr_sym": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
+    "ssr_dev_spmv_bsr_sym_f32": [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl],
     "ssr_dev_convert_f64_f32": [_i64, _vp, _vp],
     "ssr_dot": [_i64, _vp, _vp, _vp, _vp],
     "ssr_node_norm_max": [_int, _i64, _vp, _vp],
