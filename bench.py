@@ -75,6 +75,95 @@ def cpu_baseline(d, cb, sp, budget_s=12.0, sample=1_320_000):
     }
 
 
+def run_constitutive2d(args):
+    """The 2D (plane-strain) return mapping with tangent, the kernel of the
+    2D configurations: S (4) and DS (3 x 3) per point from E (3) and four
+    material fields, MEX layout, on the microbenchmark's strain recipe
+    (N(0, 1e-3) per component, a seeded 30 % scaled x5, lambda ~ U[1, 2],
+    Davis B).  Timed at the microbenchmark's 13.2M points and at the point
+    counts of the 2D configurations (h = 0.5 m: 70k points; the largest
+    batch member, ~200k unknowns: 700k points), with HIP events on the
+    library stream."""
+    from oracle import oracle as O
+    from slope_stability_octave_amd import _lib
+    from slope_stability_octave_amd import constitutive_mex as cm
+    from slope_stability_octave_amd import workloads as W
+    from slope_stability_octave_amd._lib import DeviceArray, EventTimer
+
+    _lib.require_gpu()
+    n = args.points
+    d = W.config2_points(n, dim=2)
+    dE = DeviceArray.from_host(d["E"])
+    dc0, dphi, dpsi, dlam = (DeviceArray.from_host(d[k]) for k in ("c0", "phi", "psi", "lam"))
+    dG, dK, dL = (DeviceArray.from_host(d[k]) for k in ("shear", "bulk", "lame"))
+    dcb, dsp = DeviceArray(n), DeviceArray(n)
+    dS, dD = DeviceArray((n, 4)), DeviceArray((n, 9))
+    cm.reduction_dev(n, "B", dc0, dphi, dpsi, dlam, True, dcb, dsp)
+    tim = EventTimer()
+    sizes = {}
+    for m in sorted({n, min(n, 700_000), min(n, 70_000)}, reverse=True):
+        row = {}
+        for with_ds in (True, False):
+            f = (lambda: cm.constitutive_dev(2, m, dE, dcb, dsp, dG, dK, dL, dS, dD)) if with_ds else \
+                (lambda: cm.constitutive_dev(2, m, dE, dcb, dsp, dG, dK, dL, dS, None))
+            for _ in range(args.warmup):
+                f()
+            _lib.synchronize()
+            steps = args.steps if m == n else 10 * args.steps
+            tim.start()
+            for _ in range(steps):
+                f()
+            tim.stop()
+            ms = tim.elapsed_ms() / steps
+            nbytes = (24 + 32 + 32 + (72 if with_ds else 0)) * m
+            row["S_DS" if with_ds else "S"] = {"ms": ms, "gpts": m / ms / 1e6, "GBs": nbytes / ms / 1e6,
+                                               "frac_hbm": nbytes / ms / 1e6 / 8000.0}
+        sizes[str(m)] = row
+    big = sizes[str(n)]["S_DS"]
+    # parity spot check against the oracle (not timed)
+    k = min(n, 200_000)
+    S0, D0, br0 = O.mc2d(np.ascontiguousarray(d["E"][:k].T), dcb.get()[:k], dsp.get()[:k], d["shear"][:k],
+                         d["bulk"][:k], d["lame"][:k])
+    cm.constitutive_dev(2, n, dE, dcb, dsp, dG, dK, dL, dS, dD)
+    S, DS = dS.get()[:k].T, dD.get()[:k].T
+    den = np.maximum(np.abs(D0).max(axis=0), 1e-300)
+    parity = {"points_checked": k,
+              "max_rel_S": float((np.abs(S - S0).max(axis=0) / np.maximum(np.abs(S0).max(axis=0), 1e-300)).max()),
+              "max_rel_DS": float((np.abs(DS - D0).max(axis=0) / den).max()),
+              "branch_histogram": np.bincount(br0, minlength=5).tolist()}
+    cpu = None
+    if not args.no_cpu_baseline:
+        O.build()
+        kind = "reference" if O.ref_available() else "port"
+        ns = min(n, 4_000_000)
+        E = np.ascontiguousarray(d["E"][:ns].T)
+        cbh, sph = dcb.get()[:ns], dsp.get()[:ns]
+        mats = (cbh, sph, d["shear"][:ns], d["bulk"][:ns], d["lame"][:ns])
+        O.mc2d(E[:, :1000], *[x[:1000] for x in mats], use_ref=(kind == "reference"))
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            O.mc2d(E, *mats, use_ref=(kind == "reference"))
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= 10.0 or reps >= 2000:
+                break
+        cpu = {"value": reps * ns / el / 1e9, "unit": "Gpt/s", "cores": O.num_threads(), "kind": kind,
+               "sample": f"{reps} x {ns} points, S+DS, OpenMP static schedule, {el:.1f} s"}
+    out = {
+        "metric": "2D constitutive S+DS throughput (plane strain, MEX layout)",
+        "value": big["gpts"], "unit": "Gpt/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": big["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D return mapping with tangent, {n} points"},
+        "roofline": {"bound": "hbm", "achieved": big["GBs"], "peak": 8000.0, "unit": "GB/s",
+                     "frac": big["frac_hbm"], "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": {"sizes": sizes, "parity": parity,
+                  "bytes_per_point": {"S_DS": 160, "S": 88}},
+    }
+    print(json.dumps(out))
+
+
 CONFIG3_CPM = 1.3  # cells per metre: 1,492,582 free unknowns (the ~1.5M-DOF configuration)
 GEOM_3D = (20.0, 20.0, 20.0, 10.0, 20.0, 20.0)
 
@@ -606,7 +695,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-time-to-fos", action="store_true",
                     help="skip the time-to-FoS leg (plain vs chosen form on the ~1.5M-DOF 3D slope)")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "assembly", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "constitutive2d", "assembly", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
                              "learned3d", "batch", "ll2d"])
     ap.add_argument("--element-kernel", type=int, default=None, choices=[0, 1, 2],
                     help="element tangent kernel form (tuning key 8)")
@@ -686,6 +775,8 @@ def main():
         L.set_tuning(L.TUNE_ELEMENT_KERNEL, args.element_kernel)
     if args.workload == "assembly":
         return run_assembly(args)
+    if args.workload == "constitutive2d":
+        return run_constitutive2d(args)
 
     from slope_stability_octave_amd import _lib
     from slope_stability_octave_amd import constitutive_mex as cm

@@ -151,7 +151,12 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         the block diagonal, the others as transposed mirrors (1) or every
  *         block (0, default): the same bits for an operator whose mirrored
  *         blocks are transposes bit for bit, as K_r is (slower on the
- *         structured slope meshes, whose mirrors lie far apart). */
+ *         structured slope meshes, whose mirrors lie far apart);
+ *  key 15: 2D constitutive kernel form with DS (1 = one wavefront per block,
+ *         strains read and S, DS written through LDS as contiguous 16-byte
+ *         spans, default when E, S and DS are 16-byte aligned; 0 = one lane
+ *         per point with direct accesses, also the form for S alone): the
+ *         same bits. */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 

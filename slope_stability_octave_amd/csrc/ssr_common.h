@@ -93,5 +93,9 @@ struct ssr_precond {
 #define SSR_TUNE_SPMV_MIRROR 14         /* 1: node-block SpMV of a symmetric operator reads the blocks on and
                                            above the diagonal only, the others as transposed mirrors; the
                                            same bits; 0 (default): every block */
-#define SSR_TUNE_COUNT 15
+#define SSR_TUNE_CONSTITUTIVE_2D 15     /* 2D return mapping with DS: 1 = one wavefront per block with the
+                                           strains read and S, DS written through LDS as contiguous spans
+                                           (default), 0 = one lane per point, direct accesses (the form for
+                                           S alone); the same bits */
+#define SSR_TUNE_COUNT 16
 int ssr_tuning(int key);

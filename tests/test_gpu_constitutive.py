@@ -129,3 +129,44 @@ def test_material_indexed_fields_equal_per_point(dim):
         out[mode] = (S.get(), DS.get())
     assert np.array_equal(out["point"][0], out["material"][0])
     assert np.array_equal(out["point"][1], out["material"][1])
+
+
+@pytest.mark.parametrize("n", [100003, 64, 1])
+def test_2d_staged_form_same_bits(n):
+    """Tuning key 15: the 2D kernel with strains, S and DS through LDS gives
+    the direct form's S, DS and branches bit for bit (partial last
+    wavefront; S only; per-material fields), and a misaligned E takes the
+    direct form with the same results."""
+    import ctypes
+    from slope_stability_octave_amd import _lib
+    d = W.config2_points(n, dim=2, seed_e=7 + n, seed_mask=8 + n)
+    cb, sp = O.reduction(d["c0"], d["phi"], d["psi"], d["lam"], "B")
+    mats = [DeviceArray.from_host(m) for m in (cb, sp, d["shear"], d["bulk"], d["lame"])]
+    dE = DeviceArray.from_host(d["E"])
+    # the same strains one double into a buffer: E misaligned by 8 bytes
+    Eoff = DeviceArray.from_host(np.concatenate([[0.0], d["E"].ravel()]))
+    n_q = 7
+    n_e = (n + n_q - 1) // n_q
+    mid = DeviceArray.from_host((np.arange(n_e) % 3).astype(np.int32))
+    tab = [DeviceArray.from_host(np.asarray(m)[:3].copy()) for m in (cb, sp, d["shear"], d["bulk"], d["lame"])]
+    out = {}
+    old = _lib.get_tuning(_lib.TUNE_CONSTITUTIVE_2D)
+    try:
+        for mode in (0, 1):
+            _lib.set_tuning(_lib.TUNE_CONSTITUTIVE_2D, mode)
+            S, DS, B = DeviceArray((n, 4)), DeviceArray((n, 9)), DeviceArray(n, np.int8)
+            cm.constitutive_dev(2, n, dE, *mats, S, DS, B)
+            S1 = DeviceArray((n, 4))
+            cm.constitutive_dev(2, n, dE, *mats, S1, None)
+            S2, DS2 = DeviceArray((n, 4)), DeviceArray((n, 9))
+            _lib.call("ssr_dev_constitutive_2D", n, ctypes.c_void_p(Eoff.ptr + 8), *[_lib.ptr(m) for m in mats],
+                      _lib.ptr(S2), _lib.ptr(DS2), _lib.ptr(None))
+            S3, DS3 = DeviceArray((n, 4)), DeviceArray((n, 9))
+            cm.constitutive_mat_dev(2, n, n_q, dE, mid, 3, *tab, S3, DS3)
+            out[mode] = [a.get() for a in (S, DS, B, S1, S2, DS2, S3, DS3)]
+    finally:
+        _lib.set_tuning(_lib.TUNE_CONSTITUTIVE_2D, old)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[1][0], out[1][3]) and np.array_equal(out[1][0], out[1][4])
+    assert np.array_equal(out[1][1], out[1][5])
