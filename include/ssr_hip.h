@@ -156,6 +156,12 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         strains read and S, DS written through LDS as contiguous 16-byte
  *         spans, default when E, S and DS are 16-byte aligned; 0 = one lane
  *         per point with direct accesses, also the form for S alone): the
+ *         same bits;
+ *  key 16: the native Newton loops (ssr_newton_ind_SSR, ssr_newton,
+ *         ssr_newton_ind_LL) replay their element phases -- strain, stress
+ *         and tangent, internal force of each build_F, and the K_r
+ *         assembly -- from HIP graphs captured on their second use within
+ *         one Newton solve (1, default), or launch them eagerly (0): the
  *         same bits. */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);

@@ -1,6 +1,7 @@
 // C ABI of the SSR hot-path library: runtime helpers (device memory, copies,
 // stream, events) and the host-pointer entry points that keep the signatures
 // of the reference's MEX files.  Declarations: include/ssr_hip.h.
+#include <atomic>
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdint.h>
@@ -41,6 +42,11 @@ void ssr_capture_stream_push(hipStream_t s)
 }
 
 void ssr_capture_stream_pop(void) { t_capturing = false; }
+
+// true once any host thread has run the library on a stream of its own
+// (concurrent instances): graph capture is then left alone
+static std::atomic<bool> g_thread_streams{false};
+bool ssr_thread_streams_used(void) { return g_thread_streams.load(); }
 
 int ssr_cu_count(void)
 {
@@ -85,6 +91,7 @@ int ssr_set_thread_stream(void *stream)
 {
     t_stream = (hipStream_t)stream;
     t_has_stream = stream != nullptr;
+    if (stream) g_thread_streams.store(true);
     return SSR_OK;
 }
 
@@ -309,7 +316,7 @@ extern "C" int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, 
     return ssr_dtoh(V, dV.p, sizeof(double) * nnz_out);
 }
 
-static int g_tuning[SSR_TUNE_COUNT] = {11, 0, 0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 1, 1, 0, 1};
+static int g_tuning[SSR_TUNE_COUNT] = {11, 0, 0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 1, 1, 0, 1, 1};
 
 int ssr_tuning(int key) { return (key >= 0 && key < SSR_TUNE_COUNT) ? g_tuning[key] : 0; }
 

@@ -11,7 +11,9 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
+#include <vector>
 
 #include "ssr_common.h"
 #include "../../include/ssr_hip.h"
@@ -24,13 +26,102 @@ namespace {
         if (_rc != SSR_OK) return _rc; \
     } while (0)
 
+// The element phases of a Newton solve replayed as captured HIP graphs
+// (tuning key 16): the strain, stress (and tangent) and internal-force
+// kernels of one build_F for a given (U, F, tangent or not), and the K_r
+// assembly for a given r.  A phase runs eagerly the first time (so that any
+// scratch a kernel allocates on first use exists), is captured the second
+// time and replayed from then on.  The graphs belong to one Newton solve:
+// its buffers do not change while it runs, and the graphs are destroyed, the
+// stream drained first, when it returns.  The Davis reduction (lambda as a
+// kernel argument) stays an ordinary launch before each replay.
+//
+// Capture happens only while this is the one Newton solve running in the
+// process and no host thread runs the library on a stream of its own
+// (concurrent instances replay nothing): another thread's synchronising
+// calls (a hipFree, a blocking copy) can invalidate a capture in progress.  A capture that fails anyway
+// is dropped and the phase runs eagerly from then on (nothing captured has
+// executed, so the eager run gives the same results).
+std::atomic<int> g_active_solves{0};
+
+struct PhaseGraph {
+    int kind;        // 0: build_F body, 1: K_r assembly
+    const void *a;   // U (kind 0)
+    const void *b;   // F (kind 0)
+    int flag;        // with the tangent (kind 0)
+    double r;        // regularisation (kind 1)
+    int uses;
+    bool nocap;      // capture failed: eager from now on
+    hipGraphExec_t exec;
+};
+
+struct GraphCache {
+    std::vector<PhaseGraph> g;
+    hipStream_t cap = nullptr;  // capture stream (default priority)
+    GraphCache() { g_active_solves.fetch_add(1); }
+    GraphCache(const GraphCache &) = delete;
+    GraphCache &operator=(const GraphCache &) = delete;
+    ~GraphCache()
+    {
+        bool any = false;
+        for (auto &e : g) any = any || e.exec;
+        if (any) (void)hipStreamSynchronize(ssr_stream());
+        for (auto &e : g)
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        if (cap) (void)hipStreamDestroy(cap);
+        g_active_solves.fetch_sub(1);
+    }
+};
+
 struct NewtonCtx {
     const ssr_constitutive *C;
     ssr_solver *solver;
     int method;
     int64_t n_dof;
     double *F, *F_eps, *G, *rhs, *V, *W, *dU, *Ua, *Fa, *tmp;
+    GraphCache *graphs;
 };
+
+// Run body() (kernel launches on ssr_stream() only), or replay it from the
+// graph captured for this key.  The kernel-only element forms are the
+// default ones (the FP64-atomic forms clear their outputs with a memset,
+// which a captured graph must not hold, so they always run eagerly).
+template <class Body>
+int replay(const NewtonCtx &x, int kind, const void *a, const void *b, int flag, double r, Body body)
+{
+    GraphCache *gc = x.graphs;
+    if (!gc || !ssr_tuning(SSR_TUNE_NEWTON_GRAPH) || ssr_tuning(SSR_TUNE_ASSEMBLY_FORM) != 0) return body();
+    PhaseGraph *e = nullptr;
+    for (auto &q : gc->g)
+        if (q.kind == kind && q.a == a && q.b == b && q.flag == flag && q.r == r) e = &q;
+    if (!e) {
+        gc->g.push_back(PhaseGraph{kind, a, b, flag, r, 0, false, nullptr});
+        e = &gc->g.back();
+    }
+    if (!e->exec) {
+        if (e->nocap || e->uses++ == 0 || g_active_solves.load() > 1 || ssr_thread_streams_used()) return body();
+        if (!gc->cap) SSR_HIP_CHECK(hipStreamCreateWithFlags(&gc->cap, hipStreamNonBlocking));
+        SSR_HIP_CHECK(hipStreamBeginCapture(gc->cap, hipStreamCaptureModeThreadLocal));
+        ssr_capture_stream_push(gc->cap);
+        const int rc = body();
+        ssr_capture_stream_pop();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(gc->cap, &g);
+        hipGraphExec_t ge = nullptr;
+        hipError_t ei = hipErrorUnknown;
+        if (!rc && ec == hipSuccess && g) ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            // nothing of the capture ran: clear its error and run the phase
+            (void)hipGetLastError();
+            e->nocap = true;
+            return body();
+        }
+        e->exec = ge;
+    }
+    SSR_HIP_CHECK(hipGraphLaunch(e->exec, ssr_stream()));
+    return SSR_OK;
+}
 
 // host seconds spent in a phase (the calls that end in a synchronisation
 // absorb the device work queued before them)
@@ -73,10 +164,9 @@ int reduction(const NewtonCtx &x, double lam)
 // build_F_all / build_F_and_DS_all (CONSTITUTIVE.m:202-285): reduction, E = B U,
 // S (and DS), F = B'(w S); the *_reduced forms (reduce = false) keep the
 // material the caller reduced
-int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with_ds, bool reduce = true)
+int build_F_body(const NewtonCtx &x, const double *U, double *F, int with_ds)
 {
     const ssr_constitutive *C = x.C;
-    if (reduce) NCHECK(reduction(x, lam));
     if (C->coord)
         NCHECK(ssr_dev_strain_geo(C->pattern, C->n_q, C->coord, U, C->E));
     else
@@ -93,6 +183,22 @@ int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with
                                        with_ds ? C->DS : nullptr, nullptr));
     if (C->coord) return ssr_dev_force_geo(C->pattern, C->n_q, C->coord, C->S, C->s_rows, C->WEIGHT, F);
     return ssr_dev_force(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->S, C->s_rows, C->WEIGHT, F);
+}
+
+int build_F(const NewtonCtx &x, double lam, const double *U, double *F, int with_ds, bool reduce = true)
+{
+    if (reduce) NCHECK(reduction(x, lam));
+    return replay(x, 0, U, F, with_ds, 0.0, [&] { return build_F_body(x, U, F, with_ds); });
+}
+
+// K_r (CONSTITUTIVE.m:487-532) from the tangent of the last build_F with DS
+int build_K_r(const NewtonCtx &x, double r)
+{
+    const ssr_constitutive *C = x.C;
+    return replay(x, 1, nullptr, nullptr, 0, r, [&] {
+        return ssr_dev_assemble_K_r(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT, r,
+                                    C->K_elast, C->K_r);
+    });
 }
 
 // ||a(Q) - b(Q)||
@@ -205,6 +311,7 @@ int prepare(const ssr_constitutive *C, ssr_solver *solver, int method, int it_ne
     x->solver = solver;
     x->method = method;
     x->n_dof = n_dof;
+    x->graphs = nullptr;
     double **slots[10] = {&x->F, &x->F_eps, &x->G, &x->rhs, &x->V, &x->W, &x->dU, &x->Ua, &x->Fa, &x->tmp};
     for (int k = 0; k < 10; ++k) *slots[k] = work + k * n_dof;
     return SSR_OK;
@@ -230,6 +337,8 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
                 "newton_ind_SSR: null argument");
     NewtonCtx x;
     NCHECK(prepare(C, solver, method, it_newt_max, it_damp_max, work, work_len, &x));
+    GraphCache graphs;
+    x.graphs = &graphs;
     const int64_t n_dof = x.n_dof;
 
     // newton_ind_SSR.m:22-33
@@ -268,8 +377,7 @@ extern "C" int ssr_newton_ind_SSR(const ssr_constitutive *C, ssr_solver *solver,
         }
         if (h) h[1] = r;
         // K_r (CONSTITUTIVE.m:487-532)
-        NCHECK(ssr_dev_assemble_K_r(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT, r,
-                                    C->K_elast, C->K_r));
+        NCHECK(build_K_r(x, r));
         // G = (F(lambda + eps) - F) / eps   (:94-96)
         NCHECK(build_F(x, lambda_it + eps, U_it, x.F_eps, 0));
         NCHECK(ssr_dev_lincomb(n_dof, 1.0 / eps, x.F_eps, -1.0 / eps, x.F, x.G));
@@ -356,6 +464,8 @@ extern "C" int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int met
     SSR_REQUIRE(C && solver && U_ini && work && U_it && flag_out && it_out && status_out, "newton: null argument");
     NewtonCtx x;
     NCHECK(prepare(C, solver, method, it_newt_max, it_damp_max, work, work_len, &x));
+    GraphCache graphs;
+    x.graphs = &graphs;
     const int64_t n_dof = x.n_dof;
 
     // newton.m:36-44
@@ -391,8 +501,7 @@ extern "C" int ssr_newton(const ssr_constitutive *C, ssr_solver *solver, int met
             h[0] = rel_resid;
         }
         if (h) h[1] = r;
-        NCHECK(ssr_dev_assemble_K_r(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT, r,
-                                    C->K_elast, C->K_r));
+        NCHECK(build_K_r(x, r));
         tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
         tm[2] += lap.take();
@@ -458,6 +567,8 @@ extern "C" int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, 
                 "newton_ind_LL: null argument");
     NewtonCtx x;
     NCHECK(prepare(C, solver, method, it_newt_max, it_damp_max, work, work_len, &x));
+    GraphCache graphs;
+    x.graphs = &graphs;
     const int64_t n_dof = x.n_dof;
     double *zero = x.G;  // the damping's target: F(U) against f = 0
     NCHECK(ssr_dev_fill(n_dof, 0.0, zero));
@@ -495,8 +606,7 @@ extern "C" int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, 
             break;
         }
         if (h) h[1] = r;
-        NCHECK(ssr_dev_assemble_K_r(C->pattern, C->n_q, C->DPhi1, C->DPhi2, C->DPhi3, C->DS, 0, C->WEIGHT, r,
-                                    C->K_elast, C->K_r));
+        NCHECK(build_K_r(x, r));
         tm[3] += lap.take();
         NCHECK(ssr_solver_set_matrix(solver, C->K_r, 1));
         tm[2] += lap.take();

@@ -48,6 +48,8 @@ hipStream_t ssr_stream(void);
 // divert ssr_stream() of the calling thread to `s` while capturing a graph
 void ssr_capture_stream_push(hipStream_t s);
 void ssr_capture_stream_pop(void);
+// true once any host thread has set a stream of its own (ssr_set_thread_stream)
+bool ssr_thread_streams_used(void);
 int ssr_cu_count(void);  // compute units of the current device (cached)
 
 // Preconditioner interface used by the Krylov solvers (solver.cpp): block
@@ -97,5 +99,8 @@ struct ssr_precond {
                                            strains read and S, DS written through LDS as contiguous spans
                                            (default), 0 = one lane per point, direct accesses (the form for
                                            S alone); the same bits */
-#define SSR_TUNE_COUNT 16
+#define SSR_TUNE_NEWTON_GRAPH 16        /* 1: the Newton loops replay their element phases (strain, stress and
+                                           tangent, internal force; K_r assembly) from captured HIP graphs
+                                           (default), 0: eager launches; the same bits */
+#define SSR_TUNE_COUNT 17
 int ssr_tuning(int key);
