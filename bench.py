@@ -72,7 +72,19 @@ def cpu_baseline(d, cb, sp, budget_s=12.0, sample=1_320_000):
         "kind": kind,
         "sample": f"{reps} x {n} points (first {n} of the {d['E'].shape[0]}-point workload), "
                   f"S+DS, OpenMP static schedule, {el:.1f} s",
+        "host": _host_cpus(),
     }
+
+
+def _host_cpus():
+    """The CPUs this process may run on and the machine's count, beside the
+    OpenMP thread count the CPU baseline used."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = None
+    return {"cpus_available_to_process": avail, "cpus_on_machine": os.cpu_count(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
 
 
 def run_constitutive2d(args):
