@@ -717,8 +717,9 @@ def main():
     ap.add_argument("--omega-max", type=float, default=7e7)
     ap.add_argument("--precond", default="amg", choices=["amg", "block_jacobi"])
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--newton-driver", default="native", choices=["python", "native"],
-                    help="FoS workloads: Newton loop inside the library (default) or in Python over the C ABI")
+    ap.add_argument("--newton-driver", default="native", choices=["python", "native", "library"],
+                    help="FoS workloads: Newton loop inside the library (default), in Python over the C ABI, "
+                         "or the whole continuation inside the library (library)")
     ap.add_argument("--variant", type=int, default=None, help="3D constitutive kernel form (tuning key 0; 11 is the default)")
     ap.add_argument("--linear-tol", type=float, default=None)
     ap.add_argument("--amg-rebuild-every", type=int, default=None,

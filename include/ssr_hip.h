@@ -349,6 +349,11 @@ int ssr_solver_deflation_size(const ssr_solver *s, int *k);
 int ssr_solver_clear_deflation(ssr_solver *s);
 int ssr_solver_expand_deflation(ssr_solver *s, const double *v);       /* DFGMRES.m:163-170 */
 int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src); /* DFGMRES.m:204-221 */
+/* DFGMRES.copy (DFGMRES.m:204-221): a new solver with src's settings
+ * (tolerances, precision, symmetry) and a copy of its deflation basis; with
+ * share_precond the copy applies src's preconditioner object (one AMG
+ * hierarchy for all copies), else it has a node-block Jacobi of its own. */
+int ssr_solver_clone(const ssr_solver *src, int share_precond, ssr_solver **out);
 /* dst uses src's preconditioner object (shared; freed with the last user).
  * No reference counterpart: DFGMRES.m:204-221 gives a copy its own
  * preconditioner, set up again on first use.  Sharing keeps one AMG hierarchy
@@ -481,6 +486,32 @@ int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, int method,
                       double t_ini, double omega, int it_newt_max, int it_damp_max, double tol, double r_min,
                       double *work, int64_t work_len, double *U_it, double *t_out, int *flag_N, int *it,
                       int *status, double *hist, double *times);
+
+/* [U, lambda_hist, omega_hist, Umax_hist] = SSR_indirect_continuation(lambda_init,
+ *     d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max, omega_max_stop,
+ *     it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, constitutive_matrix_builder,
+ *     linear_system_solver)
+ * replaces slope_stability/+CONTINUATION/SSR_indirect_continuation.m:1-237 with
+ * init_phase_SSR_indirect_continuation.m:1-143 (secant predictor, :119): the whole
+ * continuation in the library, over ssr_newton (initial phase) and
+ * ssr_newton_ind_SSR, each Newton solve on a copy of `solver`
+ * (ssr_solver_clone with share_precond) whose basis the accepted fields expand.
+ * params: lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min,
+ * step_max, omega_max_stop, it_newt_max, it_damp_max, tol, r_min.  U_out (device,
+ * dim*n_n) receives the last accepted field; the histories (host, hist_cap each)
+ * the accepted (lambda, omega, max nodal |U|) from the two initial points on;
+ * attempt_its / init_its (host) the Newton iterations of every attempt of the
+ * main loop and of the initial phase; krylov_its their Krylov iterations.  stop:
+ * 1 minimal increase of lambda, 2 maximal omega, 3 omega cannot increase,
+ * 4 maximal number of steps; on the reference's errors (:initial choice of lambda
+ * too large, FoS equal to lambda_init, FoS above 10, omega_max_stop too small)
+ * stop is -1..-4, the call returns SSR_ERR_STATE and ssr_last_error holds the
+ * reference's message. */
+int ssr_SSR_indirect_continuation(const ssr_constitutive *C, ssr_solver *solver, int method, int share_precond,
+                                  const double *params, double *U_out, double *lambda_hist, double *omega_hist,
+                                  double *Umax_hist, int hist_cap, int *n_hist, int *attempt_its, int attempt_cap,
+                                  int *n_attempts, int *init_its, int init_cap, int *n_init, int64_t *krylov_its,
+                                  int *stop);
 
 /* ---- learned continuation predictor (small MLP) --------------------------- */
 /* No reference counterpart: the reference continuation has the secant

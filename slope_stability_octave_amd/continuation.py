@@ -92,12 +92,88 @@ def init_phase_SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_mi
     return U1, U2, omega1, omega2, lambda1, lambda2, its
 
 
+_STOP_REASONS = {1: "minimal increase of lambda achieved", 2: "maximal omega achieved",
+                 3: "impossible to increase omega", 4: "maximal number of steps achieved"}
+
+
+def _shares_hierarchy(solver):
+    """Whether solver.copy() shares the solver's preconditioner object (AMG
+    with reuse_hierarchy) or builds a node-block Jacobi of its own; None when
+    the copy would build a new AMG (not done by the library's continuation)."""
+    from .linear_solvers import _amg_option
+    name = getattr(solver, "preconditioner_name", None)
+    if name == "amg":
+        return True if _amg_option(solver.amg_options, "reuse_hierarchy") else None
+    return False if name == "block_jacobi" else None
+
+
+def _SSR_indirect_continuation_native(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
+                                      omega_max_stop, it_newt_max, it_damp_max, tol, r_min, Q, f, cmb,
+                                      linear_system_solver, log=None):
+    """The whole continuation as one ssr_SSR_indirect_continuation call (the
+    same steps as the host loop below, over the native Newton loops)."""
+    from . import _lib
+    from .newton import native_description
+    t_total = time.perf_counter()
+    if not cmb.pattern_initialized:
+        cmb.init_K_r_pattern(Q)
+    share = _shares_hierarchy(linear_system_solver)
+    if share is None:
+        raise ValueError("the library's continuation needs a solver whose copies share its AMG hierarchy "
+                         "(reuse_hierarchy) or use node-block Jacobi")
+    desc = native_description(cmb, f)
+    params = np.array([lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max, omega_max_stop,
+                       it_newt_max, it_damp_max, tol, r_min], dtype=np.float64)
+    cap = int(step_max) + 8
+    lam_h, om_h, um_h = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+    acap, icap = 6 * cap, 4096
+    att, ini = np.zeros(acap, dtype=np.int32), np.zeros(icap, dtype=np.int32)
+    nh, na, ni, stop = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    kry = ctypes.c_int64(0)
+    U = DeviceArray(f.shape)
+    method = 1 if getattr(linear_system_solver, "method", "dfgmres") == "dcg" else 0
+    lib = _lib.load()
+    rc = lib.ssr_SSR_indirect_continuation(
+        ctypes.byref(desc), linear_system_solver.handle, method, 1 if share else 0, ptr(params), ptr(U), ptr(lam_h),
+        ptr(om_h), ptr(um_h), cap, ctypes.byref(nh), ptr(att), acap, ctypes.byref(na), ptr(ini), icap,
+        ctypes.byref(ni), ctypes.byref(kry), ctypes.byref(stop))
+    if rc != 0:
+        msg = lib.ssr_last_error().decode(errors="replace")
+        if stop.value < 0:
+            raise ContinuationError(msg)
+        raise _lib.SSRError(f"ssr_SSR_indirect_continuation: {msg} (status {rc})")
+    linear_system_solver._K = cmb.K_r
+    n = nh.value
+    if log:
+        for k in range(2, n):
+            log(f"  accepted: lambda={lam_h[k]:.6g}, omega={om_h[k]:.6g}, U_max={um_h[k]:.6g}")
+        log(f"Stop: {_STOP_REASONS.get(stop.value, 'unknown')}.")
+    c = linear_system_solver.counters
+    if c is not None:
+        c["DFGMRES.solve.n_iters"] = c.get("DFGMRES.solve.n_iters", 0) + int(kry.value)
+    stats = dict(attempt_newton_iterations=[int(v) for v in att[:na.value]],
+                 init_newton_iterations=[int(v) for v in ini[:ni.value]],
+                 step_lambda=[float(v) for v in lam_h[2:n]], step_omega=[float(v) for v in om_h[2:n]],
+                 predictor_used=[False] * max(0, na.value), stop_reason=_STOP_REASONS.get(stop.value, "unknown"),
+                 krylov_iterations=int(kry.value), total_wall_time=time.perf_counter() - t_total)
+    return U, lam_h[:n].copy(), om_h[:n].copy(), um_h[:n].copy(), stats
+
+
 def SSR_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
                               omega_max_stop, it_newt_max, it_damp_max, tol, r_min, K_elast, Q, f, cmb,
                               linear_system_solver, log=None, predictor=None, recorder=None, driver="python"):
     """Returns (U, lambda_hist, omega_hist, Umax_hist, stats).  ``driver``
     picks the Newton loops (newton_ind_SSR and the initial phase's newton):
-    "python" (newton.py) or "native" (the same loops in libssr_hip.so)."""
+    "python" (newton.py) or "native" (the same loops in libssr_hip.so);
+    "library" runs the whole continuation in libssr_hip.so
+    (ssr_SSR_indirect_continuation: the secant predictor, no predictor or
+    recorder hooks) and gives the "native" driver's numbers."""
+    if driver == "library":
+        if predictor is not None or recorder is not None:
+            raise ValueError("the library's continuation has no predictor or recorder hooks")
+        return _SSR_indirect_continuation_native(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min,
+                                                 step_max, omega_max_stop, it_newt_max, it_damp_max, tol, r_min, Q,
+                                                 f, cmb, linear_system_solver, log)
     t_total = time.perf_counter()
     ws = Workspace(cmb.n_n, cmb.dim)
     mask_ready = cmb.pattern_initialized or cmb.init_K_r_pattern(Q) or True  # noqa: F841

@@ -11,6 +11,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <vector>
@@ -679,4 +680,215 @@ extern "C" int ssr_newton_ind_LL(const ssr_constitutive *C, ssr_solver *solver, 
     *it_out = it;
     *status_out = status;
     return SSR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The indirect SSR continuation in the library: SSR_indirect_continuation.m
+// (omega stepping, secant predictor :119, retries with half the omega step
+// :164-167, omega step doubling :214-216, stop rules :203-211 and :220-227)
+// with its initial phase init_phase_SSR_indirect_continuation.m, over the
+// Newton loops above.  Every field stays on the device; the host holds the
+// histories and the step control.  The order of operations (reductions,
+// Newton solves on solver copies, basis expansions, dot products) is that of
+// the host driver continuation.py, so the two give the same numbers.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DevVec {
+    double *p = nullptr;
+    ~DevVec() { ssr_pool_free(p); }
+    int alloc(int64_t n) { return ssr_pool_malloc((void **)&p, sizeof(double) * (n > 0 ? n : 1)); }
+};
+
+struct SolverCopy {
+    ssr_solver *s = nullptr;
+    ~SolverCopy() { (void)ssr_solver_destroy(s); }
+};
+
+// Newton iterations and the Krylov iterations of their solves, from the
+// per-iteration records (columns `kcols` hold the Krylov counts)
+void add_krylov(const std::vector<double> &hist, int width, int its, int kc0, int kc1, int64_t *krylov)
+{
+    for (int i = 0; i < its; ++i)
+        for (int c = kc0; c <= kc1; ++c) {
+            const double v = hist[(size_t)width * i + c];
+            if (v == v) *krylov += (int64_t)v;
+        }
+}
+
+}  // namespace
+
+extern "C" int ssr_SSR_indirect_continuation(const ssr_constitutive *C, ssr_solver *solver, int method,
+                                             int share_precond, const double *params, double *U_out,
+                                             double *lambda_hist, double *omega_hist, double *Umax_hist,
+                                             int hist_cap, int *n_hist, int *attempt_its, int attempt_cap,
+                                             int *n_attempts, int *init_its, int init_cap, int *n_init,
+                                             int64_t *krylov_its, int *stop)
+{
+    SSR_REQUIRE(C && solver && params && U_out && lambda_hist && omega_hist && Umax_hist && n_hist && attempt_its &&
+                    n_attempts && init_its && n_init && krylov_its && stop && hist_cap >= 2 && attempt_cap >= 1 &&
+                    init_cap >= 2,
+                "SSR_indirect_continuation: bad arguments");
+    const double lambda_init = params[0], d_lambda_init = params[1], d_lambda_min = params[2],
+                 d_lambda_diff_scaled_min = params[3], omega_max_stop = params[5], tol = params[8], r_min = params[9];
+    const int step_max = (int)params[4], it_newt_max = (int)params[6], it_damp_max = (int)params[7];
+    SSR_REQUIRE(it_newt_max > 0 && it_damp_max > 0 && step_max > 0, "SSR_indirect_continuation: bad parameters");
+    int64_t info[8];
+    NCHECK(ssr_pattern_info(C->pattern, info));
+    const int64_t n_dof = info[0] * info[1], n_n = info[1];
+    const int dim = (int)info[0];
+    int64_t wl = 0;
+    NCHECK(ssr_newton_work_len(C, &wl));
+    DevVec work, Ua, Ub, Uc, Uini;
+    NCHECK(work.alloc(wl));
+    NCHECK(Ua.alloc(n_dof));
+    NCHECK(Ub.alloc(n_dof));
+    NCHECK(Uc.alloc(n_dof));
+    NCHECK(Uini.alloc(n_dof));
+    std::vector<double> hist((size_t)6 * it_newt_max);
+    double tm[4] = {0, 0, 0, 0};
+    NewtonCtx x{};
+    x.C = C;
+    *n_hist = *n_attempts = *n_init = 0;
+    *krylov_its = 0;
+    *stop = 0;
+
+    // init_phase_SSR_indirect_continuation.m, on one copy of the solver
+    double *U1 = Ua.p, *U2 = Ub.p, *Ut = Uc.p;
+    double omega1 = 0.0, omega2 = 0.0, lambda1 = lambda_init, lambda2 = 0.0;
+    {
+        SolverCopy init;
+        NCHECK(ssr_solver_clone(solver, share_precond, &init.s));
+        NCHECK(ssr_dev_fill(n_dof, 0.0, Uini.p));
+        NCHECK(reduction(x, lambda_init));
+        int flag = 0, it = 0, st = 0;
+        std::fill(hist.begin(), hist.end(), NAN);
+        NCHECK(ssr_newton(C, init.s, method, Uini.p, it_newt_max, it_damp_max, tol, r_min, work.p, wl, U1, &flag, &it,
+                          &st, hist.data(), tm));
+        add_krylov(hist, 5, it, 3, 3, krylov_its);
+        init_its[(*n_init)++] = it;
+        if (flag == 1) {
+            *stop = -1;
+            ssr_set_error("Initial choice of lambda seems to be too large.");
+            return SSR_ERR_STATE;
+        }
+        NCHECK(ssr_dot(n_dof, C->f, U1, C->mask, &omega1));
+        double d_lambda = d_lambda_init;
+        NCHECK(ssr_solver_expand_deflation(init.s, U1));
+        while (true) {
+            const double lambda_it = lambda1 + d_lambda;
+            NCHECK(reduction(x, lambda_it));
+            std::fill(hist.begin(), hist.end(), NAN);
+            NCHECK(ssr_newton(C, init.s, method, U1, it_newt_max, it_damp_max, tol, r_min, work.p, wl, Ut, &flag, &it,
+                              &st, hist.data(), tm));
+            add_krylov(hist, 5, it, 3, 3, krylov_its);
+            if (*n_init < init_cap) init_its[(*n_init)++] = it;
+            if (flag == 1) {
+                d_lambda = d_lambda / 2;
+            } else {
+                std::swap(U2, Ut);
+                NCHECK(ssr_dot(n_dof, C->f, U2, C->mask, &omega2));
+                lambda2 = lambda_it;
+                if ((omega2 - omega1) / fmax(1.0, omega1) < 1e-5) {
+                    std::swap(U1, U2);
+                    lambda1 = lambda2;
+                    omega1 = omega2;
+                } else {
+                    break;
+                }
+            }
+            if (d_lambda < d_lambda_min) {
+                *stop = -2;
+                ssr_set_error("It seems that the FoS is equal to lambda_init.");
+                return SSR_ERR_STATE;
+            }
+            if (lambda1 > 10) {
+                *stop = -3;
+                ssr_set_error("It seems that the FoS is greater than 10.");
+                return SSR_ERR_STATE;
+            }
+        }
+    }
+    // SSR_indirect_continuation.m:86-110
+    double *U_old = U1, *U = U2;  // Ut is free
+    NCHECK(ssr_solver_expand_deflation(solver, U_old));
+    int nh = 0;
+    auto push_hist = [&](double lam, double om, const double *Uf) -> int {
+        SSR_REQUIRE(nh < hist_cap, "SSR_indirect_continuation: history capacity %d exceeded", hist_cap);
+        double um = 0.0;
+        NCHECK(ssr_node_norm_max(dim, n_n, Uf, &um));
+        lambda_hist[nh] = lam;
+        omega_hist[nh] = om;
+        Umax_hist[nh] = um;
+        ++nh;
+        *n_hist = nh;
+        return SSR_OK;
+    };
+    NCHECK(push_hist(lambda1, omega1, U_old));
+    NCHECK(push_hist(lambda2, omega2, U));
+    double omega_old = omega1, omega = omega2, lam = lambda2;
+    double d_omega = omega - omega_old;
+    if (omega_max_stop < omega + d_omega) {
+        *stop = -4;
+        ssr_set_error("Too small value of omega_max_stop. It is necessary to increase this input quantinty.");
+        return SSR_ERR_STATE;
+    }
+    int step = 2, n_omega = 0;
+    const int n_omega_max = 5;
+    while (true) {
+        const double omega_it = fmin(omega + d_omega, omega_max_stop);
+        d_omega = omega_it - omega;
+        // secant predictor (:119): U_ini = a (U - U_old) + U
+        const double a = d_omega / (omega - omega_old);
+        NCHECK(ssr_dev_lincomb(n_dof, 1.0 + a, U, -a, U_old, Uini.p));
+        SolverCopy cp;
+        NCHECK(ssr_solver_clone(solver, share_precond, &cp.s));
+        double lambda_it = 0.0;
+        int flag = 0, it = 0, st = 0;
+        std::fill(hist.begin(), hist.end(), NAN);
+        NCHECK(ssr_newton_ind_SSR(C, cp.s, method, Uini.p, omega_it, lam, it_newt_max, it_damp_max, tol, r_min, work.p,
+                                  wl, Ut, &lambda_it, &flag, &it, &st, hist.data(), tm));
+        add_krylov(hist, 6, it, 3, 4, krylov_its);
+        if (*n_attempts < attempt_cap) attempt_its[(*n_attempts)++] = it;
+        if (flag == 1) {
+            // a failed solve asks for a fresh preconditioner before the retry
+            NCHECK(ssr_solver_invalidate_preconditioner(solver));
+            d_omega = d_omega / 2;
+            n_omega += 1;
+        } else {
+            step += 1;
+            double *freed = U_old;
+            U_old = U;
+            U = Ut;
+            Ut = freed;
+            NCHECK(ssr_solver_expand_deflation(solver, U));
+            omega_old = omega;
+            omega = omega_it;
+            const double d_lambda = lambda_it - lam;
+            lam = lambda_it;
+            n_omega = 0;
+            NCHECK(push_hist(lam, omega, U));
+            const double scaled = d_lambda / d_omega * (omega_hist[nh - 1] - omega_hist[0]);
+            if (scaled < d_lambda_diff_scaled_min) {
+                *stop = 1;  // minimal increase of lambda
+                break;
+            }
+            if (omega >= omega_max_stop) {
+                *stop = 2;  // maximal omega
+                break;
+            }
+            if ((lambda_hist[nh - 1] - lambda_hist[nh - 2]) < 0.9 * (lambda_hist[nh - 2] - lambda_hist[nh - 3]))
+                d_omega = 2 * d_omega;
+        }
+        if (n_omega >= n_omega_max) {
+            *stop = 3;  // impossible to increase omega
+            break;
+        }
+        if (step >= step_max) {
+            *stop = 4;  // maximal number of steps
+            break;
+        }
+    }
+    NCHECK(ssr_dev_lincomb(n_dof, 1.0, U, 0.0, U, U_out));
+    return ssr_synchronize();
 }

@@ -551,6 +551,34 @@ int ssr_solver_expand_deflation(ssr_solver *s, const double *v)
     return SSR_OK;
 }
 
+// DFGMRES.copy (DFGMRES.m:204-221): a new solver with src's settings and a
+// copy of its deflation basis; with share_precond the copy applies (and sets
+// up) src's preconditioner object, else it gets a node-block Jacobi of its
+// own, as a fresh solver does.
+int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src);
+
+int ssr_solver_clone(const ssr_solver *src, int share_precond, ssr_solver **out)
+{
+    SSR_REQUIRE(src && out, "solver_clone: null");
+    *out = nullptr;
+    ssr_solver *d = nullptr;
+    int rc = ssr_solver_create(src->P, src->maxit, src->tol, src->defl_tol, 0, &d);
+    if (rc) return rc;
+    d->precision = src->precision;
+    d->max_refine = src->max_refine;
+    d->symmetric = src->symmetric;
+    if (share_precond) {
+        d->Msp = src->Msp;
+        d->M = d->Msp.get();
+    }
+    if ((rc = ssr_solver_copy_deflation(d, src))) {
+        delete d;
+        return rc;
+    }
+    *out = d;
+    return SSR_OK;
+}
+
 // dst's basis := src's basis (DFGMRES.copy keeps the deflation basis).
 int ssr_solver_copy_deflation(ssr_solver *dst, const ssr_solver *src)
 {

@@ -43,6 +43,9 @@ _lib.extend_signatures({
     "ssr_newton": [_vp, _vp, _int, _vp, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "ssr_newton_ind_LL": [_vp, _vp, _int, _vp, _dbl, _dbl, _int, _int, _dbl, _dbl, _vp, _i64, _vp, _vp, _vp, _vp,
                           _vp, _vp, _vp],
+    "ssr_SSR_indirect_continuation": [_vp, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _int, _vp,
+                                      _vp, _int, _vp, _vp, _vp],
+    "ssr_solver_clone": [_vp, _int, _vp],
 })
 
 DRIVERS = ("python", "native")
