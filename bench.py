@@ -674,9 +674,16 @@ def run_batch(args):
           f"free DOFs", file=sys.stderr, flush=True)
     p = dict(problems.PARAMS_FAMILY, linear_tol=args.linear_tol or 1e-2)
     log = (lambda s: print(s, file=sys.stderr, flush=True))
-    seq, t_seq = problems.run_batch(probs, workers=1, params=p, precision=args.precision, log=log)
-    log(f"one after another: {t_seq:.1f} s")
-    par, t_par = problems.run_batch(probs, workers=args.workers, params=p, precision=args.precision, log=log)
+    drv = args.newton_driver
+    if args.no_sequential:
+        seq, t_seq = None, float("nan")
+    else:
+        seq, t_seq = problems.run_batch(probs, workers=1, params=p, precision=args.precision, log=log, driver=drv)
+        log(f"one after another: {t_seq:.1f} s")
+    par, t_par = problems.run_batch(probs, workers=args.workers, params=p, precision=args.precision, log=log,
+                                    driver=drv)
+    if seq is None:
+        seq = par
     ok = [i for i in range(len(probs)) if "error" not in seq[i] and "error" not in par[i]]
     dfos = max((abs(seq[i]["FoS"] - par[i]["FoS"]) for i in ok), default=float("nan"))
     out = {
@@ -688,7 +695,8 @@ def run_batch(args):
         "dtype": "f64" if args.precision == 64 else "f64 (Newton) / f32 (Krylov, AMG)",
         "data": "synthetic",
         "config": {"workload": f"{args.batch} 2D slope-family instances, {sizes}",
-                   "free_dof": [int(pr["n_dof_free"]) for pr in probs], "workers": args.workers},
+                   "free_dof": [int(pr["n_dof_free"]) for pr in probs], "workers": args.workers,
+                   "driver": drv},
         "wall_sequential_s": t_seq, "wall_concurrent_s": t_par, "speedup": t_seq / t_par,
         "max_dFoS_concurrent_vs_sequential": dfos,
         "stopped_with_error": [i for i in range(len(probs)) if "error" in seq[i] or "error" in par[i]],
@@ -742,6 +750,7 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="instances of the batch workload")
     ap.add_argument("--h-range", default=None, help="batch: mesh sizes h ~ U[lo, hi] as 'lo,hi' (0.16,0.5: 20k-200k DOF)")
     ap.add_argument("--workers", type=int, default=8, help="host threads (streams) of the batch workload")
+    ap.add_argument("--no-sequential", action="store_true", help="batch: skip the one-after-another baseline")
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--train-cpm", default="0.5,0.6,0.7,0.8,0.9",
                     help="learned3d: cells per metre of the training meshes (the last one validates)")

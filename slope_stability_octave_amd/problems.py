@@ -209,14 +209,16 @@ def run_ssr_direct(prob, params=None, preconditioner="amg", log=None, amg_option
 
 
 def run_batch(problems, workers=8, params=None, preconditioner="amg", precision=64, predictor_factories=None,
-              log=None):
+              log=None, driver="native"):
     """Independent SSR instances (of any sizes) solved concurrently: each
     worker thread drives its instances on a HIP stream of its own, so the
     kernels of different instances overlap on the device (the library
     releases the Python lock inside every call).  Returns the per-instance
     results (in input order; an instance whose continuation stops with an
     error gives FoS nan and the message) and the wall time of the whole
-    batch."""
+    batch.  ``driver`` as for run_ssr ("library": each instance's whole
+    continuation is one library call, so a worker holds the Python lock only
+    between instances; not with predictors)."""
     from concurrent.futures import ThreadPoolExecutor
     from ._lib import ThreadStream
     import threading
@@ -233,7 +235,8 @@ def run_batch(problems, workers=8, params=None, preconditioner="amg", precision=
         pf = predictor_factories[i] if predictor_factories else None
         from .continuation import ContinuationError
         try:
-            res = run_ssr(problems[i], params, preconditioner=preconditioner, precision=precision, predictor=pf)
+            res = run_ssr(problems[i], params, preconditioner=preconditioner, precision=precision, predictor=pf,
+                          driver=driver if pf is None else "native")
         except ContinuationError as exc:  # the instance's own outcome, reported like the reference's error
             res = dict(FoS=float("nan"), error=str(exc))
         if log:
