@@ -209,6 +209,7 @@ TUNE_CHEB_PER_DOF = 13
 TUNE_SPMV_MIRROR = 14
 TUNE_CONSTITUTIVE_2D = 15
 TUNE_NEWTON_GRAPH = 16
+TUNE_KRYLOV_PREFETCH = 17
 
 
 def set_tuning(key: int, value: int):

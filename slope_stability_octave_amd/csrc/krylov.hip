@@ -528,7 +528,12 @@ __device__ __forceinline__ double wave_sum_n(double (&s)[N], int l)
 // combines the four wavefronts into this block's partial for each column.
 constexpr int kMdCols = 8, kMdBlocks = 8192;
 
-template <class T, int RPT>
+//
+// PF: all of a pass's loads (RPT rows of u and of the column group) are
+// issued before the first product, with clamped indices instead of guards,
+// and the products then added in the plain form's order (the same bits); the
+// plain form's guarded loads wait one after another.
+template <class T, int RPT, bool PF = false>
 __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int k, const T *__restrict__ V,
                                                                   int64_t ld, const T *__restrict__ u,
                                                                   double *__restrict__ part)
@@ -543,14 +548,39 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int
     for (int c = 0; c < kMdCols; ++c) s[c] = 0.0;
     const int64_t span = (int64_t)RPT * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * span; base < n; base += (int64_t)gridDim.x * span) {
+        if constexpr (PF) {
+            T uu[RPT], vv[RPT][kMdCols];
 #pragma unroll
-        for (int rr = 0; rr < RPT; ++rr) {
-            const int64_t i = base + (int64_t)rr * blockDim.x + threadIdx.x;
-            if (i < n) {
-                const double ui = (double)u[i];
+            for (int rr = 0; rr < RPT; ++rr) {
+                const int64_t i = base + (int64_t)rr * blockDim.x + threadIdx.x;
+                const int64_t ic = i < n ? i : n - 1;
+                uu[rr] = u[ic];
 #pragma unroll
-                for (int c = 0; c < kMdCols; ++c)
-                    if (c < nc) s[c] += (double)V[(int64_t)(c0 + c) * ld + i] * ui;
+                for (int c = 0; c < kMdCols; ++c) {
+                    const int cc = c < nc ? c0 + c : c0;
+                    vv[rr][c] = V[(int64_t)cc * ld + ic];
+                }
+            }
+#pragma unroll
+            for (int rr = 0; rr < RPT; ++rr) {
+                const int64_t i = base + (int64_t)rr * blockDim.x + threadIdx.x;
+                if (i < n) {
+                    const double ui = (double)uu[rr];
+#pragma unroll
+                    for (int c = 0; c < kMdCols; ++c)
+                        if (c < nc) s[c] += (double)vv[rr][c] * ui;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int rr = 0; rr < RPT; ++rr) {
+                const int64_t i = base + (int64_t)rr * blockDim.x + threadIdx.x;
+                if (i < n) {
+                    const double ui = (double)u[i];
+#pragma unroll
+                    for (int c = 0; c < kMdCols; ++c)
+                        if (c < nc) s[c] += (double)V[(int64_t)(c0 + c) * ld + i] * ui;
+                }
             }
         }
     }
@@ -566,7 +596,9 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int
 }
 
 // u = alpha*u + beta * V(:,0:k) h   (h on device, k <= 128)
-template <class TV, class TU>
+// PF: the columns are loaded eight at a time (clamped to the last column)
+// before their products are added, in the plain form's order (the same bits).
+template <class TV, class TU, bool PF = false>
 __global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV *__restrict__ V, int64_t ld,
                                                      const double *__restrict__ h, double alpha, double beta,
                                                      TU *__restrict__ u)
@@ -577,7 +609,18 @@ __global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int c = 0; c < k; ++c) s += (double)V[(int64_t)c * ld + i] * sh[c];
+    if constexpr (PF) {
+        for (int c = 0; c < k; c += 8) {
+            TV v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = V[(int64_t)(c + j < k ? c + j : k - 1) * ld + i];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (c + j < k) s += (double)v[j] * sh[c + j];
+        }
+    } else {
+        for (int c = 0; c < k; ++c) s += (double)V[(int64_t)c * ld + i] * sh[c];
+    }
     u[i] = (TU)((alpha == 0.0 ? 0.0 : alpha * (double)u[i]) + beta * s);
 }
 
@@ -1248,9 +1291,16 @@ int multidot_into(int64_t n, int k, const T *V, int64_t ld, const T *u, double *
     const int64_t nbl = (n + (int64_t)kRedThreads * rpt - 1) / ((int64_t)kRedThreads * rpt);
     const unsigned nb = (unsigned)(nbl < 1 ? 1 : (nbl > kMdBlocks ? kMdBlocks : nbl));
     const dim3 grid(nb, (unsigned)((k + kMdCols - 1) / kMdCols));
-    if (rpt == 4)
+    const bool pf = ssr_tuning(SSR_TUNE_KRYLOV_PREFETCH) != 0;
+    if (rpt == 4 && pf)
+        hipLaunchKernelGGL((k_multidot_partial<T, 4, true>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld,
+                           u, part);
+    else if (rpt == 4)
         hipLaunchKernelGGL((k_multidot_partial<T, 4>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
                            part);
+    else if (pf)
+        hipLaunchKernelGGL((k_multidot_partial<T, 2, true>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld,
+                           u, part);
     else
         hipLaunchKernelGGL((k_multidot_partial<T, 2>), grid, dim3(kRedThreads), 0, ssr_stream(), n, k, V, ld, u,
                            part);
@@ -1274,6 +1324,8 @@ template <class TV, class TU>
 int gemv_update(int64_t n, int k, const TV *V, int64_t ld, const double *h, double alpha, double beta, TU *u)
 {
     SSR_REQUIRE(k >= 0 && k <= 128, "gemv_update: k must be in [0,128]");
+    if (k > 0 && ssr_tuning(SSR_TUNE_KRYLOV_PREFETCH))
+        SSR_LAUNCH1((k_gemv_update<TV, TU, true>), n, n, k, V, ld, h, alpha, beta, u);
     SSR_LAUNCH1((k_gemv_update<TV, TU>), n, n, k, V, ld, h, alpha, beta, u);
 }
 

@@ -568,3 +568,36 @@ def test_vcycle_halving_sums_same_bits(prec, dim, key):
     else:
         assert out[0][1] == out[1][1]
         assert np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("k", [1, 5, 8, 13, 70])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_multidot_and_update_prefetch_same_bits(k, dtype):
+    """Tuning key 17: the multi-dot and the basis update with a pass's loads
+    issued before its products give the guarded form's results bit for bit
+    (long and short columns, partial column groups, both precisions)."""
+    import ctypes
+    from slope_stability_octave_amd.linear_solvers import call, ptr  # noqa: F401  (binds signatures)
+    f32 = dtype == np.float32
+    out = {}
+    for n in (300_007, 20_011):
+        rng = np.random.default_rng(k + n)
+        V = rng.normal(size=(k, n)).astype(dtype)
+        u = rng.normal(size=n).astype(dtype)
+        dV, du = DeviceArray.from_host(V), DeviceArray.from_host(u)
+        old = _lib.get_tuning(_lib.TUNE_KRYLOV_PREFETCH)
+        try:
+            for mode in (0, 1):
+                _lib.set_tuning(_lib.TUNE_KRYLOV_PREFETCH, mode)
+                dh = DeviceArray(k)
+                call("ssr_dev_multidot_f32" if f32 else "ssr_dev_multidot", n, k, ptr(dV), n, ptr(du), ptr(dh))
+                du2 = DeviceArray.from_host(u)
+                call("ssr_dev_gemv_update_f32" if f32 else "ssr_dev_gemv_update", n, k, ptr(dV), n, ptr(dh),
+                     ctypes.c_double(0.5), ctypes.c_double(-2.0), ptr(du2))
+                out[(n, mode)] = (dh.get(), du2.get())
+        finally:
+            _lib.set_tuning(_lib.TUNE_KRYLOV_PREFETCH, old)
+        assert np.array_equal(out[(n, 0)][0], out[(n, 1)][0])
+        assert np.array_equal(out[(n, 0)][1], out[(n, 1)][1])
+        ref = V.astype(np.float64) @ u.astype(np.float64)
+        assert np.abs(out[(n, 1)][0] - ref).max() <= (1e-4 if f32 else 1e-11) * np.abs(V).max() * np.abs(u).max() * np.sqrt(n)
