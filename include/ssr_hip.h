@@ -162,7 +162,13 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         and tangent, internal force of each build_F, and the K_r
  *         assembly -- from HIP graphs captured on their second use within
  *         one Newton solve (1, default), or launch them eagerly (0): the
- *         same bits. */
+ *         same bits;
+ *  key 17: the Krylov multi-dot V'u issuing each pass's loads (clamped
+ *         indices) before its products (1, default) or guarded loads in turn
+ *         (0): the same bits;
+ *  key 18: the float block-CSR products of the AMG transfers (3 x 6 and
+ *         6 x 3 blocks) loading a lane's next block with its current one (1,
+ *         default) or in turn (0): the same bits. */
 int ssr_set_tuning(int key, int value);
 int ssr_get_tuning(int key, int *value);
 

@@ -210,6 +210,7 @@ TUNE_SPMV_MIRROR = 14
 TUNE_CONSTITUTIVE_2D = 15
 TUNE_NEWTON_GRAPH = 16
 TUNE_KRYLOV_PREFETCH = 17
+TUNE_BRC_PREFETCH = 18
 
 
 def set_tuning(key: int, value: int):

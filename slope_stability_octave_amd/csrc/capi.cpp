@@ -316,7 +316,7 @@ extern "C" int ssr_assemble_K_tangent_vals(int dim, int64_t n_p, int64_t n_int, 
     return ssr_dtoh(V, dV.p, sizeof(double) * nnz_out);
 }
 
-static int g_tuning[SSR_TUNE_COUNT] = {11, 0, 0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 1, 1, 0, 1, 1, 1};
+static int g_tuning[SSR_TUNE_COUNT] = {11, 0, 0, 0, 1, 0, 0, 0, 0, 1, 1, 0, 1, 1, 0, 1, 1, 1, 1};
 
 int ssr_tuning(int key) { return (key >= 0 && key < SSR_TUNE_COUNT) ? g_tuning[key] : 0; }
 

@@ -343,7 +343,11 @@ __global__ __launch_bounds__(256) void k_spmv_bsr_w(int64_t n_rows, const int32_
 // 3D, 3 in 2D) and nodes: one column index per block instead of one per
 // entry, 8.1 instead of 12 bytes per FP64 value of a 6 x 6 block.  G lanes per
 // block row as in k_spmv_bsr.
-template <int RB, int CB, int G, class T, bool HALVE = true>
+// PF: a lane loads its next block (clamped to the row's last) together with
+// the current one and adds the two blocks' products one after the other, as
+// the plain loop does (the same bits); the plain loop's loads wait for the
+// previous block's.
+template <int RB, int CB, int G, class T, bool HALVE = true, bool PF = false>
 __global__ __launch_bounds__(256) void k_spmv_brc(int64_t n_rows, const int32_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
                                                   const T *__restrict__ x, T *y, double alpha, double beta,
@@ -357,16 +361,42 @@ __global__ __launch_bounds__(256) void k_spmv_brc(int64_t n_rows, const int32_t 
     for (int i = 0; i < RB; ++i) acc[i] = T(0);
     if (r < n_rows) {
         const int32_t b0 = row_ptr[r], b1 = row_ptr[r + 1];
-        for (int32_t b = b0 + l; b < b1; b += G) {
-            const int64_t c = col[b];
-            T xv[CB];
+        if constexpr (PF) {
+            for (int32_t b = b0 + l; b < b1; b += 2 * G) {
+                const int32_t bb = b + G < b1 ? b + G : b;
+                const int64_t c0 = col[b], c1 = col[bb];
+                T x0[CB], x1[CB], v0[RB * CB], v1[RB * CB];
 #pragma unroll
-            for (int j = 0; j < CB; ++j) xv[j] = x[c * CB + j];
-            const T *v = val + (int64_t)b * RB * CB;
+                for (int j = 0; j < CB; ++j) x0[j] = x[c0 * CB + j];
 #pragma unroll
-            for (int i = 0; i < RB; ++i)
+                for (int j = 0; j < CB; ++j) x1[j] = x[c1 * CB + j];
 #pragma unroll
-                for (int j = 0; j < CB; ++j) acc[i] += v[i * CB + j] * xv[j];
+                for (int e = 0; e < RB * CB; ++e) v0[e] = val[(int64_t)b * RB * CB + e];
+#pragma unroll
+                for (int e = 0; e < RB * CB; ++e) v1[e] = val[(int64_t)bb * RB * CB + e];
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) acc[i] += v0[i * CB + j] * x0[j];
+                if (b + G < b1) {
+#pragma unroll
+                    for (int i = 0; i < RB; ++i)
+#pragma unroll
+                        for (int j = 0; j < CB; ++j) acc[i] += v1[i * CB + j] * x1[j];
+                }
+            }
+        } else {
+            for (int32_t b = b0 + l; b < b1; b += G) {
+                const int64_t c = col[b];
+                T xv[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) xv[j] = x[c * CB + j];
+                const T *v = val + (int64_t)b * RB * CB;
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+#pragma unroll
+                    for (int j = 0; j < CB; ++j) acc[i] += v[i * CB + j] * xv[j];
+            }
         }
     }
     if (!HALVE) {
@@ -596,9 +626,7 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot_partial(int64_t n, int
 }
 
 // u = alpha*u + beta * V(:,0:k) h   (h on device, k <= 128)
-// PF: the columns are loaded eight at a time (clamped to the last column)
-// before their products are added, in the plain form's order (the same bits).
-template <class TV, class TU, bool PF = false>
+template <class TV, class TU>
 __global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV *__restrict__ V, int64_t ld,
                                                      const double *__restrict__ h, double alpha, double beta,
                                                      TU *__restrict__ u)
@@ -609,18 +637,7 @@ __global__ __launch_bounds__(256) void k_gemv_update(int64_t n, int k, const TV 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    if constexpr (PF) {
-        for (int c = 0; c < k; c += 8) {
-            TV v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = V[(int64_t)(c + j < k ? c + j : k - 1) * ld + i];
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (c + j < k) s += (double)v[j] * sh[c + j];
-        }
-    } else {
-        for (int c = 0; c < k; ++c) s += (double)V[(int64_t)c * ld + i] * sh[c];
-    }
+    for (int c = 0; c < k; ++c) s += (double)V[(int64_t)c * ld + i] * sh[c];
     u[i] = (TU)((alpha == 0.0 ? 0.0 : alpha * (double)u[i]) + beta * s);
 }
 
@@ -1223,10 +1240,18 @@ int spmv_brc(int rb, int cb, int64_t n_rows, const int32_t *rp, const int32_t *c
     while (G < 64 && n_rows * G < (int64_t)ssr_cu_count() * 2048 && avg > G / 2) G *= 2;
     const dim3 blk(256);
     const bool halve = ssr_tuning(SSR_TUNE_HALVING_SUMS) != 0;
+    // the prefetching form pays for the transfers' 3 x 6 and 6 x 3 blocks in
+    // float (51.9 -> 44.5 and 46.9 -> 39.2 us on the 1.5M slope) but not in
+    // double (94 -> 111 and 86 -> 101 us) nor for the 6 x 6 coarse operator
+    // (float 11.5 -> 16.0 us): the second block doubles the value registers
+    const bool pf = ssr_tuning(SSR_TUNE_BRC_PREFETCH) != 0 && rb != cb && sizeof(T) == 4;
 #define BRC_G(RB_, CB_, G_)                                                                                        \
     do {                                                                                                           \
         const dim3 grid(ssr_blocks(n_rows * G_, 256));                                                             \
-        if (halve)                                                                                                 \
+        if (halve && pf)                                                                                           \
+            hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, G_, T, true, true>), grid, blk, 0, ssr_stream(), n_rows, rp, col, \
+                               val, x, y, alpha, beta, z);                                                         \
+        else if (halve)                                                                                            \
             hipLaunchKernelGGL((k_spmv_brc<RB_, CB_, G_, T, true>), grid, blk, 0, ssr_stream(), n_rows, rp, col, val, \
                                x, y, alpha, beta, z);                                                              \
         else                                                                                                       \
@@ -1324,8 +1349,6 @@ template <class TV, class TU>
 int gemv_update(int64_t n, int k, const TV *V, int64_t ld, const double *h, double alpha, double beta, TU *u)
 {
     SSR_REQUIRE(k >= 0 && k <= 128, "gemv_update: k must be in [0,128]");
-    if (k > 0 && ssr_tuning(SSR_TUNE_KRYLOV_PREFETCH))
-        SSR_LAUNCH1((k_gemv_update<TV, TU, true>), n, n, k, V, ld, h, alpha, beta, u);
     SSR_LAUNCH1((k_gemv_update<TV, TU>), n, n, k, V, ld, h, alpha, beta, u);
 }
 

@@ -525,12 +525,13 @@ def test_float_multidot_and_basis_update(k):
 
 
 
-@pytest.mark.parametrize("key", ["TUNE_HALVING_SUMS", "TUNE_CHEB_PER_DOF"])
+@pytest.mark.parametrize("key", ["TUNE_HALVING_SUMS", "TUNE_CHEB_PER_DOF", "TUNE_BRC_PREFETCH"])
 @pytest.mark.parametrize("prec,dim", [(64, 3), (32, 3), (64, 2), (32, 2)])
 def test_vcycle_halving_sums_same_bits(prec, dim, key):
-    """Tuning keys 12 and 13 on the V-cycle: the block-CSR SpMVs of the coarse
-    levels, prolongators and restrictions with lane sums by halving exchanges
-    (12), and the finest-level Chebyshev block step with one thread per
+    """Tuning keys 12, 13 and 18 on the V-cycle: the block-CSR SpMVs of the
+    coarse levels, prolongators and restrictions with lane sums by halving
+    exchanges (12) and with each lane's next block loaded with its current
+    one (18), and the finest-level Chebyshev block step with one thread per
     unknown (13), give the other forms' results bit for bit (one V-cycle in
     double; a whole float solve, V-cycles in float, in precision 32)."""
     from slope_stability_octave_amd import _lib
@@ -573,9 +574,10 @@ def test_vcycle_halving_sums_same_bits(prec, dim, key):
 @pytest.mark.parametrize("k", [1, 5, 8, 13, 70])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_multidot_and_update_prefetch_same_bits(k, dtype):
-    """Tuning key 17: the multi-dot and the basis update with a pass's loads
-    issued before its products give the guarded form's results bit for bit
-    (long and short columns, partial column groups, both precisions)."""
+    """Tuning key 17: the multi-dot with a pass's loads issued before its
+    products gives the guarded form's results bit for bit (long and short
+    columns, partial column groups, both precisions); the update after it
+    as well."""
     import ctypes
     from slope_stability_octave_amd.linear_solvers import call, ptr  # noqa: F401  (binds signatures)
     f32 = dtype == np.float32
