@@ -164,8 +164,9 @@ int ssr_dev_potential(int dim, int64_t n_int, const double *E, const double *c_b
  *         one Newton solve (1, default), or launch them eagerly (0): the
  *         same bits;
  *  key 17: the Krylov multi-dot V'u issuing each pass's loads (clamped
- *         indices) before its products (1, default) or guarded loads in turn
- *         (0): the same bits;
+ *         indices) before its products, and the double update u += V h
+ *         reading h directly instead of through LDS (1, default), or the
+ *         guarded, staged forms (0): the same bits;
  *  key 18: the float block-CSR products of the AMG transfers (3 x 6 and
  *         6 x 3 blocks) loading a lane's next block with its current one (1,
  *         default) or in turn (0): the same bits. */

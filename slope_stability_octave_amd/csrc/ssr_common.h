@@ -102,8 +102,10 @@ struct ssr_precond {
 #define SSR_TUNE_NEWTON_GRAPH 16        /* 1: the Newton loops replay their element phases (strain, stress and
                                            tangent, internal force; K_r assembly) from captured HIP graphs
                                            (default), 0: eager launches; the same bits */
-#define SSR_TUNE_KRYLOV_PREFETCH 17     /* 1: the Krylov multi-dot issues a pass's loads before its products
-                                           (default), 0: guarded loads in turn; the same bits */
+#define SSR_TUNE_KRYLOV_PREFETCH 17     /* 1: the Krylov multi-dot issues a pass's loads before its products,
+                                           and the double basis update reads its coefficients directly
+                                           (default), 0: guarded loads in turn, coefficients through LDS;
+                                           the same bits */
 #define SSR_TUNE_BRC_PREFETCH 18        /* 1: the float block-CSR SpMV of the AMG transfers (rectangular blocks)
                                            loads a lane's next block with its current one (default), 0: in
                                            turn; the same bits */
