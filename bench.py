@@ -176,6 +176,74 @@ def run_constitutive2d(args):
     print(json.dumps(out))
 
 
+def run_spmv(args):
+    """The node-block SpMV of the time-to-FoS slope's operator (K_r of a
+    plastic state; 60x30x20 m slope at --cells-per-metre, 1.3 by default:
+    514,047 node rows, 14.4M 3x3 blocks) in double and in float (the float
+    copy of the FP32 Krylov form), timed with HIP events on the library
+    stream.  Algorithmic bytes per product: every block's values once, its
+    column index, x once and y once."""
+    from slope_stability_octave_amd import _lib, problems
+    from slope_stability_octave_amd import linear_solvers as LS
+    from slope_stability_octave_amd._lib import DeviceArray, EventTimer
+
+    _lib.require_gpu()
+    cpm = args.cells_per_metre if args.cells_per_metre != 1.0 else CONFIG3_CPM
+    counts = tuple(int(round(c * cpm)) for c in (20, 20, 20, 10, 20, 20))
+    prob = problems.slope_3d(counts, *GEOM_3D, problems.HOMO_2D, device=True)
+    cmb = problems.make_constitutive(prob)
+    cmb.reduction(1.0)
+    rng = np.random.default_rng(1)
+    U = DeviceArray.from_host(rng.normal(size=(cmb.n_n, 3)) * 1e-3 * cmb.Q.T)
+    cmb.build_F_and_DS_all(1.0, U)
+    K = cmb.build_K_r_QQ_vals(1e-3)
+    P = cmb.pattern
+    nv = P.nnzb * P.bs
+    K32 = DeviceArray(nv, dtype=np.float32)
+    _lib.call("ssr_dev_convert_f64_f32", nv, _lib.ptr(K), _lib.ptr(K32))
+    if args.copy_calibration:
+        # plain copy of the double operator (2 x 8 x nv bytes): the reference
+        # point for scaling FETCH_SIZE / WRITE_SIZE counter readings
+        dst = DeviceArray(nv)
+        for _ in range(2):
+            LS.axpby(1.0, K, 0.0, dst)
+        _lib.synchronize()
+        del dst
+    tim = EventTimer()
+    res = {}
+    for bits in (64, 32):
+        dt = np.float64 if bits == 64 else np.float32
+        Kb = K if bits == 64 else K32
+        x = DeviceArray.from_host(rng.normal(size=(P.n_n, 3)).astype(dt))
+        y = DeviceArray((P.n_n, 3), dtype=dt)
+        fn = "ssr_dev_spmv_bsr" if bits == 64 else "ssr_dev_spmv_bsr_f32"
+        a = (3, P.n_n, _lib.ptr(P.d_row_ptr), _lib.ptr(P.d_col), _lib.ptr(Kb), _lib.ptr(x), _lib.ptr(y), 1.0, 0.0)
+        for _ in range(args.warmup):
+            _lib.call(fn, *a)
+        _lib.synchronize()
+        tim.start()
+        for _ in range(args.steps):
+            _lib.call(fn, *a)
+        tim.stop()
+        ms = tim.elapsed_ms() / args.steps
+        w = 8 if bits == 64 else 4
+        nbytes = P.nnzb * (9 * w + 4) + (P.n_n + 1) * 4 + 2 * 3 * P.n_n * w
+        res[f"fp{bits}"] = {"ms": ms, "GBs": nbytes / ms / 1e6, "frac_hbm": nbytes / ms / 1e6 / HBM_PEAK_GBS,
+                            "bytes": nbytes}
+    out = {
+        "metric": "node-block SpMV throughput (time-to-FoS slope operator)",
+        "value": res["fp64"]["GBs"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["fp64"]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"SpMV of K_r, 3D slope cells {counts}: {P.n_n} node rows, {P.nnzb} 3x3 blocks"},
+        "roofline": {"bound": "hbm", "achieved": res["fp64"]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": res["fp64"]["frac_hbm"], "traffic": None},
+        "cpu_baseline": None,
+        "extra": res,
+    }
+    print(json.dumps(out))
+
+
 CONFIG3_CPM = 1.3  # cells per metre: 1,492,582 free unknowns (the ~1.5M-DOF configuration)
 GEOM_3D = (20.0, 20.0, 20.0, 10.0, 20.0, 20.0)
 
@@ -715,7 +783,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-time-to-fos", action="store_true",
                     help="skip the time-to-FoS leg (plain vs chosen form on the ~1.5M-DOF 3D slope)")
-    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "constitutive2d", "assembly", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
+    ap.add_argument("--workload", default="constitutive", choices=["constitutive", "constitutive2d", "spmv", "assembly", "fos2d", "fos3d", "fos3d_slope", "fos3d_hetero", "fos2d_family", "learned",
                              "learned3d", "batch", "ll2d"])
     ap.add_argument("--element-kernel", type=int, default=None, choices=[0, 1, 2],
                     help="element tangent kernel form (tuning key 8)")
@@ -799,6 +867,8 @@ def main():
         return run_assembly(args)
     if args.workload == "constitutive2d":
         return run_constitutive2d(args)
+    if args.workload == "spmv":
+        return run_spmv(args)
 
     from slope_stability_octave_amd import _lib
     from slope_stability_octave_amd import constitutive_mex as cm
